@@ -1,0 +1,198 @@
+/*
+ * mstcfd.h -- C ABI of the MI355X (gfx950) density-based explicit residual path of MST-CFD.
+ *
+ * The reference's solver "plugin" is duck-typed C++ (rhoSolver/RhoSolver.h:14-24): a class built
+ * from (MshBlock*, fstream*, AllData*) with setDT / solve / getOldValue / getNewValue /
+ * updateNewToOld, selected in Time::goNextTimeStep (time/Time.cpp:57-61).  This header is the
+ * C boundary that a GPU RhoSolver binds; every entry point names the reference interface it
+ * replaces.  Plain pointers and sizes only.  Conventions:
+ *   - return 0 on success, a negative MST_E_* code on error; mst_last_error() gives the text;
+ *   - host arrays are owned by the caller; state arrays use the reference's AllData layout
+ *     (cell-major, nv = dim+2 doubles per cell: rho, rho*u_1..rho*u_dim, rho*E) in FILE cell order;
+ *   - a context owns its device buffers and one HIP stream; calls are stream-ordered, and the
+ *     mst_get_* calls synchronise; one context per host thread (no internal locking).
+ */
+#ifndef MSTCFD_H
+#define MSTCFD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MST_OK 0
+#define MST_E_ARG (-1)      /* invalid argument / size mismatch */
+#define MST_E_IO (-2)       /* file cannot be opened or parsed */
+#define MST_E_HIP (-3)      /* HIP runtime error (no device, launch failure, out of memory) */
+#define MST_E_MESH (-4)     /* mesh inconsistent (bad cell ids, open cell, unsupported element) */
+#define MST_E_STATE (-5)    /* non-physical state detected (rho<=0, p<=0 or NaN) */
+#define MST_E_UNSUPPORTED (-6)
+
+/* Fluent boundary types as the reference's solver switches on them (rhoSolver/RhoSolver.cpp:97-230);
+ * PERIODIC (12) is new: the ghost is the partner face's cell. */
+#define MST_BC_INTERIOR 2
+#define MST_BC_WALL 3
+#define MST_BC_OUTLET 5
+#define MST_BC_SYMMETRY 7
+#define MST_BC_INLET 10
+#define MST_BC_PERIODIC 12
+
+#define MST_SCHEME_ROE 0   /* RHOSOLVER solverRoe   (rhoSolver/SolverRoe.h:9) */
+#define MST_SCHEME_AUSM 1  /* RHOSOLVER SolverAusm  (rhoSolver/SolverAusm.h:9) */
+#define MST_LIM_NONE 0     /* the reference's unlimited ACCURACY 2 reconstruction */
+#define MST_LIM_MINMOD 1
+#define MST_LIM_VANLEER 2
+
+/* reference behaviours behind switches (defaults: MST_COMPAT_REFERENCE) */
+#define MST_COMPAT_BRANCH2_LR 1      /* literal inverted L/R of mesh/MshBlock.cpp:294-303 */
+#define MST_COMPAT_GRAD_LASTINT 2    /* rhoSolver/RhoSolver.cpp:437-440 loop start N_int-1 */
+#define MST_COMPAT_AUSM_LOCAL_A 4    /* rhoSolver/SolverAusm.cpp:112-113 */
+#define MST_COMPAT_AUSM_P_UNSCALED 8 /* rhoSolver/SolverAusm.cpp:130,136 operator precedence */
+#define MST_COMPAT_REFERENCE (MST_COMPAT_GRAD_LASTINT | MST_COMPAT_AUSM_LOCAL_A | MST_COMPAT_AUSM_P_UNSCALED)
+
+/* Runtime replacement of the compile-time macros of include/CONST.h:3-83
+ * (RHOSOLVER :10, ACCURACY :6, GAMMA :41, EOR :42, inlet state :78-83). */
+typedef struct {
+    int32_t scheme;        /* MST_SCHEME_* */
+    int32_t order;         /* 1 (ACCURACY 1) or 2 (ACCURACY 2) */
+    int32_t limiter;       /* MST_LIM_* (order 2) */
+    int32_t rk;            /* 1 forward Euler (time/Time.cpp:62-65), 2 SSP-RK2, 3 SSP-RK3 */
+    int32_t compat;        /* MST_COMPAT_* */
+    int32_t reserved;      /* must be 0 */
+    double gamma;          /* 1.4 */
+    double eor;            /* 1e-10 */
+    double entropy_delta;  /* 0.125 (rhoSolver/SolverRoe.cpp:115) */
+    double cfl;            /* > 0: mst_step computes dt = cfl * min V / sum (|u_d|+a) S_d; 0: use mst_set_dt */
+    double q_inlet[5];     /* inlet ghost state (dim+2 used) */
+} mst_config;
+
+typedef struct mst_mesh mst_mesh; /* host mesh in file order (MshBlock, mesh/MshBlock.h:19-76) */
+typedef struct mst_ctx mst_ctx;   /* device solver context (RhoSolver + AllData device mirror) */
+
+/* ---------------------------------------------------------------- errors */
+/* last error text of the calling thread; returns the full length */
+size_t mst_last_error(char *buf, size_t n);
+
+/* ---------------------------------------------------------------- host mesh (integer work, bit-exact) */
+/* MshBlock::readMsh (mesh/MshBlock.cpp:5-18, 2D :75-432, 3D :436-789): Fluent ASCII .msh */
+int mst_mesh_read_msh(const char *path, mst_mesh **out);
+/* inverse of the reader (Fluent ASCII, lowercase hex, the reference's zone layout) */
+int mst_mesh_write_msh(const mst_mesh *m, const char *path);
+/* mesh from raw arrays: faces in file order; c1[f] = -1 on boundary faces; ftype per face;
+ * partner[f] = periodic partner face or -1 (may be NULL); fnn nodes per face */
+int mst_mesh_from_arrays(int dim, int64_t n_nodes, const double *xyz, int64_t n_cells, int64_t n_faces, int fnn,
+                         const int32_t *fnodes, const int32_t *c0, const int32_t *c1, const int32_t *ftype,
+                         const int32_t *partner, mst_mesh **out);
+/* out[0..6] = dim, fnn, n_nodes, n_cells, n_faces, n_int (end of the interior zone, MshBlock.cpp:223-226),
+ * n_zones */
+int mst_mesh_counts(const mst_mesh *m, int64_t out[7]);
+int mst_mesh_arrays(const mst_mesh *m, double *xyz, int32_t *fnodes, int32_t *c0, int32_t *c1, int32_t *ftype,
+                    int32_t *partner);
+/* zone table (FacesInf, mesh/FacesInf.h): n_zones x {zone id, first face, last face + 1, type} */
+int mst_mesh_zones(const mst_mesh *m, int32_t *out);
+void mst_mesh_destroy(mst_mesh *m);
+
+/* Generators for the synthetic workloads (no reference counterpart; the reference only reads files).
+ * 2D: nx x ny quads on [x0,x1]x[y0,y1], each split into 2 triangles, interior nodes jittered by
+ * +-jitter*h (boundary nodes tangentially), seeded mt19937_64.  bc[4] = type of the x0, x1, y0, y1
+ * sides (MST_BC_*; PERIODIC must be given on both sides of a pair).  Every face's area vector points
+ * into its c0 cell. */
+int mst_mesh_gen_tri(int nx, int ny, double x0, double x1, double y0, double y1, double jitter, uint64_t seed,
+                     const int32_t bc[4], mst_mesh **out);
+/* 3D: nx x ny x nz hexes on the box lo..hi, each split into 6 tets (Kuhn split); bc[6] = x0,x1,y0,y1,z0,z1 */
+int mst_mesh_gen_tet(int nx, int ny, int nz, const double lo[3], const double hi[3], double jitter, uint64_t seed,
+                     const int32_t bc[6], mst_mesh **out);
+
+/* ---------------------------------------------------------------- host preprocessing (no device needed)
+ * The one-time part of the reference's mesh set-up (mesh/MshBlock.cpp:272-334, mesh/Face.cpp:8-69,
+ * mesh/Cell.cpp:6-61) plus the device layout: geometry in FP64, reverse Cuthill-McKee cell order,
+ * faces sorted by kind then lowest cell, greedy face colouring, slot-major cell->face table. */
+typedef struct mst_prep mst_prep;
+int mst_prep_create(const mst_mesh *m, mst_prep **out);
+void mst_prep_destroy(mst_prep *p);
+/* out[0..7] = dim, nv, n_cells, n_faces, max faces per cell, colours, interior faces, periodic faces */
+int mst_prep_counts(const mst_prep *p, int64_t out[8]);
+/* new index of each file-order cell / face, and the colour of each file-order face */
+int mst_prep_permutation(const mst_prep *p, int32_t *cell_new_of_old, int32_t *face_new_of_old,
+                         int32_t *face_colour);
+/* geometry in file order, as the reference stores it on Face/Cell; any pointer may be NULL */
+int mst_prep_geometry(const mst_prep *p, double *fcen, double *S, double *eta, int32_t *dac, double *ccen,
+                      double *vol);
+/* device-order integer tables: fc0, fc1 (n_faces), kind (n_faces), slot-major cell->face table
+ * (maxf*n_cells, (face<<1)|is_c1, -1 padding); any pointer may be NULL */
+int mst_prep_tables(const mst_prep *p, int32_t *fc0, int32_t *fc1, int32_t *kind, int32_t *cell_faces);
+
+/* ---------------------------------------------------------------- solver context
+ * RhoSolver::RhoSolver (rhoSolver/RhoSolver.cpp:3-26) + AllData::createAllData (data/AllData.cpp:3-28):
+ * preprocesses the mesh (as mst_prep_create), uploads the SoA arrays and allocates the state
+ * buffers on the current HIP device.  Fails with MST_E_HIP when no device is present. */
+int mst_ctx_create(const mst_mesh *m, const mst_config *cfg, mst_ctx **out);
+int mst_ctx_destroy(mst_ctx *ctx);
+/* replace the configuration (same mesh); rebuilds the per-face L/R masks */
+int mst_ctx_set_config(mst_ctx *ctx, const mst_config *cfg);
+/* the context's preprocessing (owned by the context) */
+const mst_prep *mst_ctx_prep(const mst_ctx *ctx);
+
+/* ---------------------------------------------------------------- state (AllData p1Old/p1NewCellQs) */
+/* write p1OldCellQs (data/AllData.h getP1OldCellQs); Q: n_cells*nv doubles, file order */
+int mst_set_state(mst_ctx *ctx, const double *Q, int64_t n_cells);
+/* RhoSolver::getOldValue (rhoSolver/RhoSolver.cpp:504-506) */
+int mst_get_state(mst_ctx *ctx, double *Q, int64_t n_cells);
+/* RhoSolver::getNewValue (rhoSolver/RhoSolver.cpp:507-509) */
+int mst_get_new_state(mst_ctx *ctx, double *Q, int64_t n_cells);
+
+/* ---------------------------------------------------------------- hot path */
+/* RhoSolver::setDT (rhoSolver/RhoSolver.cpp:33-35) */
+int mst_set_dt(mst_ctx *ctx, double dt);
+/* RhoSolver::updateFaceFlux / updateFaceFlux2ndOrder (+ updateGradFlux) (rhoSolver/RhoSolver.cpp:90-369,
+ * 430-454) on the Old state: per face G_f = sum_d n_out,d(c0) F_f[:,d] (device-resident) */
+int mst_face_flux(mst_ctx *ctx);
+/* the gather + forward-Euler update of RhoSolver::solve (rhoSolver/RhoSolver.cpp:45-68): New = Old - DT/V R */
+int mst_residual_update(mst_ctx *ctx);
+/* RhoSolver::solve (rhoSolver/RhoSolver.cpp:37-89) = mst_face_flux + mst_residual_update */
+int mst_solve(mst_ctx *ctx);
+/* RhoSolver::updateNewToOld (rhoSolver/RhoSolver.cpp:513-517); a pointer swap on the device */
+int mst_update_new_to_old(mst_ctx *ctx);
+/* Time::goNextTimeStep (time/Time.cpp:54-80) generalised: nsteps steps of cfg->rk stages, dt from cfl
+ * (device-side reduction) when cfg->cfl > 0 else the last mst_set_dt value.  Old holds the result. */
+int mst_step(mst_ctx *ctx, int nsteps);
+/* NEW: dt = cfl * min_c V_c / sum_d (|u_d| + a) Shalf_c,d on the Old state */
+int mst_compute_dt(mst_ctx *ctx, double cfl, double *dt);
+/* NEW: L2 norm over cells of R(Q_old)/V per field (nv values) from the first stage of the last step */
+int mst_residual_norms(mst_ctx *ctx, double *out);
+/* Time::goNextTimeStep monitor (time/Time.cpp:69-76): max_i |New-Old|/Old per field */
+int mst_monitor_maxrel(mst_ctx *ctx, double *out);
+
+/* ---------------------------------------------------------------- inspection (tests) */
+/* residual R_i = sum_f sum_d n_out,d F_f[:,d] (rhoSolver/RhoSolver.cpp:51-55) of the Old state, file order */
+int mst_get_residual(mst_ctx *ctx, double *R, int64_t n_cells);
+/* per-face G_f of the last mst_face_flux, file order, n_faces*nv */
+int mst_get_face_flux(mst_ctx *ctx, double *G, int64_t n_faces);
+/* gradient (n_cells*nv*dim) and limiter factor (n_cells*nv) of the Old state, file order */
+int mst_get_gradient(mst_ctx *ctx, double *grad, double *phi, int64_t n_cells);
+
+/* flux-scheme plugin, batched on the device: RHOSOLVER::set(L,R) + solverAll(dir[i])
+ * (rhoSolver/SolverRoe.cpp:3-16,70-112; rhoSolver/SolverAusm.cpp:3-26,53-63,105-143).
+ * QL, QR: n*(dim+2); dir: n; F: n*(dim+2); host arrays. */
+int mst_riemann_flux(const mst_config *cfg, int dim, int64_t n, const double *QL, const double *QR,
+                     const int32_t *dir, double *F);
+
+/* ---------------------------------------------------------------- timing (HIP events on the ctx stream) */
+#define MST_K_GRAD 0
+#define MST_K_FLUX 1
+#define MST_K_UPDATE 2
+#define MST_K_DT 3
+#define MST_K_COUNT 4
+int mst_timing_enable(mst_ctx *ctx, int on);
+/* per kernel class: launches timed and total milliseconds since the last reset */
+int mst_timing_get(mst_ctx *ctx, int kernel, int64_t *count, double *total_ms);
+int mst_timing_reset(mst_ctx *ctx);
+int mst_sync(mst_ctx *ctx);
+/* algorithmic (compulsory) HBM bytes of one launch of each kernel class for this mesh/config */
+int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,380 @@
+"""Python binding (ctypes) of libmstcfd.so, the MI355X residual path of MST-CFD.
+
+The library is the product; this module only marshals numpy arrays across its C ABI
+(include/mstcfd.h).  There is no Python or CPU fallback for any compute entry point: if the
+library is missing the import-time loader raises, and every solver call runs the HIP kernels.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libmstcfd.so")
+
+I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+F64P = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+BC_INTERIOR, BC_WALL, BC_OUTLET, BC_SYMMETRY, BC_INLET, BC_PERIODIC = 2, 3, 5, 7, 10, 12
+ROE, AUSM = 0, 1
+LIM_NONE, LIM_MINMOD, LIM_VANLEER = 0, 1, 2
+COMPAT_BRANCH2_LR, COMPAT_GRAD_LASTINT, COMPAT_AUSM_LOCAL_A, COMPAT_AUSM_P_UNSCALED = 1, 2, 4, 8
+COMPAT_REFERENCE = COMPAT_GRAD_LASTINT | COMPAT_AUSM_LOCAL_A | COMPAT_AUSM_P_UNSCALED
+K_GRAD, K_FLUX, K_UPDATE, K_DT = 0, 1, 2, 3
+KERNEL_NAMES = {K_GRAD: "k_grad", K_FLUX: "k_flux", K_UPDATE: "k_update", K_DT: "k_cfl"}
+
+# public exported symbols of include/mstcfd.h (checked by the CPU tests)
+EXPORTS = [
+    "mst_last_error", "mst_mesh_read_msh", "mst_mesh_write_msh", "mst_mesh_from_arrays", "mst_mesh_counts",
+    "mst_mesh_arrays", "mst_mesh_zones", "mst_mesh_destroy", "mst_mesh_gen_tri", "mst_mesh_gen_tet",
+    "mst_prep_create", "mst_prep_destroy", "mst_prep_counts", "mst_prep_permutation", "mst_prep_geometry",
+    "mst_prep_tables", "mst_ctx_create", "mst_ctx_destroy", "mst_ctx_set_config", "mst_ctx_prep", "mst_set_state",
+    "mst_get_state", "mst_get_new_state", "mst_set_dt", "mst_face_flux", "mst_residual_update", "mst_solve",
+    "mst_update_new_to_old", "mst_step", "mst_compute_dt", "mst_residual_norms", "mst_monitor_maxrel",
+    "mst_get_residual", "mst_get_face_flux", "mst_get_gradient", "mst_riemann_flux", "mst_timing_enable",
+    "mst_timing_get", "mst_timing_reset", "mst_sync", "mst_algorithmic_bytes",
+]
+
+
+class MstError(RuntimeError):
+    pass
+
+
+class Config(C.Structure):
+    """mst_config (include/mstcfd.h)."""
+
+    _fields_ = [
+        ("scheme", C.c_int32),
+        ("order", C.c_int32),
+        ("limiter", C.c_int32),
+        ("rk", C.c_int32),
+        ("compat", C.c_int32),
+        ("reserved", C.c_int32),
+        ("gamma", C.c_double),
+        ("eor", C.c_double),
+        ("entropy_delta", C.c_double),
+        ("cfl", C.c_double),
+        ("q_inlet", C.c_double * 5),
+    ]
+
+
+def reference_inlet(dim=2):
+    """Inlet state of include/CONST.h:70-83 (rho=1, u=0, E = rho*(T*CV + |u|^2/2), T = 1/286.32, CV = 715.8)."""
+    E = 1 * (1 / 286.32 * 715.8 + 0.5 * (0 * 0 + 0 * 0))
+    return [1.0] + [0.0] * dim + [E]
+
+
+def make_config(scheme=ROE, order=1, limiter=LIM_NONE, rk=1, compat=COMPAT_REFERENCE, gamma=1.4, eor=1e-10,
+                entropy_delta=0.125, cfl=0.0, q_inlet=None, dim=2):
+    c = Config()
+    c.scheme, c.order, c.limiter, c.rk, c.compat, c.reserved = scheme, order, limiter, rk, compat, 0
+    c.gamma, c.eor, c.entropy_delta, c.cfl = gamma, eor, entropy_delta, cfl
+    q = list(reference_inlet(dim) if q_inlet is None else q_inlet)
+    q = q + [0.0] * (5 - len(q))
+    for i in range(5):
+        c.q_inlet[i] = float(q[i])
+    return c
+
+
+_LIB = None
+
+
+def lib():
+    """Load libmstcfd.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise MstError(f"libmstcfd.so not built at {LIB_PATH}; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, pp = C.c_void_p, C.POINTER(C.c_void_p)
+    sig = {
+        "mst_last_error": (C.c_size_t, [C.c_char_p, C.c_size_t]),
+        "mst_mesh_read_msh": (C.c_int, [C.c_char_p, pp]),
+        "mst_mesh_write_msh": (C.c_int, [vp, C.c_char_p]),
+        "mst_mesh_from_arrays": (C.c_int, [C.c_int, C.c_int64, F64P, C.c_int64, C.c_int64, C.c_int, I32P, I32P, I32P,
+                                           I32P, I32P, pp]),
+        "mst_mesh_counts": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "mst_mesh_arrays": (C.c_int, [vp, F64P, I32P, I32P, I32P, I32P, I32P]),
+        "mst_mesh_zones": (C.c_int, [vp, I32P]),
+        "mst_mesh_destroy": (None, [vp]),
+        "mst_mesh_gen_tri": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_uint64, I32P, pp]),
+        "mst_mesh_gen_tet": (C.c_int, [C.c_int, C.c_int, C.c_int, F64P, F64P, C.c_double, C.c_uint64, I32P, pp]),
+        "mst_prep_create": (C.c_int, [vp, pp]),
+        "mst_prep_destroy": (None, [vp]),
+        "mst_prep_counts": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "mst_prep_permutation": (C.c_int, [vp, I32P, I32P, I32P]),
+        "mst_prep_geometry": (C.c_int, [vp, F64P, F64P, F64P, I32P, F64P, F64P]),
+        "mst_prep_tables": (C.c_int, [vp, I32P, I32P, I32P, I32P]),
+        "mst_ctx_create": (C.c_int, [vp, C.POINTER(Config), pp]),
+        "mst_ctx_destroy": (C.c_int, [vp]),
+        "mst_ctx_set_config": (C.c_int, [vp, C.POINTER(Config)]),
+        "mst_ctx_prep": (vp, [vp]),
+        "mst_set_state": (C.c_int, [vp, F64P, C.c_int64]),
+        "mst_get_state": (C.c_int, [vp, F64P, C.c_int64]),
+        "mst_get_new_state": (C.c_int, [vp, F64P, C.c_int64]),
+        "mst_set_dt": (C.c_int, [vp, C.c_double]),
+        "mst_face_flux": (C.c_int, [vp]),
+        "mst_residual_update": (C.c_int, [vp]),
+        "mst_solve": (C.c_int, [vp]),
+        "mst_update_new_to_old": (C.c_int, [vp]),
+        "mst_step": (C.c_int, [vp, C.c_int]),
+        "mst_compute_dt": (C.c_int, [vp, C.c_double, C.POINTER(C.c_double)]),
+        "mst_residual_norms": (C.c_int, [vp, F64P]),
+        "mst_monitor_maxrel": (C.c_int, [vp, F64P]),
+        "mst_get_residual": (C.c_int, [vp, F64P, C.c_int64]),
+        "mst_get_face_flux": (C.c_int, [vp, F64P, C.c_int64]),
+        "mst_get_gradient": (C.c_int, [vp, F64P, F64P, C.c_int64]),
+        "mst_riemann_flux": (C.c_int, [C.POINTER(Config), C.c_int, C.c_int64, F64P, F64P, I32P, F64P]),
+        "mst_timing_enable": (C.c_int, [vp, C.c_int]),
+        "mst_timing_get": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+        "mst_timing_reset": (C.c_int, [vp]),
+        "mst_sync": (C.c_int, [vp]),
+        "mst_algorithmic_bytes": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = L
+    return L
+
+
+def last_error():
+    buf = C.create_string_buffer(1024)
+    lib().mst_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise MstError(f"{what} failed ({rc}): {last_error()}")
+
+
+class Mesh:
+    """Host mesh in file order (the reference's MshBlock)."""
+
+    def __init__(self, handle):
+        self.h = handle
+        c = (C.c_int64 * 7)()
+        _check(lib().mst_mesh_counts(self.h, c), "mst_mesh_counts")
+        self.dim, self.fnn, self.n_nodes, self.n_cells, self.n_faces, self.n_int, self.n_zones = [int(v) for v in c]
+        self.nv = self.dim + 2
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mst_mesh_destroy(self.h)
+            self.h = None
+
+    @staticmethod
+    def _new(fn, *args):
+        h = C.c_void_p()
+        _check(fn(*args, C.byref(h)), fn.__name__)
+        return Mesh(h)
+
+    @classmethod
+    def gen_tri(cls, nx, ny, x0, x1, y0, y1, jitter=0.0, seed=12345, bc=(BC_WALL,) * 4):
+        return cls._new(lib().mst_mesh_gen_tri, int(nx), int(ny), float(x0), float(x1), float(y0), float(y1),
+                        float(jitter), int(seed), np.asarray(bc, np.int32))
+
+    @classmethod
+    def gen_tet(cls, nx, ny, nz, lo=(0, 0, 0), hi=(1, 1, 1), jitter=0.0, seed=12345, bc=(BC_WALL,) * 6):
+        return cls._new(lib().mst_mesh_gen_tet, int(nx), int(ny), int(nz), np.asarray(lo, np.float64),
+                        np.asarray(hi, np.float64), float(jitter), int(seed), np.asarray(bc, np.int32))
+
+    @classmethod
+    def read_msh(cls, path):
+        return cls._new(lib().mst_mesh_read_msh, str(path).encode())
+
+    @classmethod
+    def from_raw(cls, raw):
+        xyz = np.ascontiguousarray(raw["xyz"], np.float64).reshape(-1)
+        fn = np.ascontiguousarray(raw["fnodes"], np.int32)
+        dim = int(raw["dim"])
+        return cls._new(lib().mst_mesh_from_arrays, dim, xyz.size // dim, xyz, int(raw["n_cells"]),
+                        int(fn.shape[0]), int(fn.shape[1]), fn.reshape(-1), np.ascontiguousarray(raw["c0"], np.int32),
+                        np.ascontiguousarray(raw["c1"], np.int32), np.ascontiguousarray(raw["ftype"], np.int32),
+                        np.ascontiguousarray(raw["partner"], np.int32))
+
+    def write_msh(self, path):
+        _check(lib().mst_mesh_write_msh(self.h, str(path).encode()), "mst_mesh_write_msh")
+
+    def raw(self):
+        """File-order arrays (the oracle's input)."""
+        xyz = np.zeros((self.n_nodes, self.dim))
+        fnodes = np.zeros((self.n_faces, self.fnn), np.int32)
+        c0 = np.zeros(self.n_faces, np.int32)
+        c1 = np.zeros(self.n_faces, np.int32)
+        ft = np.zeros(self.n_faces, np.int32)
+        pa = np.zeros(self.n_faces, np.int32)
+        _check(lib().mst_mesh_arrays(self.h, xyz.reshape(-1), fnodes.reshape(-1), c0, c1, ft, pa), "mst_mesh_arrays")
+        return dict(dim=self.dim, xyz=xyz, fnodes=fnodes, c0=c0, c1=c1, ftype=ft, partner=pa, n_cells=self.n_cells,
+                    n_int=self.n_int)
+
+    def zones(self):
+        z = np.zeros((max(self.n_zones, 1), 4), np.int32)
+        _check(lib().mst_mesh_zones(self.h, z.reshape(-1)), "mst_mesh_zones")
+        return z[: self.n_zones]
+
+
+class Prep:
+    """Host preprocessing (geometry, RCM order, face order, colouring) without a device."""
+
+    def __init__(self, mesh=None, handle=None, owner=None):
+        if handle is None:
+            h = C.c_void_p()
+            _check(lib().mst_prep_create(mesh.h, C.byref(h)), "mst_prep_create")
+            self.h, self._own = h, True
+        else:
+            self.h, self._own, self._owner = handle, False, owner
+        c = (C.c_int64 * 8)()
+        _check(lib().mst_prep_counts(self.h, c), "mst_prep_counts")
+        (self.dim, self.nv, self.n_cells, self.n_faces, self.maxf, self.ncolours, self.n_interior,
+         self.n_periodic) = [int(v) for v in c]
+
+    def __del__(self):
+        if getattr(self, "_own", False) and getattr(self, "h", None):
+            lib().mst_prep_destroy(self.h)
+            self.h = None
+
+    def permutation(self):
+        cn = np.zeros(self.n_cells, np.int32)
+        fnw = np.zeros(self.n_faces, np.int32)
+        col = np.zeros(self.n_faces, np.int32)
+        _check(lib().mst_prep_permutation(self.h, cn, fnw, col), "mst_prep_permutation")
+        return cn, fnw, col
+
+    def geometry(self):
+        d, nf, nc = self.dim, self.n_faces, self.n_cells
+        out = dict(fcen=np.zeros((nf, d)), S=np.zeros((nf, d)), eta=np.zeros(nf), dac=np.zeros(nf, np.int32),
+                   ccen=np.zeros((nc, d)), vol=np.zeros(nc))
+        _check(lib().mst_prep_geometry(self.h, out["fcen"].reshape(-1), out["S"].reshape(-1), out["eta"], out["dac"],
+                                       out["ccen"].reshape(-1), out["vol"]), "mst_prep_geometry")
+        return out
+
+    def tables(self):
+        fc0 = np.zeros(self.n_faces, np.int32)
+        fc1 = np.zeros(self.n_faces, np.int32)
+        kind = np.zeros(self.n_faces, np.int32)
+        ell = np.zeros(self.maxf * self.n_cells, np.int32)
+        _check(lib().mst_prep_tables(self.h, fc0, fc1, kind, ell), "mst_prep_tables")
+        return fc0, fc1, kind, ell.reshape(self.maxf, self.n_cells)
+
+
+class Solver:
+    """Device context: the GPU RhoSolver (rhoSolver/RhoSolver.h:14-24) over the C ABI."""
+
+    def __init__(self, mesh, cfg):
+        h = C.c_void_p()
+        _check(lib().mst_ctx_create(mesh.h, C.byref(cfg), C.byref(h)), "mst_ctx_create")
+        self.h = h
+        self.cfg = cfg
+        self.mesh = mesh
+        self.prep = Prep(handle=lib().mst_ctx_prep(self.h), owner=self)
+        self.n_cells, self.n_faces, self.nv, self.dim = self.prep.n_cells, self.prep.n_faces, self.prep.nv, self.prep.dim
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mst_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_config(self, cfg):
+        _check(lib().mst_ctx_set_config(self.h, C.byref(cfg)), "mst_ctx_set_config")
+        self.cfg = cfg
+
+    def set_state(self, Q):
+        Q = np.ascontiguousarray(Q, np.float64).reshape(self.n_cells, self.nv)
+        _check(lib().mst_set_state(self.h, Q.reshape(-1), self.n_cells), "mst_set_state")
+
+    def get_state(self):
+        Q = np.zeros((self.n_cells, self.nv))
+        _check(lib().mst_get_state(self.h, Q.reshape(-1), self.n_cells), "mst_get_state")
+        return Q
+
+    def get_new_state(self):
+        Q = np.zeros((self.n_cells, self.nv))
+        _check(lib().mst_get_new_state(self.h, Q.reshape(-1), self.n_cells), "mst_get_new_state")
+        return Q
+
+    def set_dt(self, dt):
+        _check(lib().mst_set_dt(self.h, float(dt)), "mst_set_dt")
+
+    def face_flux(self):
+        _check(lib().mst_face_flux(self.h), "mst_face_flux")
+
+    def residual_update(self):
+        _check(lib().mst_residual_update(self.h), "mst_residual_update")
+
+    def solve(self):
+        _check(lib().mst_solve(self.h), "mst_solve")
+
+    def update_new_to_old(self):
+        _check(lib().mst_update_new_to_old(self.h), "mst_update_new_to_old")
+
+    def step(self, n=1):
+        _check(lib().mst_step(self.h, int(n)), "mst_step")
+
+    def compute_dt(self, cfl):
+        dt = C.c_double()
+        _check(lib().mst_compute_dt(self.h, float(cfl), C.byref(dt)), "mst_compute_dt")
+        return dt.value
+
+    def residual_norms(self):
+        out = np.zeros(self.nv)
+        _check(lib().mst_residual_norms(self.h, out), "mst_residual_norms")
+        return out
+
+    def monitor_maxrel(self):
+        out = np.zeros(self.nv)
+        _check(lib().mst_monitor_maxrel(self.h, out), "mst_monitor_maxrel")
+        return out
+
+    def residual(self):
+        R = np.zeros((self.n_cells, self.nv))
+        _check(lib().mst_get_residual(self.h, R.reshape(-1), self.n_cells), "mst_get_residual")
+        return R
+
+    def face_flux_values(self):
+        G = np.zeros((self.n_faces, self.nv))
+        _check(lib().mst_get_face_flux(self.h, G.reshape(-1), self.n_faces), "mst_get_face_flux")
+        return G
+
+    def gradient(self):
+        g = np.zeros((self.n_cells, self.nv, self.dim))
+        phi = np.zeros((self.n_cells, self.nv))
+        _check(lib().mst_get_gradient(self.h, g.reshape(-1), phi.reshape(-1), self.n_cells), "mst_get_gradient")
+        return g, phi
+
+    def timing(self, on=True):
+        _check(lib().mst_timing_enable(self.h, 1 if on else 0), "mst_timing_enable")
+
+    def timing_get(self, kernel):
+        n, ms = C.c_int64(), C.c_double()
+        _check(lib().mst_timing_get(self.h, int(kernel), C.byref(n), C.byref(ms)), "mst_timing_get")
+        return n.value, ms.value
+
+    def timing_reset(self):
+        _check(lib().mst_timing_reset(self.h), "mst_timing_reset")
+
+    def sync(self):
+        _check(lib().mst_sync(self.h), "mst_sync")
+
+    def algorithmic_bytes(self, kernel):
+        b = C.c_double()
+        _check(lib().mst_algorithmic_bytes(self.h, int(kernel), C.byref(b)), "mst_algorithmic_bytes")
+        return b.value
+
+
+def riemann_flux(cfg, QL, QR, dirs, dim=2):
+    """Batched device Riemann flux: F[i] = scheme.set(QL[i], QR[i]).solverAll(dirs[i])."""
+    QL = np.ascontiguousarray(QL, np.float64).reshape(-1, dim + 2)
+    QR = np.ascontiguousarray(QR, np.float64).reshape(-1, dim + 2)
+    d = np.ascontiguousarray(dirs, np.int32).reshape(-1)
+    F = np.zeros_like(QL)
+    _check(lib().mst_riemann_flux(C.byref(cfg), int(dim), QL.shape[0], QL.reshape(-1), QR.reshape(-1), d,
+                                  F.reshape(-1)), "mst_riemann_flux")
+    return F

@@ -1,0 +1,87 @@
+"""Shared inputs and parity checks for the tests (states, meshes, tolerance checks)."""
+import numpy as np
+
+GAMMA = 1.4
+
+
+def cons(rho, vel, p, gamma=GAMMA):
+    """Conserved state from primitives; vel has shape (n, dim)."""
+    rho = np.asarray(rho, np.float64)
+    vel = np.asarray(vel, np.float64)
+    n, dim = vel.shape
+    Q = np.zeros((n, dim + 2))
+    Q[:, 0] = rho
+    Q[:, 1:dim + 1] = rho[:, None] * vel
+    Q[:, dim + 1] = p / (gamma - 1) + 0.5 * rho * (vel ** 2).sum(1)
+    return Q
+
+
+def prim(Q, gamma=GAMMA):
+    dim = Q.shape[1] - 2
+    rho = Q[:, 0]
+    vel = Q[:, 1:dim + 1] / rho[:, None]
+    p = (gamma - 1) * (Q[:, dim + 1] - 0.5 * rho * (vel ** 2).sum(1))
+    return rho, vel, p
+
+
+def sod_state(ccen, x_split=0.5, left=(1.0, 1.0), right=(0.125, 0.1), gamma=GAMMA):
+    n, dim = ccen.shape
+    L = ccen[:, 0] <= x_split
+    rho = np.where(L, left[0], right[0])
+    p = np.where(L, left[1], right[1])
+    return cons(rho, np.zeros((n, dim)), p, gamma)
+
+
+def smooth_state(ccen, seed=0, amp=0.3, mach=0.8, gamma=GAMMA):
+    """Smooth, everywhere-physical field with sub- and supersonic regions (mach is the peak speed/a)."""
+    rng = np.random.default_rng(seed)
+    n, dim = ccen.shape
+    k = rng.uniform(2, 7, size=(4, dim))
+    ph = rng.uniform(0, 2 * np.pi, size=4)
+    x = ccen / (np.abs(ccen).max(0) + 1e-30)
+    w = [np.sin(x @ k[i] * np.pi + ph[i]) for i in range(4)]
+    rho = 1 + amp * w[0]
+    p = 1 + amp * w[1]
+    a = np.sqrt(gamma * p / rho)
+    vel = np.zeros((n, dim))
+    vel[:, 0] = mach * a * w[2]
+    vel[:, 1] = mach * a * w[3]
+    if dim == 3:
+        vel[:, 2] = 0.5 * mach * a * w[2] * w[3]
+    return cons(rho, vel, p, gamma)
+
+
+def jumpy_state(n, dim=2, seed=0, gamma=GAMMA, vmax=2.0):
+    """Independent random states per cell: large jumps, sub/supersonic, both flow directions."""
+    rng = np.random.default_rng(seed)
+    rho = rng.uniform(0.3, 3.0, n)
+    p = rng.uniform(0.2, 3.0, n)
+    a = np.sqrt(gamma * p / rho)
+    vel = rng.uniform(-vmax, vmax, (n, dim)) * a[:, None]
+    return cons(rho, vel, p, gamma)
+
+
+def residual_parity_error(Rg, Rr, Sr):
+    """max over cells and fields of |R_gpu - R_ref| / S_ref (S = sum_f sum_d |n_d| |F_d|)."""
+    den = np.where(Sr > 0, Sr, 1.0)
+    err = np.abs(Rg - Rr) / den
+    err = np.where(Sr > 0, err, np.abs(Rg - Rr) * 1e300)
+    return float(err.max())
+
+
+def rel_state_error(Qg, Qr):
+    """max |Q_gpu - Q_ref| / max(|Q_ref|, field scale)."""
+    scale = np.maximum(np.abs(Qr), np.abs(Qr).max(0, keepdims=True) * 1e-3)
+    return float((np.abs(Qg - Qr) / scale).max())
+
+
+def flip_faces(raw, frac=0.5, seed=0):
+    """Reverse the node order of a random subset of 2D faces: their area vector then points out of c0
+    (the reference reader's "branch 2", as in all bundled Fluent meshes)."""
+    rng = np.random.default_rng(seed)
+    out = dict(raw)
+    fn = raw["fnodes"].copy()
+    sel = rng.random(fn.shape[0]) < frac
+    fn[sel] = fn[sel][:, ::-1]
+    out["fnodes"] = np.ascontiguousarray(fn)
+    return out, sel

@@ -1,0 +1,211 @@
+"""GPU parity: the HIP residual path (through the C ABI) against the CPU oracle.
+
+Bar: per cell and field |R_gpu - R_ref| <= 1e-12 * S_ref with S = sum_f sum_d |n_d||F_d| (the
+survey's parity metric), and <= 1e-12 relative on updated states.
+"""
+import numpy as np
+import pytest
+
+import mstcfd as M
+from helpers import cons, flip_faces, jumpy_state, rel_state_error, residual_parity_error, smooth_state
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+ROE, AUSM = M.ROE, M.AUSM
+
+
+def both_cfg(dim=2, **kw):
+    return M.make_config(dim=dim, **kw), O.make_config(dim=dim, **kw)
+
+
+_MESHES = {}
+
+
+def mesh(name):
+    if name not in _MESHES:
+        if name == "strip_wall":
+            m = M.Mesh.gen_tri(200, 4, 0, 1, 0, 0.02, 0.1, 12345, (M.BC_WALL,) * 4)
+        elif name == "allbc":
+            m = M.Mesh.gen_tri(40, 30, 0, 4, 0, 2, 0.1, 7, (M.BC_INLET, M.BC_OUTLET, M.BC_WALL, M.BC_SYMMETRY))
+        elif name == "periodic":
+            m = M.Mesh.gen_tri(40, 40, -10, 10, -10, 10, 0.1, 3, (M.BC_PERIODIC,) * 4)
+        elif name == "tet":
+            m = M.Mesh.gen_tet(6, 5, 4, (0, 0, 0), (1, 0.8, 0.7), 0.15, 11,
+                               (M.BC_INLET, M.BC_OUTLET, M.BC_SYMMETRY, M.BC_WALL, M.BC_SYMMETRY, M.BC_SYMMETRY))
+        else:
+            raise KeyError(name)
+        _MESHES[name] = (m, O.OracleMesh(m.raw()), M.Prep(m).geometry())
+    return _MESHES[name]
+
+
+def state(name, kind, geo, dim, order=1):
+    if kind == "smooth":
+        return smooth_state(geo["ccen"], seed=5, mach=1.3)
+    if order == 1:
+        return jumpy_state(geo["vol"].shape[0], dim=dim, seed=9)
+    # second order: cell-to-cell jumps small enough that reconstructed states stay physical
+    return mild_jumps(geo["vol"].shape[0], dim, seed=9)
+
+
+def mild_jumps(n, dim, seed):
+    rng = np.random.default_rng(seed)
+    rho = rng.uniform(0.85, 1.2, n)
+    p = rng.uniform(0.85, 1.2, n)
+    vel = rng.uniform(-0.4, 0.4, (n, dim))
+    return cons(rho, vel, p)
+
+
+def inlet(dim):
+    q = cons(np.array([1.2]), np.array([[0.6] + [0.1] * (dim - 1)]), np.array([1.1]))[0]
+    return list(q)
+
+
+def test_riemann_batch_matches_oracle():
+    rng = np.random.default_rng(1)
+    for dim in (2, 3):
+        n = 4000
+        QL = jumpy_state(n, dim, seed=2)
+        QR = jumpy_state(n, dim, seed=3)
+        # near-equal pairs exercise the Harten fix and the subsonic AUSM branch
+        QR[: n // 4] = QL[: n // 4] * (1 + 1e-3 * rng.standard_normal((n // 4, dim + 2)))
+        dirs = rng.integers(0, dim, n).astype(np.int32)
+        for scheme in (ROE, AUSM):
+            for compat in (O.COMPAT_REFERENCE, 0):
+                cg, co = both_cfg(dim=dim, scheme=scheme, compat=compat)
+                Fg = M.riemann_flux(cg, QL, QR, dirs, dim=dim)
+                fn = O.roe_flux if scheme == ROE else O.ausm_flux
+                Fr = np.array([fn(QL[i], QR[i], int(dirs[i]), co, dim=dim) for i in range(n)])
+                scale = np.abs(Fr).max(1) + np.abs(QL).max(1) + np.abs(QR).max(1)
+                err = (np.abs(Fg - Fr).max(1) / scale).max()
+                print(f"riemann dim={dim} scheme={scheme} compat={compat}: max scaled err {err:.2e}", flush=True)
+                assert err <= TOL
+
+
+CASES = [
+    ("strip_wall", ROE, 1, 0), ("strip_wall", AUSM, 1, 0),
+    ("allbc", ROE, 1, 0), ("allbc", AUSM, 1, 0),
+    ("periodic", ROE, 1, 0), ("periodic", AUSM, 1, 0),
+    ("strip_wall", ROE, 2, M.LIM_MINMOD), ("allbc", ROE, 2, M.LIM_NONE), ("allbc", AUSM, 2, M.LIM_MINMOD),
+    ("allbc", ROE, 2, M.LIM_VANLEER), ("periodic", ROE, 2, M.LIM_VANLEER), ("periodic", AUSM, 2, M.LIM_NONE),
+    ("tet", AUSM, 1, 0), ("tet", ROE, 1, 0), ("tet", AUSM, 2, M.LIM_MINMOD), ("tet", ROE, 2, M.LIM_VANLEER),
+]
+
+
+@pytest.mark.parametrize("mname,scheme,order,lim", CASES)
+@pytest.mark.parametrize("kind", ["smooth", "jumpy"])
+def test_residual_parity(mname, scheme, order, lim, kind):
+    m, om, geo = mesh(mname)
+    dim = m.dim
+    cg, co = both_cfg(dim=dim, scheme=scheme, order=order, limiter=lim, q_inlet=inlet(dim))
+    Q = state(mname, kind, geo, dim, order)
+    s = M.Solver(m, cg)
+    s.set_state(Q)
+    Rg = s.residual()
+    Rr, Sr = om.residual(co, Q)
+    err = residual_parity_error(Rg, Rr, Sr)
+    print(f"{mname} scheme={scheme} order={order} lim={lim} {kind}: residual err/S = {err:.2e}", flush=True)
+    assert err <= TOL
+    s.close()
+
+
+@pytest.mark.parametrize("mname,lim", [("allbc", M.LIM_MINMOD), ("periodic", M.LIM_VANLEER), ("tet", M.LIM_MINMOD),
+                                       ("strip_wall", M.LIM_NONE)])
+def test_gradient_and_limiter_parity(mname, lim):
+    m, om, geo = mesh(mname)
+    dim = m.dim
+    cg, co = both_cfg(dim=dim, order=2, limiter=lim)
+    Q = smooth_state(geo["ccen"], seed=4)
+    s = M.Solver(m, cg)
+    s.set_state(Q)
+    gg, phg = s.gradient()
+    gr = om.gradient(co, Q)
+    phr = om.limiter(co, Q, gr)
+    scale = np.abs(gr).max(axis=(0, 2), keepdims=True)  # per-field gradient scale
+    gerr = float((np.abs(gg - gr) / scale).max())
+    perr = float(np.abs(phg - phr).max())
+    print(f"{mname} lim={lim}: grad err {gerr:.2e}, phi err {perr:.2e}", flush=True)
+    assert gerr <= 1e-11
+    assert perr <= 1e-9
+
+
+@pytest.mark.parametrize("mname,scheme", [("allbc", ROE), ("strip_wall", AUSM), ("tet", AUSM)])
+def test_euler_solve_matches_oracle(mname, scheme):
+    """RhoSolver::solve: Q_new = Q_old - DT/V R, then updateNewToOld."""
+    m, om, geo = mesh(mname)
+    dim = m.dim
+    cg, co = both_cfg(dim=dim, scheme=scheme, q_inlet=inlet(dim))
+    Q = smooth_state(geo["ccen"], seed=2, mach=0.5)
+    dt = 1.0 / 4000.0 if mname != "allbc" else 1e-3
+    s = M.Solver(m, cg)
+    s.set_state(Q)
+    s.set_dt(dt)
+    s.solve()
+    Qg = s.get_new_state()
+    Qr = om.euler(co, Q, dt)
+    assert rel_state_error(Qg, Qr) <= TOL
+    s.update_new_to_old()
+    assert np.array_equal(s.get_state(), Qg)
+
+
+@pytest.mark.parametrize("mname,scheme,order,lim,rk", [
+    ("strip_wall", ROE, 2, M.LIM_MINMOD, 2), ("periodic", ROE, 2, M.LIM_VANLEER, 3),
+    ("allbc", AUSM, 2, M.LIM_MINMOD, 2), ("tet", AUSM, 2, M.LIM_MINMOD, 2), ("allbc", ROE, 1, 0, 1)])
+def test_rk_steps_match_oracle(mname, scheme, order, lim, rk):
+    m, om, geo = mesh(mname)
+    dim = m.dim
+    cg, co = both_cfg(dim=dim, scheme=scheme, order=order, limiter=lim, rk=rk, cfl=0.4, q_inlet=inlet(dim))
+    Q = smooth_state(geo["ccen"], seed=3, mach=0.6)
+    s = M.Solver(m, cg)
+    s.set_state(Q)
+    Qr = Q.copy()
+    for it in range(10):
+        dt_r = om.cfl_dt(co, Qr)
+        dt_g = s.compute_dt(0.4)
+        assert abs(dt_g - dt_r) <= 1e-13 * dt_r
+        Qr, res = om.rk_step(co, Qr, dt_r)
+        s.step(1)
+        ng = s.residual_norms()
+        assert np.allclose(ng, res, rtol=1e-10, atol=0)
+    Qg = s.get_state()
+    err = rel_state_error(Qg, Qr)
+    print(f"{mname} rk{rk}: 10-step state err {err:.2e}", flush=True)
+    assert err <= 1e-11
+
+
+def test_branch2_orientation_compat():
+    """Faces whose area vector points out of c0 (every bundled Fluent mesh): physical L/R gives the
+    same residual as the unflipped mesh bit for bit; the literal reference flags match the oracle."""
+    m, om, geo = mesh("allbc")
+    raw2, sel = flip_faces(m.raw(), 0.5, seed=4)
+    m2 = M.Mesh.from_raw(raw2)
+    om2 = O.OracleMesh(raw2)
+    Q = smooth_state(geo["ccen"], seed=7, mach=0.7)
+    for scheme in (ROE, AUSM):
+        cg, co = both_cfg(scheme=scheme, q_inlet=inlet(2))
+        s1, s2 = M.Solver(m, cg), M.Solver(m2, cg)
+        s1.set_state(Q)
+        s2.set_state(Q)
+        R1, R2 = s1.residual(), s2.residual()
+        assert residual_parity_error(R2, *om2.residual(co, Q)) <= TOL
+        assert np.abs(R1 - R2).max() <= 1e-12 * np.abs(R1).max()
+        cg2, co2 = both_cfg(scheme=scheme, q_inlet=inlet(2), compat=O.COMPAT_REFERENCE | O.COMPAT_BRANCH2_LR)
+        s2.set_config(cg2)
+        Rg = s2.residual()
+        Rr, Sr = om2.residual(co2, Q)
+        assert residual_parity_error(Rg, Rr, Sr) <= TOL
+        assert np.abs(Rg - R1).max() > 1e-6  # the literal flags really do differ
+
+
+def test_monitor_maxrel_matches_reference_rule():
+    m, om, geo = mesh("strip_wall")
+    cg, co = both_cfg()
+    Q = smooth_state(geo["ccen"], seed=8, mach=0.3)
+    s = M.Solver(m, cg)
+    s.set_state(Q)
+    s.set_dt(1e-4)
+    s.solve()
+    got = s.monitor_maxrel()
+    ref = O.monitor_maxrel(s.get_new_state(), s.get_state())
+    assert np.allclose(got, ref, rtol=0, atol=0)
