@@ -1,0 +1,134 @@
+"""Benchmark of the MI355X residual path: face-flux updates/s and achieved HBM bandwidth.
+
+Default workload: 2D isentropic vortex (strength 5, Mach 0.5 freestream, gamma 1.4) on [-10,10]^2,
+1000x1000 quads split into 2,000,000 jittered triangles (+-0.1 h, seed 12345), periodic on all
+sides; Roe + van Leer, SSP-RK3, CFL 0.5 (device-side time step), FP64.  One "step" is one RK3
+time step = 3 residual evaluations (gradient+limiter, face flux, gather+update each) + the CFL
+reduction.  face-updates/s = N_faces * 3 * steps / wall time, inputs resident in HBM.
+
+Prints ONE JSON line (see the keys below).  Only one GPU is used: --gpus N>1 exits with a message.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "mst-cfd_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def vortex_state(ccen, beta=5.0, mach=0.5, gamma=1.4):
+    """Shu's isentropic vortex centred at the origin on a rho=1, p=1 freestream moving in +x."""
+    x, y = ccen[:, 0], ccen[:, 1]
+    r2 = x * x + y * y
+    uinf = mach * math.sqrt(gamma)
+    e = np.exp(0.5 * (1.0 - r2))
+    du = -beta / (2 * math.pi) * y * e
+    dv = beta / (2 * math.pi) * x * e
+    T = 1.0 - (gamma - 1) * beta ** 2 / (8 * gamma * math.pi ** 2) * np.exp(1.0 - r2)
+    rho = T ** (1.0 / (gamma - 1))
+    p = rho * T
+    u, v = uinf + du, dv
+    Q = np.stack([rho, rho * u, rho * v, p / (gamma - 1) + 0.5 * rho * (u * u + v * v)], axis=1)
+    return np.ascontiguousarray(Q)
+
+
+def build_workload(M, name):
+    if name == "vortex2d":
+        mesh = M.Mesh.gen_tri(1000, 1000, -10.0, 10.0, -10.0, 10.0, 0.1, 12345, (M.BC_PERIODIC,) * 4)
+        cfg = dict(scheme=M.ROE, order=2, limiter=M.LIM_VANLEER, rk=3, cfl=0.5)
+        desc = dict(workload="isentropic vortex 2D, 1000x1000x2 jittered triangles, periodic",
+                    cells=mesh.n_cells, faces=mesh.n_faces, scheme="Roe", limiter="van Leer", rk=3, cfl=0.5)
+        return mesh, cfg, desc
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(mesh, cfg_kw, Q, threads, budget_s=10.0):
+    """Oracle (the CPU restatement of the reference path, OpenMP) on one RK3 step of the same mesh."""
+    sys.path.insert(0, ROOT)
+    from oracle import pyoracle as O
+
+    O.set_threads(threads)
+    om = O.OracleMesh(mesh.raw())
+    co = O.make_config(**cfg_kw)
+    Qc = Q
+    nsteps, t = 0, 0.0
+    while nsteps < 1 or (t < budget_s and nsteps < 200):
+        dt = om.cfl_dt(co, Qc)
+        t0 = time.perf_counter()
+        Qc, _ = om.rk_step(co, Qc, dt)
+        t += time.perf_counter() - t0
+        nsteps += 1
+    return dict(value=mesh.n_faces * cfg_kw["rk"] * nsteps / t, unit="face-updates/s", cores=threads, kind="port",
+                sample=f"{nsteps} RK{cfg_kw['rk']} steps ({cfg_kw['rk']} residual evaluations each) of the same "
+                       f"{mesh.n_cells}-cell workload, oracle/mst_oracle.c with OpenMP, {t:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="vortex2d")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); the multi-GPU path is a "
+              f"design (mesh partition + halo exchange), not built in this tree.", file=sys.stderr)
+        sys.exit(2)
+    import mstcfd as M
+
+    mesh, cfg_kw, desc = build_workload(M, args.workload)
+    solver = M.Solver(mesh, M.make_config(**cfg_kw))
+    geo = solver.prep.geometry()
+    Q0 = vortex_state(geo["ccen"])
+    solver.set_state(Q0)
+    solver.step(args.warmup)
+    solver.sync()
+    solver.timing_reset()
+    solver.timing(True)
+    solver.sync()
+    t0 = time.perf_counter()
+    solver.step(args.steps)
+    solver.sync()
+    t1 = time.perf_counter()
+    solver.timing(False)
+    wall = t1 - t0
+    stages = cfg_kw["rk"]
+    value = mesh.n_faces * stages * args.steps / wall
+    kern = {}
+    for k, name in M.KERNEL_NAMES.items():
+        n, ms = solver.timing_get(k)
+        if n:
+            kern[name] = dict(launches=n, avg_us=1e3 * ms / n, total_ms=ms,
+                              alg_bytes=solver.algorithmic_bytes(k))
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    d = kern[dom]
+    achieved = d["alg_bytes"] / (d["avg_us"] * 1e-6) / 1e9
+    for k in kern:
+        kern[k]["gbs"] = kern[k]["alg_bytes"] / (kern[k]["avg_us"] * 1e-6) / 1e9
+    out = dict(
+        metric="face-flux updates/s (FP64 residual path); %HBM peak of the dominant kernel",
+        value=value, unit="face-updates/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+        ms_per_step=1e3 * wall / args.steps, higher_is_better=True, scaling="weak", vs_baseline=None,
+        dtype="f64", data="synthetic", config=desc,
+        roofline=dict(bound="hbm", kernel=dom, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                      frac=achieved / HBM_PEAK_GBS, traffic=None),
+        kernels=kern,
+    )
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(mesh, cfg_kw, Q0, threads)
+    Qf = solver.get_state()
+    out["state_finite"] = bool(np.isfinite(Qf).all())
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
