@@ -80,7 +80,7 @@ void geometry(const mst_mesh &m, Prepared &P, std::vector<int64_t> &cptr, std::v
     }
     P.dac.assign(nf, 1);
     for (int64_t f = 0; f < nf; f++) {
-        double dx[3];
+        double dx[3] = {0, 0, 0};
         for (int k = 0; k < dim; k++) dx[k] = P.fcen[f * dim + k] - P.ccen[(int64_t)m.c0[f] * dim + k];
         P.dac[f] = dotn(&P.S[f * dim], dx, dim) < 0 ? -1 : 1;
     }
@@ -121,7 +121,7 @@ void geometry(const mst_mesh &m, Prepared &P, std::vector<int64_t> &cptr, std::v
             for (int64_t j = b; j < b + k; j++) {
                 int32_t f = cface[j];
                 double sg = (c == m.c0[f]) ? (double)P.dac[f] : -(double)P.dac[f];
-                double dx[3];
+                double dx[3] = {0, 0, 0};
                 for (int q = 0; q < dim; q++) dx[q] = P.fcen[(int64_t)f * dim + q] - P.ccen[c * dim + q];
                 V += sg * dotn(&P.S[(int64_t)f * dim], dx, dim);
             }
@@ -135,7 +135,7 @@ void geometry(const mst_mesh &m, Prepared &P, std::vector<int64_t> &cptr, std::v
     P.eta.assign(nf, 1.0);
     for (int64_t f = 0; f < nf; f++) {
         const double *xf = &P.fcen[f * dim];
-        double d0[3], d1[3];
+        double d0[3] = {0, 0, 0}, d1[3] = {0, 0, 0};
         if (m.c1[f] >= 0) {
             for (int k = 0; k < dim; k++) {
                 d0[k] = P.ccen[(int64_t)m.c0[f] * dim + k] - xf[k];
@@ -284,8 +284,12 @@ int prepare(const mst_mesh &m, Prepared &P) {
     std::vector<int64_t> cptr;
     std::vector<int32_t> cface;
     geometry(m, P, cptr, cface);
+    P.bad_cells = 0;
     for (int64_t c = 0; c < nc; c++)
-        if (!(P.vol[c] > 0)) return fail(MST_E_MESH, "cell " + std::to_string(c) + " has non-positive volume");
+        if (!(P.vol[c] > 0)) {
+            if (P.bad_cells == 0) P.first_bad_cell = c;
+            P.bad_cells++;
+        }
 
     // ---- cells: RCM
     rcm(m, P.cell_old_of_new);

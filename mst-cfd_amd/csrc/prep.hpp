@@ -24,6 +24,8 @@ struct Prepared {
     int64_t n_interior = 0;           // new-order faces [0, n_interior) have two cells
     int64_t pstart = 0, pcount = 0;   // periodic faces are the contiguous range [pstart, pstart+pcount)
     int64_t quirk_face = -1;          // new index of file face N_int-1 (RhoSolver.cpp:437-440), -1 if none
+    int64_t bad_cells = 0;            // cells whose reference area formula gives <= 0 or NaN (degenerate)
+    int64_t first_bad_cell = -1;
 
     // geometry in FILE order (what the reference stores on Face/Cell)
     std::vector<double> fcen, S, eta;  // nf*dim, nf*dim, nf

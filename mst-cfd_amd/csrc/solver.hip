@@ -992,6 +992,12 @@ int mst_ctx_create(const mst_mesh *m, const mst_config *cfg, mst_ctx **out) {
         delete x;
         return rc;
     }
+    if (x->prep.P.bad_cells > 0) {
+        rc = mst::fail(MST_E_MESH, std::to_string(x->prep.P.bad_cells) + " cells have non-positive or NaN volume "
+                                   "(first: file cell " + std::to_string(x->prep.P.first_bad_cell) + ")");
+        delete x;
+        return rc;
+    }
     const mst::Prepared &P = x->prep.P;
     x->cfg = *cfg;
     x->dim = P.dim;

@@ -1,0 +1,88 @@
+"""C ABI: the libraries load, export what the headers declare, and report errors instead of crashing."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import mstcfd as M
+from oracle import pyoracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    txt = open(header).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(mst_\w+|orc_\w+)\s*\(", txt)))
+
+
+def exported(path):
+    L = C.CDLL(path)
+    return L
+
+
+def test_product_exports_every_declared_symbol():
+    names = declared(os.path.join(ROOT, "include", "mstcfd.h"))
+    assert len(names) >= 40
+    L = M.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert sorted(names) == sorted(M.EXPORTS)
+
+
+def test_oracle_exports_every_declared_symbol():
+    names = declared(os.path.join(ROOT, "oracle", "mst_oracle.h"))
+    L = O.lib()
+    assert not [n for n in names if not hasattr(L, n)]
+
+
+def test_error_paths_return_codes():
+    h = C.c_void_p()
+    rc = M.lib().mst_mesh_read_msh(b"/nonexistent/file.msh", C.byref(h))
+    assert rc == -2 and "cannot open" in M.last_error()
+    with pytest.raises(M.MstError):
+        M.Mesh.gen_tri(0, 4, 0, 1, 0, 1)
+    with pytest.raises(M.MstError):
+        M.Mesh.gen_tri(4, 4, 0, 1, 0, 1, bc=(M.BC_PERIODIC, M.BC_WALL, M.BC_WALL, M.BC_WALL))
+    with pytest.raises(M.MstError):
+        M.Mesh.gen_tet(2, 2, 2, bc=(M.BC_PERIODIC,) * 6)
+
+
+def test_malformed_mesh_is_rejected(tmp_path):
+    m = M.Mesh.gen_tri(6, 4, 0, 1, 0, 1, 0.1, 1)
+    p = tmp_path / "m.msh"
+    m.write_msh(p)
+    txt = p.read_text().splitlines()
+    cut = tmp_path / "cut.msh"
+    cut.write_text("\n".join(txt[: len(txt) // 2]) + "\n")
+    with pytest.raises(M.MstError):
+        M.Mesh.read_msh(cut)
+    raw = m.raw()
+    raw["c0"] = raw["c0"].copy()
+    raw["c0"][3] = m.n_cells + 5
+    with pytest.raises(M.MstError):
+        M.Mesh.from_raw(raw)
+
+
+def test_ctx_requires_a_device():
+    """Without a HIP device the compute path fails loudly (no CPU fallback)."""
+    import subprocess
+    import sys
+
+    code = ("import sys; sys.path.insert(0, %r); import mstcfd as M\n"
+            "m = M.Mesh.gen_tri(4, 4, 0, 1, 0, 1)\n"
+            "try:\n    M.Solver(m, M.make_config()); print('DEVICE')\n"
+            "except M.MstError as e:\n    print('ERR', e)\n") % os.path.join(ROOT, "mst-cfd_amd")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120).stdout
+    if "DEVICE" in out:
+        pytest.skip("a HIP device is present")
+    assert "no HIP device" in out
+
+
+def test_config_layout_matches_oracle():
+    assert C.sizeof(M.Config) == C.sizeof(O.Config) == 4 * 6 + 8 * 4 + 8 * 5
+    for (a, ta), (b, tb) in zip(M.Config._fields_, O.Config._fields_):
+        assert a == b and ta == tb
+    assert np.allclose(M.reference_inlet(), O.reference_inlet()[:4])

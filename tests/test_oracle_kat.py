@@ -1,0 +1,370 @@
+"""Pins of the CPU oracle (CPU only): closed forms, invariants and exact solutions.
+
+The reference has no tests and cannot be compiled here (Eigen 3 absent), so each oracle function is
+pinned by mathematics chosen so that a plausible restatement mistake (sign, index, transposed
+operand, dropped term) fails at least one check."""
+import math
+
+import numpy as np
+import pytest
+
+import mstcfd as M
+from exact import sod_exact
+from helpers import cons, jumpy_state, prim, smooth_state, sod_state
+from oracle import pyoracle as O
+
+G = 1.4
+
+
+def phys_flux(Q, d, dim):
+    """Euler flux along axis d from primitive variables (textbook form)."""
+    rho, vel, p = prim(Q[None, :])
+    rho, vel, p = rho[0], vel[0], p[0]
+    E = Q[dim + 1]
+    F = np.zeros(dim + 2)
+    F[0] = rho * vel[d]
+    F[1:dim + 1] = rho * vel[d] * vel
+    F[1 + d] += p
+    F[dim + 1] = vel[d] * (E + p)
+    return F
+
+
+def mirror(Q, d):
+    Q = Q.copy()
+    Q[1 + d] = -Q[1 + d]
+    return Q
+
+
+# ------------------------------------------------------------------------ thermodynamics
+@pytest.mark.parametrize("dim", [2, 3])
+def test_thermo_closed_forms(dim):
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        rho, p = rng.uniform(0.1, 5), rng.uniform(0.1, 5)
+        vel = rng.uniform(-3, 3, dim)
+        Q = cons([rho], vel[None, :], [p])[0]
+        assert abs(O.pressure(Q, dim) - p) <= 1e-14 * (p + rho * (vel ** 2).sum())
+        H = G / (G - 1) * p / rho + 0.5 * (vel ** 2).sum()
+        assert abs(O.enthalpy(Q, dim) - H) <= 1e-14 * H * 10
+
+
+# ------------------------------------------------------------------------ Roe
+def roe_jacobian(u, H, d, dim):
+    """Flux Jacobian dF_d/dQ at (u, H) for gamma = 1.4 (textbook)."""
+    nv = dim + 2
+    q2 = (u ** 2).sum()
+    A = np.zeros((nv, nv))
+    A[0, 1 + d] = 1
+    for k in range(dim):
+        A[1 + k, 0] = -u[k] * u[d] + (0.5 * (G - 1) * q2 if k == d else 0)
+        for j in range(dim):
+            A[1 + k, 1 + j] = (u[d] if j == k else 0) + (u[k] if j == d else 0) - ((G - 1) * u[j] if k == d else 0)
+        A[1 + k, nv - 1] = (G - 1) if k == d else 0
+    A[nv - 1, 0] = u[d] * (0.5 * (G - 1) * q2 - H)
+    for j in range(dim):
+        A[nv - 1, 1 + j] = (H if j == d else 0) - (G - 1) * u[j] * u[d]
+    A[nv - 1, nv - 1] = G * u[d]
+    return A
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_roe_eigenvectors_and_inverse(dim):
+    cfg = O.make_config(dim=dim)
+    QL, QR = jumpy_state(20, dim, seed=1), jumpy_state(20, dim, seed=2)
+    for i in range(20):
+        for d in range(dim):
+            Rm, Ri, lam = O.roe_eigvecs(QL[i], QR[i], d, cfg, dim)
+            assert np.abs(Rm @ Ri - np.eye(dim + 2)).max() <= 1e-12
+            # Roe averages in the textbook sqrt(rho) weighting, independent of the reference's D form
+            rl, vl, pl = prim(QL[i:i + 1])
+            rr, vr, pr = prim(QR[i:i + 1])
+            wl, wr = math.sqrt(rl[0]), math.sqrt(rr[0])
+            u = (wl * vl[0] + wr * vr[0]) / (wl + wr)
+            H = (wl * O.enthalpy(QL[i], dim) + wr * O.enthalpy(QR[i], dim)) / (wl + wr)
+            a = math.sqrt((G - 1) * (H - 0.5 * (u ** 2).sum()))
+            raw = np.array([u[d] - a] + [u[d]] * dim + [u[d] + a])
+            A = roe_jacobian(u, H, d, dim)
+            for j in range(dim + 2):  # columns are eigenvectors with the raw eigenvalues
+                assert np.abs(A @ Rm[:, j] - raw[j] * Rm[:, j]).max() <= 1e-11 * (1 + np.abs(A).max())
+            fix = np.where(np.abs(raw) > 0.125, np.abs(raw), (raw ** 2 + 0.125 ** 2) / 0.25)
+            assert np.allclose(lam, fix, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_roe_consistency_upwinding_and_mirror(dim):
+    cfg0 = O.make_config(dim=dim, eor=0.0)
+    cfg = O.make_config(dim=dim)
+    Q = jumpy_state(40, dim, seed=3)
+    for i in range(40):
+        for d in range(dim):
+            F = O.roe_flux(Q[i], Q[i], d, cfg0, dim)
+            assert np.abs(F - phys_flux(Q[i], d, dim)).max() <= 1e-13 * np.abs(F).max()
+            # rho + EOR in the momentum denominators only shifts F by ~1e-10 relative
+            Fe = O.roe_flux(Q[i], Q[i], d, cfg, dim)
+            assert 0 < np.abs(Fe - F).max() <= 1e-9 * np.abs(F).max() or np.abs(Q[i, 1:dim + 1]).max() == 0
+    # both states supersonic along +d with every wave speed above the Harten delta: F = F(Q_L)
+    for d in range(dim):
+        vl = np.zeros((1, dim)); vl[0, d] = 3.0
+        vr = np.zeros((1, dim)); vr[0, d] = 2.8
+        QL, QR = cons([1.0], vl, [0.7])[0], cons([0.9], vr, [0.6])[0]
+        F = O.roe_flux(QL, QR, d, cfg0, dim)
+        assert np.abs(F - phys_flux(QL, d, dim)).max() <= 1e-12 * np.abs(F).max()
+    # mirror symmetry: swapping sides and reflecting u_d negates every component except momentum d
+    QL, QR = jumpy_state(30, dim, seed=4), jumpy_state(30, dim, seed=5)
+    for i in range(30):
+        for d in range(dim):
+            F = O.roe_flux(QL[i], QR[i], d, cfg, dim)
+            Fm = O.roe_flux(mirror(QR[i], d), mirror(QL[i], d), d, cfg, dim)
+            assert np.abs(Fm + mirror(F, d)).max() <= 1e-12 * np.abs(F).max()
+
+
+def test_roe_harten_fix_continuity():
+    cfg = O.make_config()
+    base = cons([1.0], [[0.0, 0.0]], [1.0])[0]
+    ups = np.linspace(0.125 - 1e-6, 0.125 + 1e-6, 201)
+    Fs = []
+    for u in ups:
+        QL = cons([1.0], [[u, 0.05]], [1.0])[0]
+        QR = cons([1.1], [[u, 0.05]], [1.05])[0]
+        Fs.append(O.roe_flux(QL, QR, 0, cfg))
+    jumps = np.abs(np.diff(np.array(Fs), axis=0)).max()
+    assert jumps <= 1e-6 and base is not None
+
+
+# ------------------------------------------------------------------------ AUSM
+def test_ausm_consistency_and_quirk():
+    for dim in (2, 3):
+        plain = O.make_config(dim=dim, scheme=O.AUSM, compat=0)
+        unsc = O.make_config(dim=dim, scheme=O.AUSM, compat=O.COMPAT_AUSM_P_UNSCALED)
+        ref = O.make_config(dim=dim, scheme=O.AUSM)
+        Q = jumpy_state(60, dim, seed=6, vmax=3.0)
+        for i in range(60):
+            for d in range(dim):
+                P = phys_flux(Q[i], d, dim)
+                # with a_face in place of the local a: F(Q,Q) is the Euler flux for every Mach number
+                for c in (plain, unsc):
+                    F = O.ausm_flux(Q[i], Q[i], d, c, dim)
+                    assert np.abs(F - P).max() <= 1e-13 * np.abs(P).max()
+                # the reference scales the convective part by a/a~ (SolverAusm.cpp:112-113)
+                rho, vel, p = prim(Q[i:i + 1])
+                H = O.enthalpy(Q[i], dim)
+                ast = math.sqrt(2 * (G - 1) / (G + 1) * H)
+                at = ast * ast / max(ast, math.sqrt((vel ** 2).sum()))
+                a = math.sqrt(G * p[0] / rho[0])
+                expect = (a / at) * (P - np.where(np.arange(dim + 2) == 1 + d, p[0], 0)) + \
+                    np.where(np.arange(dim + 2) == 1 + d, p[0], 0)
+                F = O.ausm_flux(Q[i], Q[i], d, ref, dim)
+                assert np.abs(F - expect).max() <= 1e-12 * np.abs(expect).max()
+
+
+def test_ausm_supersonic_and_one_sided_branch():
+    cfg = O.make_config(scheme=O.AUSM, compat=0)
+    QL = cons([1.0], [[3.0, 0.2]], [0.5])[0]
+    QR = cons([0.8], [[2.9, 0.1]], [0.45])[0]
+    F = O.ausm_flux(QL, QR, 0, cfg)
+    assert np.abs(F - phys_flux(QL, 0, 2)).max() <= 1e-13 * np.abs(F).max()
+    # subsonic pairs are mirror symmetric; the one-sided "M <= 1" test breaks it for M_L < -1
+    QL, QR = cons([1.0], [[0.3, 0.1]], [1.0])[0], cons([0.9], [[-0.2, 0.0]], [0.8])[0]
+    F = O.ausm_flux(QL, QR, 0, cfg)
+    Fm = O.ausm_flux(mirror(QR, 0), mirror(QL, 0), 0, cfg)
+    assert np.abs(Fm + mirror(F, 0)).max() <= 1e-13 * np.abs(F).max()
+    QL, QR = cons([1.0], [[-3.0, 0.0]], [0.5])[0], cons([1.0], [[-2.4, 0.0]], [0.5])[0]
+    F = O.ausm_flux(QL, QR, 0, cfg)
+    Fm = O.ausm_flux(mirror(QR, 0), mirror(QL, 0), 0, cfg)
+    assert np.abs(Fm + mirror(F, 0)).max() > 1e-3 * np.abs(F).max()
+
+
+# ------------------------------------------------------------------------ gradient / limiter
+def test_gradient_exact_for_linear_fields_on_regular_mesh():
+    m = M.Mesh.gen_tri(12, 10, 0, 1.2, 0, 1, 0.0, 0, (M.BC_WALL,) * 4)
+    raw = m.raw()
+    om = O.OracleMesh(raw)
+    geo = om.geometry()
+    g = np.array([[0.3, -0.2], [1.1, 0.4], [-0.7, 0.9], [0.5, 0.25]])
+    Q = 2.0 + geo["ccen"] @ g.T
+    cfg = O.make_config(order=2, compat=0)
+    grad = om.gradient(cfg, Q)
+    inner = np.ones(m.n_cells, bool)
+    inner[raw["c0"][raw["c1"] < 0]] = False
+    assert np.abs(grad[inner] - g[None]).max() <= 1e-11
+    assert np.abs(om.gradient(cfg, np.full_like(Q, 3.0))).max() <= 1e-11
+    # the reference's boundary loop starting at N_int-1 changes only the two cells of that face
+    g2 = om.gradient(O.make_config(order=2, compat=O.COMPAT_GRAD_LASTINT), Q)
+    f = raw["n_int"] - 1
+    diff = np.where(np.abs(g2 - grad).max(axis=(1, 2)) > 0)[0]
+    assert set(diff.tolist()) <= {raw["c0"][f], raw["c1"][f]} and len(diff) >= 1
+    # limiters leave a linear field unlimited in the interior
+    for lim in (O.LIM_MINMOD, O.LIM_VANLEER):
+        phi = om.limiter(O.make_config(order=2, limiter=lim, compat=0), Q, grad)
+        assert (phi[inner] == 1).all()
+
+
+@pytest.mark.parametrize("lim", [O.LIM_MINMOD, O.LIM_VANLEER])
+def test_limiter_bounds_and_extrema(lim):
+    m = M.Mesh.gen_tri(20, 16, 0, 1, 0, 1, 0.15, 2, (M.BC_WALL,) * 4)
+    raw = m.raw()
+    om = O.OracleMesh(raw)
+    geo = om.geometry()
+    Q = jumpy_state(m.n_cells, 2, seed=3)
+    cfg = O.make_config(order=2, limiter=lim)
+    grad = om.gradient(cfg, Q)
+    phi = om.limiter(cfg, Q, grad)
+    assert (phi >= 0).all() and (phi <= 1).all()
+    nb = [[] for _ in range(m.n_cells)]
+    fc = [[] for _ in range(m.n_cells)]
+    for f in range(m.n_faces):
+        a, b = raw["c0"][f], raw["c1"][f]
+        fc[a].append(f)
+        if b >= 0:
+            nb[a].append(b)
+            nb[b].append(a)
+            fc[b].append(f)
+    for c in range(m.n_cells):
+        qs = Q[[c] + nb[c]]
+        lo, hi = qs.min(0), qs.max(0)
+        for f in fc[c]:
+            rec = Q[c] + phi[c] * (grad[c] @ (geo["fcen"][f] - geo["ccen"][c]))
+            if lim == O.LIM_MINMOD:  # Barth-Jespersen bound
+                assert (rec <= hi + 1e-12 * np.abs(hi)).all() and (rec >= lo - 1e-12 * np.abs(lo)).all()
+        ext = (Q[c] >= hi) | (Q[c] <= lo)
+        dgrad = np.abs(grad[c]).max(1) > 1e-9
+        assert (phi[c][ext & dgrad] == 0).all()
+
+
+def test_vanleer_function_values():
+    """One interior cell of a regular mesh with a linear field: raising Q_c by s sets the ratio r of
+    its limiting face, and Phi must equal the textbook psi(r) = (r+|r|)/(1+|r|)."""
+    m = M.Mesh.gen_tri(8, 8, 0, 1, 0, 1, 0.0, 0, (M.BC_WALL,) * 4)
+    raw = m.raw()
+    om = O.OracleMesh(raw)
+    geo = om.geometry()
+    c = 2 * (4 * 8 + 4)  # an interior triangle
+    Q0 = 1.0 + 0.5 * geo["ccen"][:, 0]
+    Q = np.stack([Q0, 0.1 * Q0, 0.0 * Q0, 2.5 + 0 * Q0], 1)
+    nbs = [raw["c1"][f] if raw["c0"][f] == c else raw["c0"][f] for f in range(m.n_faces)
+           if c in (raw["c0"][f], raw["c1"][f])]
+    qmax = Q0[nbs].max()
+    for s in (0.0, 0.3, 0.6, 0.9):
+        Qs = Q.copy()
+        Qs[c, 0] = Q0[c] + s * (qmax - Q0[c])
+        for lim, psi in ((O.LIM_VANLEER, lambda r: (r + abs(r)) / (1 + abs(r))),
+                         (O.LIM_MINMOD, lambda r: max(0.0, min(1.0, r)))):
+            cfg = O.make_config(order=2, limiter=lim)
+            grad = om.gradient(cfg, Qs)
+            phi = om.limiter(cfg, Qs, grad)
+            rs = []
+            for f in range(m.n_faces):
+                if c not in (raw["c0"][f], raw["c1"][f]):
+                    continue
+                dl = grad[c, 0] @ (geo["fcen"][f] - geo["ccen"][c])
+                ext = max(Qs[[c] + nbs, 0]) if dl > 0 else min(Qs[[c] + nbs, 0])
+                if abs(dl) > 1e-12:
+                    rs.append((ext - Qs[c, 0]) / dl)
+            expect = min([1.0] + [psi(r) for r in rs])
+            assert abs(phi[c, 0] - expect) <= 1e-14
+
+
+# ------------------------------------------------------------------------ residual invariants
+@pytest.mark.parametrize("scheme", [O.ROE, O.AUSM])
+@pytest.mark.parametrize("order,lim", [(1, 0), (2, O.LIM_MINMOD), (2, O.LIM_VANLEER)])
+def test_freestream_preservation_and_conservation(scheme, order, lim):
+    m = M.Mesh.gen_tri(24, 20, -1, 1, -1, 1, 0.2, 4, (M.BC_PERIODIC,) * 4)
+    om = O.OracleMesh(m.raw())
+    cfg = O.make_config(scheme=scheme, order=order, limiter=lim)
+    Q = np.tile(cons([1.2], [[0.4, -0.3]], [0.9]), (m.n_cells, 1))
+    R, S = om.residual(cfg, Q)
+    assert (np.abs(R) <= 1e-14 * S.max()).all()
+    geo = om.geometry()
+    Q = smooth_state(geo["ccen"], seed=1, mach=0.9)
+    R, S = om.residual(cfg, Q)
+    assert (np.abs(R.sum(0)) <= 1e-13 * S.sum(0)).all()  # periodic: fluxes cancel pairwise
+    # inlet/outlet with the inlet at the freestream state: still preserved
+    m2 = M.Mesh.gen_tri(16, 12, 0, 2, 0, 1, 0.2, 5, (M.BC_INLET, M.BC_OUTLET, M.BC_INLET, M.BC_OUTLET))
+    om2 = O.OracleMesh(m2.raw())
+    q = cons([1.2], [[0.4, -0.3]], [0.9])[0]
+    cfg2 = O.make_config(scheme=scheme, order=order, limiter=lim, q_inlet=list(q))
+    R, S = om2.residual(cfg2, np.tile(q, (m2.n_cells, 1)))
+    assert (np.abs(R) <= 1e-14 * S.max()).all()
+
+
+def test_rk_stage_one_is_euler_and_ssp_combination():
+    m = M.Mesh.gen_tri(10, 8, 0, 1, 0, 1, 0.1, 6, (M.BC_PERIODIC,) * 4)
+    om = O.OracleMesh(m.raw())
+    Q = smooth_state(om.geometry()["ccen"], seed=2)
+    e1 = om.euler(O.make_config(rk=1), Q, 1e-3)
+    q1, _ = om.rk_step(O.make_config(rk=1), Q, 1e-3)
+    assert np.array_equal(e1, q1)
+    # SSP-RK2 (Heun): Q^{n+1} = (Q^n + E(E(Q^n)))/2 with E the Euler map
+    q2, _ = om.rk_step(O.make_config(rk=2), Q, 1e-3)
+    e2 = om.euler(O.make_config(), e1, 1e-3)
+    assert np.abs(q2 - 0.5 * (Q + e2)).max() <= 1e-14 * np.abs(Q).max()
+    # SSP-RK3 (Shu-Osher)
+    q3, _ = om.rk_step(O.make_config(rk=3), Q, 1e-3)
+    s2 = 0.75 * Q + 0.25 * e2
+    s3 = Q / 3 + 2 / 3 * om.euler(O.make_config(), s2, 1e-3)
+    assert np.abs(q3 - s3).max() <= 1e-14 * np.abs(Q).max()
+
+
+def test_cfl_time_step_closed_form():
+    m = M.Mesh.gen_tri(4, 4, 0, 1, 0, 1, 0.0, 0, (M.BC_WALL,) * 4)
+    om = O.OracleMesh(m.raw())
+    Q = np.tile(cons([1.0], [[0.5, 0.0]], [1.0]), (m.n_cells, 1))
+    # right triangles of legs h: V = h^2/2, half sums of |S_x| and |S_y| are both h
+    h = 0.25
+    a = math.sqrt(1.4)
+    expect = 0.5 * (h * h / 2) / ((0.5 + a) * h + a * h)
+    assert abs(om.cfl_dt(O.make_config(cfl=0.5), Q) - expect) <= 1e-15
+
+
+# ------------------------------------------------------------------------ physics
+def run_sod(nx, compat=O.COMPAT_REFERENCE, flip=False, t_end=0.2, limiter=O.LIM_MINMOD, scheme=O.ROE):
+    m = M.Mesh.gen_tri(nx, 2, 0, 1, 0, 0.01, 0.1, 12345, (M.BC_WALL,) * 4)
+    raw = m.raw()
+    if flip:
+        raw = dict(raw)
+        raw["fnodes"] = np.ascontiguousarray(raw["fnodes"][:, ::-1])
+    om = O.OracleMesh(raw)
+    geo = om.geometry()
+    cfg = O.make_config(scheme=scheme, order=2, limiter=limiter, rk=2, cfl=0.5, compat=compat)
+    Q = sod_state(geo["ccen"])
+    t = 0.0
+    while t < t_end - 1e-14:
+        dt = min(om.cfl_dt(cfg, Q), t_end - t)
+        if not dt > 0:
+            return geo, None
+        Q, _ = om.rk_step(cfg, Q, dt)
+        t += dt
+        if not np.isfinite(Q).all() or (Q[:, 0] <= 0).any():
+            return geo, None
+    return geo, Q
+
+
+@pytest.mark.parametrize("scheme,compat", [(O.ROE, O.COMPAT_REFERENCE), (O.AUSM, O.COMPAT_GRAD_LASTINT)])
+def test_sod_converges_to_exact_solution(scheme, compat):
+    errs = []
+    for nx in (200, 400):
+        geo, Q = run_sod(nx, scheme=scheme, compat=compat)
+        ex = sod_exact(geo["ccen"][:, 0], 0.2)
+        errs.append(np.abs(Q[:, 0] - ex[:, 0]).mean())
+    assert errs[1] < 0.02 and errs[1] < 0.75 * errs[0]
+
+
+def test_reference_ausm_is_not_consistent():
+    """The reference's AUSM scales the convective flux by a/a~ (SolverAusm.cpp:112-113): its Sod error
+    does not fall with resolution (the scheme converges to a different solution)."""
+    errs = []
+    for nx in (200, 400):
+        geo, Q = run_sod(nx, scheme=O.AUSM, compat=O.COMPAT_REFERENCE)
+        ex = sod_exact(geo["ccen"][:, 0], 0.2)
+        errs.append(np.abs(Q[:, 0] - ex[:, 0]).mean())
+    assert errs[1] > 0.95 * errs[0] and errs[1] > 0.005
+
+
+def test_literal_branch2_orientation_is_downwind():
+    """Every bundled Fluent mesh is "branch 2"; with the literal flags the Riemann states are swapped
+    (downwind) and Sod breaks down, while the physical rule is unaffected by the face orientation."""
+    geo, Qok = run_sod(200, flip=True, t_end=0.05)
+    _, Qref = run_sod(200, flip=False, t_end=0.05)
+    assert Qok is not None and np.abs(Qok - Qref).max() <= 1e-12
+    _, Qbad = run_sod(200, compat=O.COMPAT_REFERENCE | O.COMPAT_BRANCH2_LR, flip=True, t_end=0.05)
+    assert Qbad is None or np.abs(Qbad - Qref).max() > 0.1
