@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="vortex2d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="first launch one plain copy kernel of BYTES (scale for hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); the multi-GPU path is a "
@@ -84,6 +86,10 @@ def main():
         sys.exit(2)
     import mstcfd as M
 
+    copy = None
+    if args.calibrate_copy:
+        ms = M.copy_probe(args.calibrate_copy)
+        copy = dict(bytes=args.calibrate_copy, ms=ms, gbs=2 * args.calibrate_copy / (ms * 1e-3) / 1e9)
     mesh, cfg_kw, desc = build_workload(M, args.workload)
     solver = M.Solver(mesh, M.make_config(**cfg_kw))
     geo = solver.prep.geometry()
@@ -122,6 +128,8 @@ def main():
                       frac=achieved / HBM_PEAK_GBS, traffic=None),
         kernels=kern,
     )
+    if copy:
+        out["copy_probe"] = copy
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(mesh, cfg_kw, Q0, threads)

@@ -189,6 +189,9 @@ int mst_timing_enable(mst_ctx *ctx, int on);
 int mst_timing_get(mst_ctx *ctx, int kernel, int64_t *count, double *total_ms);
 int mst_timing_reset(mst_ctx *ctx);
 int mst_sync(mst_ctx *ctx);
+/* measurement helper (no reference counterpart): one launch of a plain device copy of nbytes
+ * (read nbytes + write nbytes), the calibration point for hardware byte counters; ms = its time */
+int mst_copy_probe(int64_t nbytes, double *ms);
 /* algorithmic (compulsory) HBM bytes of one launch of each kernel class for this mesh/config */
 int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes);
 

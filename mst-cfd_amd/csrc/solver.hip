@@ -102,6 +102,17 @@ __device__ __forceinline__ void reflect(double *dst, const double *src, const do
     for (int d = 0; d < DIM; d++) dst[1 + d] = src[1 + d] - 2 * nh[d] * s;
 }
 
+// 1/x from the hardware reciprocal estimate refined by two Newton steps (the same seed the
+// compiler's IEEE division expands from, without the scale/fixup stages): within 1 ulp for the
+// positive, normal arguments it is used on (densities, speeds, volumes)
+__device__ __forceinline__ double frcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 __device__ __forceinline__ double hfix(double x, const DevParams &P) {  // SolverRoe.cpp:114-123
     return x > P.delta ? x : (x * x + P.dd) * P.ihd;
 }
@@ -115,7 +126,7 @@ template <int DIM>
 __device__ __forceinline__ void roe_face(const double *A, const double *B, const double *n, int mask,
                                          const DevParams &P, double *g) {
     constexpr int NV = DIM + 2;
-    const double iA = 1.0 / A[0], iB = 1.0 / B[0];
+    const double iA = frcp(A[0]), iB = frcp(B[0]);
     double mA2 = 0, mB2 = 0;
 #pragma unroll
     for (int k = 0; k < DIM; k++) {
@@ -126,7 +137,7 @@ __device__ __forceinline__ void roe_face(const double *A, const double *B, const
     const double pB = (B[NV - 1] - 0.5 * mB2 * iB) * P.gm1;
     const double HA = (A[NV - 1] + pA) * iA, HB = (B[NV - 1] + pB) * iB;
     const double D = sqrt(fabs(B[0] * iA));
-    const double i1D = 1.0 / (1.0 + D);
+    const double i1D = frcp(1.0 + D);
     double u[DIM], q2 = 0;
 #pragma unroll
     for (int k = 0; k < DIM; k++) {
@@ -136,7 +147,7 @@ __device__ __forceinline__ void roe_face(const double *A, const double *B, const
     const double Ht = (HA + D * HB) * i1D;
     const double h2 = 0.5 * q2;
     const double a = sqrt(fabs(P.gm1 * (Ht - h2)));
-    const double ia = 1.0 / a, iden = 1.0 / (Ht - h2);
+    const double ia = frcp(a), iden = 1.0 / (Ht - h2);
     const double dr = B[0] - A[0], dE = B[NV - 1] - A[NV - 1];
     double beta[DIM], udm = 0;
 #pragma unroll
@@ -147,7 +158,7 @@ __device__ __forceinline__ void roe_face(const double *A, const double *B, const
     }
     const double s = (dE - udm + h2 * dr) * iden;  // alpha(-) + alpha(+)
     const double a0 = dr - s;                      // entropy wave strength
-    const double ieA = 1.0 / (A[0] + P.eor), ieB = 1.0 / (B[0] + P.eor);
+    const double ieA = frcp(A[0] + P.eor), ieB = frcp(B[0] + P.eor);
 #pragma unroll
     for (int k = 0; k < NV; k++) g[k] = 0;
 #pragma unroll
@@ -195,7 +206,7 @@ template <int DIM>
 __device__ __forceinline__ void ausm_face(const double *A, const double *B, const double *n, int mask,
                                           const DevParams &P, double *g) {
     constexpr int NV = DIM + 2;
-    const double iA = 1.0 / A[0], iB = 1.0 / B[0];
+    const double iA = frcp(A[0]), iB = frcp(B[0]);
     double mA2 = 0, mB2 = 0;
 #pragma unroll
     for (int k = 0; k < DIM; k++) {
@@ -207,9 +218,9 @@ __device__ __forceinline__ void ausm_face(const double *A, const double *B, cons
     const double HA = (A[NV - 1] + pA) * iA, HB = (B[NV - 1] + pB) * iB;
     const double asA = sqrt(HA * P.astar_c), asB = sqrt(HB * P.astar_c);
     const double UA = sqrt(mA2 * iA * iA), UB = sqrt(mB2 * iB * iB);
-    const double tA = asA * asA / fmax(asA, UA), tB = asB * asB / fmax(asB, UB);
+    const double tA = asA * asA * frcp(fmax(asA, UA)), tB = asB * asB * frcp(fmax(asB, UB));
     const double aF = fmin(tA, tB);
-    const double iaF = 1.0 / aF;
+    const double iaF = frcp(aF);
     double aA = aF, aB = aF;
     if (P.compat & MST_COMPAT_AUSM_LOCAL_A) {
         aA = sqrt(P.gamma * pA * iA);
@@ -374,11 +385,6 @@ struct GradArgs {
     double *__restrict__ phi;
 };
 
-__device__ __forceinline__ double psi(int lim, double r) {
-    if (lim == MST_LIM_MINMOD) return r < 0 ? 0.0 : (r < 1 ? r : 1.0);
-    return (r + fabs(r)) / (1 + fabs(r));
-}
-
 // RhoSolver::updateGradFlux (RhoSolver.cpp:430-454) fused with the (new) per-cell limiter
 template <int DIM, int MAXF, int LIM>
 __global__ __launch_bounds__(BLOCK) void k_grad(const GradArgs a) {
@@ -430,7 +436,7 @@ __global__ __launch_bounds__(BLOCK) void k_grad(const GradArgs a) {
             for (int d = 0; d < DIM; d++) t[k][d] += qf[k] * n[d];
         nfc = s + 1;
     }
-    const double iV = 1.0 / a.vol[c];
+    const double iV = frcp(a.vol[c]);
     double xc[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; d++) xc[d] = a.cxc[c * DIM + d];
@@ -442,6 +448,14 @@ __global__ __launch_bounds__(BLOCK) void k_grad(const GradArgs a) {
         for (int d = 0; d < DIM; d++) t[k][d] *= iV;
     }
     if constexpr (LIM != MST_LIM_NONE) {
+        // Phi = min(1, min_f psi(r_f)) = min(1, psi(min_f r_f)) since psi is increasing: the smallest
+        // ratio r = |a|/|delta| is tracked as a fraction (bn/bd) and psi is applied once per field
+        double bn[NV], bd[NV];
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            bn[k] = 1.0;
+            bd[k] = 0.0;  // "infinite" ratio: no face limits
+        }
 #pragma unroll
         for (int s = 0; s < MAXF; s++) {
             if (s >= nfc) break;
@@ -453,13 +467,21 @@ __global__ __launch_bounds__(BLOCK) void k_grad(const GradArgs a) {
                 double delta = t[k][0] * dx[0] + t[k][1] * dx[1];
                 if constexpr (DIM == 3) delta += t[k][2] * dx[2];
                 const double eps = 1e-12 * (fabs(qmax[k]) + fabs(qmin[k]));
-                double p = 1;
-                if (delta > eps)
-                    p = psi(LIM, (qmax[k] - qc[k]) / delta);
-                else if (delta < -eps)
-                    p = psi(LIM, (qmin[k] - qc[k]) / delta);
-                ph[k] = fmin(ph[k], p);
+                const double ext = delta > 0 ? qmax[k] : qmin[k];
+                const double an = fabs(ext - qc[k]), ad = fabs(delta);
+                if (fabs(delta) > eps && (bd[k] == 0.0 || an * bd[k] < bn[k] * ad)) {
+                    bn[k] = an;
+                    bd[k] = ad;
+                }
             }
+        }
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            if (bd[k] == 0.0) continue;
+            if constexpr (LIM == MST_LIM_MINMOD)
+                ph[k] = bn[k] < bd[k] ? bn[k] / bd[k] : 1.0;
+            else
+                ph[k] = fmin(1.0, 2.0 * bn[k] / (bd[k] + bn[k]));  // (r+|r|)/(1+|r|) at r = bn/bd >= 0
         }
     }
     double out[NV * DIM];
@@ -489,28 +511,75 @@ struct UpdArgs {
     const double *__restrict__ Qs;
     const double *__restrict__ Qn;
     const double *__restrict__ vol;
-    const double *__restrict__ dt;
+    const double *__restrict__ sh;  // half sums of |S_d| per cell (CFL)
+    double *dt;                     // [0] dt of this stage; rewritten with the next step's dt (UPD_DT)
     double ca, cb;
     int stage0;
-    double gm1;
+    double gamma, cfl;
     double *__restrict__ Qout;
     double *__restrict__ R;
-    double *__restrict__ part;  // per-block partial sums of (R/V)^2, nv per block
-    int *__restrict__ bad;      // set to 1 when an updated state is not physical
+    double *__restrict__ part_l2;   // per-block partial sums of (R/V)^2: [field][block]
+    double *__restrict__ part_dt;   // per-block minimum CFL time step
+    int *__restrict__ bad;          // set to 1 when an updated state is not physical
 };
 
-enum { UPD_RESIDUAL = 0, UPD_STAGE = 1, UPD_STAGE_L2 = 2 };
+// mode bits: UPD_STAGE writes the stage update (else only R), UPD_L2 reduces |R/V|^2,
+// UPD_DT reduces the CFL time step of the updated state (for the next step)
+enum { UPD_RESIDUAL = 0, UPD_STAGE = 1, UPD_L2 = 2, UPD_DT = 4 };
+
+// Block-level reduction of W values (fixed tree order): sums, or a min when MIN is set; thread 0
+// of the block holds the result.  The per-block partials are combined by k_finish in block order,
+// so every reduction is bit-reproducible (no float atomics, no cross-block fences).
+template <int W, bool MIN>
+__device__ __forceinline__ void block_partial(double (&v)[W], double *__restrict__ out, int64_t stride) {
+    __shared__ double sh[W][BLOCK];
+#pragma unroll
+    for (int k = 0; k < W; k++) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int k = 0; k < W; k++) {
+                const double y = sh[k][threadIdx.x + w];
+                sh[k][threadIdx.x] = MIN ? fmin(sh[k][threadIdx.x], y) : sh[k][threadIdx.x] + y;
+            }
+        }
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < W) out[threadIdx.x * stride] = sh[threadIdx.x][0];
+}
+
+template <int DIM>
+__device__ __forceinline__ double local_dt(const double *q, const double *__restrict__ sh, int64_t c, double V,
+                                           double gamma, double cfl) {
+    constexpr int NV = DIM + 2;
+    double ke = 0;
+#pragma unroll
+    for (int k = 1; k <= DIM; k++) ke += q[k] * q[k];
+    const double p = (q[NV - 1] - 0.5 * ke / q[0]) * (gamma - 1);
+    const double snd = sqrt(gamma * p / q[0]);
+    double lam = 0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) lam += (fabs(q[1 + d] / q[0]) + snd) * sh[c * DIM + d];
+    const double v = cfl * V / lam;
+    return v == v ? v : -1.0;  // a NaN state reports a negative time step
+}
 
 // RhoSolver::solve gather + update (RhoSolver.cpp:45-68), SSP-RK stage form (new)
 template <int NV, int MAXF, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_update(const UpdArgs a) {
+    constexpr int DIM = NV - 2;
+    constexpr bool STAGE = MODE & UPD_STAGE, L2 = MODE & UPD_L2, DT = MODE & UPD_DT;
     const int64_t blk = block_id();
     const int64_t c = blk * BLOCK + threadIdx.x;
     const bool valid = c < a.nc;
     double r[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) r[k] = 0;
-    double l2[NV];
+    double red[NV + 1];
+#pragma unroll
+    for (int k = 0; k < NV; k++) red[k] = 0;
+    red[NV] = INFINITY;
     if (valid) {
 #pragma unroll
         for (int s = 0; s < MAXF; s++) {
@@ -526,7 +595,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(const UpdArgs a) {
                 for (int k = 0; k < NV; k++) r[k] += gf[k];
             }
         }
-        if constexpr (MODE == UPD_RESIDUAL) {
+        if constexpr (!STAGE) {
             stq<NV>(a.R + c * NV, r);
             return;
         } else {
@@ -547,33 +616,28 @@ __global__ __launch_bounds__(BLOCK) void k_update(const UpdArgs a) {
             double ke = 0;
 #pragma unroll
             for (int k = 1; k < NV - 1; k++) ke += out[k] * out[k];
-            const double p = (out[NV - 1] - 0.5 * ke / out[0]) * a.gm1;
+            const double p = (out[NV - 1] - 0.5 * ke / out[0]) * (a.gamma - 1);
             if (!(out[0] > 0) || !(p > 0)) a.bad[0] = 1;
-            if constexpr (MODE == UPD_STAGE_L2) {
-                const double iV = 1.0 / V;
+            if constexpr (L2) {
+                const double iV = frcp(V);
 #pragma unroll
                 for (int k = 0; k < NV; k++) {
                     const double x = r[k] * iV;
-                    l2[k] = x * x;
+                    red[k] = x * x;
                 }
             }
+            if constexpr (DT) red[NV] = local_dt<DIM>(out, a.sh, c, V, a.gamma, a.cfl);
         }
-    } else {
-#pragma unroll
-        for (int k = 0; k < NV; k++) l2[k] = 0;
     }
-    if constexpr (MODE == UPD_STAGE_L2) {
-        __shared__ double sh[NV][BLOCK];
+    if constexpr (STAGE && L2) {
+        double v[NV];
 #pragma unroll
-        for (int k = 0; k < NV; k++) sh[k][threadIdx.x] = l2[k];
-        __syncthreads();
-        for (int w = BLOCK / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w)
-#pragma unroll
-                for (int k = 0; k < NV; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x < NV) a.part[blk * NV + threadIdx.x] = sh[threadIdx.x][0];
+        for (int k = 0; k < NV; k++) v[k] = red[k];
+        block_partial<NV, false>(v, a.part_l2 + blk, gridDim.x);
+    }
+    if constexpr (STAGE && DT) {
+        double v[1] = {red[NV]};
+        block_partial<1, true>(v, a.part_dt + blk, 1);
     }
 }
 
@@ -582,33 +646,17 @@ __global__ __launch_bounds__(BLOCK) void k_update(const UpdArgs a) {
 template <int DIM>
 __global__ __launch_bounds__(BLOCK) void k_cfl(int64_t nc, const double *__restrict__ Q,
                                                const double *__restrict__ vol, const double *__restrict__ sh,
-                                               double gamma, double cfl, double *__restrict__ part) {
+                                               double gamma, double cfl, double *__restrict__ part_dt) {
     constexpr int NV = DIM + 2;
     const int64_t blk = block_id();
     const int64_t c = blk * BLOCK + threadIdx.x;
-    double v = INFINITY;
+    double v[1] = {INFINITY};
     if (c < nc) {
         double q[NV];
         ldq<NV>(Q + c * NV, q);
-        double ke = 0;
-#pragma unroll
-        for (int k = 1; k <= DIM; k++) ke += q[k] * q[k];
-        const double p = (q[NV - 1] - 0.5 * ke / q[0]) * (gamma - 1);
-        const double snd = sqrt(gamma * p / q[0]);
-        double lam = 0;
-#pragma unroll
-        for (int d = 0; d < DIM; d++) lam += (fabs(q[1 + d] / q[0]) + snd) * sh[c * DIM + d];
-        v = cfl * vol[c] / lam;
-        if (!(v == v)) v = -1.0;  // NaN state: report a negative dt
+        v[0] = local_dt<DIM>(q, sh, c, vol[c], gamma, cfl);
     }
-    __shared__ double s[BLOCK];
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int w = BLOCK / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) s[threadIdx.x] = fmin(s[threadIdx.x], s[threadIdx.x + w]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blk] = s[0];
+    block_partial<1, true>(v, part_dt + blk, 1);
 }
 
 // Time::goNextTimeStep monitor (time/Time.cpp:69-76): max |new-old|/old, values that do not compare
@@ -642,36 +690,59 @@ __global__ __launch_bounds__(BLOCK) void k_maxrel(int64_t nc, const double *__re
                 if (s[k][threadIdx.x] < s[k][threadIdx.x + w]) s[k][threadIdx.x] = s[k][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x < NV) part[blk * NV + threadIdx.x] = s[threadIdx.x][0];
+    if (threadIdx.x < NV) part[threadIdx.x * (int64_t)gridDim.x + blk] = s[threadIdx.x][0];
 }
 
-// one-block finish of per-block partials, fixed order: op 0 sum, 1 min, 2 max; out[k] (k < width)
-// post: 0 none, 1 sqrt(sum / count)
-__global__ __launch_bounds__(BLOCK) void k_finish(const double *__restrict__ part, int64_t nblk, int width, int op,
-                                                  int post, double count, double *__restrict__ out) {
-    __shared__ double s[BLOCK];
-    for (int k = 0; k < width; k++) {
-        double acc = op == 0 ? 0.0 : (op == 1 ? INFINITY : 0.0);
-        for (int64_t i = threadIdx.x; i < nblk; i += BLOCK) {
-            const double x = part[i * width + k];
-            if (op == 0) acc += x;
-            else if (op == 1) acc = fmin(acc, x);
-            else if (acc < x) acc = x;
-        }
-        s[threadIdx.x] = acc;
-        __syncthreads();
-        for (int w = BLOCK / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) {
-                const double y = s[threadIdx.x + w];
-                if (op == 0) s[threadIdx.x] += y;
-                else if (op == 1) s[threadIdx.x] = fmin(s[threadIdx.x], y);
-                else if (s[threadIdx.x] < y) s[threadIdx.x] = y;
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[k] = post == 1 ? sqrt(s[0] / count) : s[0];
-        __syncthreads();
+// One-block finish of the per-block partials.  Slot k of the W slots is a sum (kind 0), a min
+// (kind 1) or a max (kind 2); every thread walks the blocks with a fixed stride, then a fixed
+// butterfly inside each wave and a fixed pass over the waves, so the result is bit-reproducible.
+constexpr int FIN = 1024;
+struct FinArgs {
+    const double *src[12];
+    int kind[12];
+    double *dst[12];
+    int post[12];  // 1: sqrt(v / count)
+    int n;
+    int64_t nblk;
+    double count;
+};
+
+__device__ __forceinline__ double fin_op(int kind, double a, double b) {
+    return kind == 0 ? a + b : (kind == 1 ? fmin(a, b) : (a < b ? b : a));
+}
+
+__global__ __launch_bounds__(FIN) void k_finish(const FinArgs a) {
+    constexpr int MAXW = 12;
+    __shared__ double s[MAXW][FIN / 64];
+    double acc[MAXW];
+#pragma unroll
+    for (int k = 0; k < MAXW; k++) acc[k] = (k < a.n && a.kind[k] == 1) ? INFINITY : 0.0;
+    for (int64_t i = threadIdx.x; i < a.nblk; i += FIN) {
+#pragma unroll
+        for (int k = 0; k < MAXW; k++)
+            if (k < a.n) acc[k] = fin_op(a.kind[k], acc[k], a.src[k][i]);
     }
+#pragma unroll
+    for (int k = 0; k < MAXW; k++) {
+        if (k < a.n) {
+            for (int m = 32; m > 0; m >>= 1) acc[k] = fin_op(a.kind[k], acc[k], __shfl_xor(acc[k], m, 64));
+            if ((threadIdx.x & 63) == 0) s[k][threadIdx.x >> 6] = acc[k];
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < a.n) {
+        const int k = threadIdx.x;
+        double v = s[k][0];
+        for (int w = 1; w < FIN / 64; w++) v = fin_op(a.kind[k], v, s[k][w]);
+        a.dst[k][0] = a.post[k] ? sqrt(v / a.count) : v;
+    }
+}
+
+// plain streaming copy: the reference point that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes
+__global__ __launch_bounds__(BLOCK) void k_copy(int64_t n2, const double2 *__restrict__ src,
+                                                double2 *__restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n2) dst[i] = src[i];
 }
 
 // batched flux-scheme plugin: F = solverAll(dir) after set(QL, QR)
@@ -741,6 +812,7 @@ struct mst_ctx {
     double *G = nullptr, *lg = nullptr, *graw = nullptr, *phi = nullptr, *R = nullptr;
     double *scal = nullptr;  // [0] dt, [1..5] l2 norms, [6..10] maxrel
     double *part = nullptr;
+    bool dt_valid = false;            // scal[0] holds the CFL time step of the current Old state
     int *bad = nullptr;
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[MST_K_COUNT];
@@ -853,19 +925,27 @@ int run_update_t(mst_ctx *x, int mode, const double *Qs, const double *Qn, doubl
     a.Qs = Qs;
     a.Qn = Qn;
     a.vol = x->cvol;
+    a.sh = x->cshalf;
     a.dt = x->scal;
     a.ca = ca;
     a.cb = cb;
     a.stage0 = stage0;
-    a.gm1 = x->cfg.gamma - 1.0;
+    a.gamma = x->cfg.gamma;
+    a.cfl = x->cfg.cfl;
     a.Qout = Qout;
     a.R = x->R;
-    a.part = x->part;
+    a.part_l2 = x->part;
+    a.part_dt = x->part + x->nblk_c * x->nv;
     a.bad = x->bad;
-#define MST_UPD(MF)                                                                                       \
-    if (mode == UPD_RESIDUAL) return launch(x, MST_K_UPDATE, k_update<NV, MF, UPD_RESIDUAL>, x->nc, a);    \
-    if (mode == UPD_STAGE) return launch(x, MST_K_UPDATE, k_update<NV, MF, UPD_STAGE>, x->nc, a);          \
-    return launch(x, MST_K_UPDATE, k_update<NV, MF, UPD_STAGE_L2>, x->nc, a);
+#define MST_UPD(MF)                                                                                          \
+    switch (mode) {                                                                                          \
+        case UPD_RESIDUAL: return launch(x, MST_K_UPDATE, k_update<NV, MF, UPD_RESIDUAL>, x->nc, a);         \
+        case UPD_STAGE: return launch(x, MST_K_UPDATE, k_update<NV, MF, UPD_STAGE>, x->nc, a);               \
+        case UPD_STAGE | UPD_L2: return launch(x, MST_K_UPDATE, k_update<NV, MF, UPD_STAGE | UPD_L2>, x->nc, a); \
+        case UPD_STAGE | UPD_DT: return launch(x, MST_K_UPDATE, k_update<NV, MF, UPD_STAGE | UPD_DT>, x->nc, a); \
+        default:                                                                                             \
+            return launch(x, MST_K_UPDATE, k_update<NV, MF, UPD_STAGE | UPD_L2 | UPD_DT>, x->nc, a);         \
+    }
     if (x->maxf <= 3) { MST_UPD(3) }
     if (x->maxf <= 4) { MST_UPD(4) }
     MST_UPD(8)
@@ -878,18 +958,40 @@ int run_update(mst_ctx *x, int mode, const double *Qs, const double *Qn, double 
                       : run_update_t<5>(x, mode, Qs, Qn, ca, cb, stage0, Qout);
 }
 
-int finish_l2(mst_ctx *x) {
-    return launch(x, -1, k_finish, BLOCK, (const double *)x->part, x->nblk_c, x->nv, 0, 1, (double)x->nc,
-                  x->scal + 1);
+// one finish launch: L2 norms (nl2 fields) and/or the minimum time step, from the per-block partials
+int run_finish(mst_ctx *x, int nl2, int want_dt) {
+    FinArgs f;
+    memset(&f, 0, sizeof(f));
+    const int64_t nb = x->nblk_c;
+    f.nblk = nb;
+    f.count = (double)x->nc;
+    for (int k = 0; k < nl2; k++) {
+        f.src[f.n] = x->part + k * nb;
+        f.kind[f.n] = 0;
+        f.dst[f.n] = x->scal + 1 + k;
+        f.post[f.n] = 1;
+        f.n++;
+    }
+    if (want_dt) {
+        f.src[f.n] = x->part + x->nv * nb;
+        f.kind[f.n] = 1;
+        f.dst[f.n] = x->scal;
+        f.post[f.n] = 0;
+        f.n++;
+    }
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(FIN), 0, x->stream, f);
+    HIPCHK(hipGetLastError());
+    return MST_OK;
 }
 
 int run_cfl(mst_ctx *x, const double *Qsrc, double cfl) {
+    double *pdt = x->part + x->nblk_c * x->nv;
     int rc = x->dim == 2 ? launch(x, MST_K_DT, k_cfl<2>, x->nc, x->nc, Qsrc, (const double *)x->cvol,
-                                  (const double *)x->cshalf, x->cfg.gamma, cfl, x->part)
+                                  (const double *)x->cshalf, x->cfg.gamma, cfl, pdt)
                          : launch(x, MST_K_DT, k_cfl<3>, x->nc, x->nc, Qsrc, (const double *)x->cvol,
-                                  (const double *)x->cshalf, x->cfg.gamma, cfl, x->part);
+                                  (const double *)x->cshalf, x->cfg.gamma, cfl, pdt);
     if (rc) return rc;
-    return launch(x, -1, k_finish, BLOCK, (const double *)x->part, x->nblk_c, 1, 1, 0, 1.0, x->scal);
+    return run_finish(x, 0, 1);
 }
 
 int upload_config(mst_ctx *x) {
@@ -1038,7 +1140,7 @@ int mst_ctx_create(const mst_mesh *m, const mst_config *cfg, mst_ctx **out) {
     TRY(dalloc(&x->G, x->nf * x->nv));
     TRY(dalloc(&x->lg, x->nc * x->nv * x->dim));
     TRY(dalloc(&x->scal, 16));
-    TRY(dalloc(&x->part, std::max(x->nblk_c, x->nblk_f) * x->nv + 16));
+    TRY(dalloc(&x->part, std::max(x->nblk_c, x->nblk_f) * (x->nv + 1) + 16));
     TRY(dalloc(&x->bad, 1));
     if (hipMemset(x->bad, 0, sizeof(int)) != hipSuccess || hipMemset(x->scal, 0, 16 * sizeof(double)) != hipSuccess ||
         hipMemset(x->lg, 0, sizeof(double) * x->nc * x->nv * x->dim) != hipSuccess) {
@@ -1064,6 +1166,7 @@ int mst_ctx_set_config(mst_ctx *ctx, const mst_config *cfg) {
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->cfg = *cfg;
+    ctx->dt_valid = false;
     return upload_config(ctx);
 }
 
@@ -1076,6 +1179,7 @@ int mst_set_state(mst_ctx *ctx, const double *Q, int64_t n_cells) {
     permute_in(ctx, Q, buf);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(ctx->Q[ctx->iold], buf.data(), sizeof(double) * buf.size(), hipMemcpyHostToDevice));
+    ctx->dt_valid = false;
     return MST_OK;
 }
 
@@ -1096,6 +1200,7 @@ int mst_set_dt(mst_ctx *ctx, double dt) {
     if (!(dt > 0)) return mst::fail(MST_E_ARG, "dt must be positive");
     HIPCHK(hipMemcpyAsync(ctx->scal, &dt, sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->dt_valid = false;
     return MST_OK;
 }
 
@@ -1131,14 +1236,16 @@ int mst_step(mst_ctx *ctx, int nsteps) {
     static const double A3[3] = {0, 0.75, 1.0 / 3.0}, B3[3] = {1, 0.25, 2.0 / 3.0};
     const int ns = ctx->cfg.rk;
     const double *A = ns == 3 ? A3 : A2, *B = ns == 3 ? B3 : B2;
+    const bool cfl = ctx->cfg.cfl > 0;
     for (int it = 0; it < nsteps; it++) {
         int rc;
         double *Qn = ctx->Q[ctx->iold];
-        if (ctx->cfg.cfl > 0) {
+        if (cfl && !ctx->dt_valid) {
             rc = run_cfl(ctx, Qn, ctx->cfg.cfl);
             if (rc) return rc;
         }
-        // stage buffers: the two spare buffers alternate, the last stage lands in "new"
+        // stage buffers: the two spare buffers alternate, the last stage lands in "new"; the first
+        // stage reduces |R/V|, the last one the CFL time step of the new state (next step's dt)
         int spare[2], w = 0;
         for (int i = 0; i < 4; i++)
             if (i != ctx->iold && i != ctx->inew) spare[w++] = i;
@@ -1147,14 +1254,14 @@ int mst_step(mst_ctx *ctx, int nsteps) {
             double *dst = (s == ns - 1) ? ctx->Q[ctx->inew] : ctx->Q[spare[s & 1]];
             rc = run_face_flux(ctx, src);
             if (rc) return rc;
-            rc = run_update(ctx, s == 0 ? UPD_STAGE_L2 : UPD_STAGE, src, Qn, A[s], B[s], s == 0 ? 1 : 0, dst);
+            int mode = UPD_STAGE | (s == 0 ? UPD_L2 : 0) | ((cfl && s == ns - 1) ? UPD_DT : 0);
+            rc = run_update(ctx, mode, src, Qn, A[s], B[s], s == 0 ? 1 : 0, dst);
             if (rc) return rc;
-            if (s == 0) {
-                rc = finish_l2(ctx);
-                if (rc) return rc;
-            }
             src = dst;
         }
+        rc = run_finish(ctx, ctx->nv, cfl ? 1 : 0);
+        if (rc) return rc;
+        ctx->dt_valid = cfl;
         std::swap(ctx->iold, ctx->inew);
     }
     return check_bad(ctx);
@@ -1167,6 +1274,7 @@ int mst_compute_dt(mst_ctx *ctx, double cfl, double *dt) {
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(dt, ctx->scal, sizeof(double), hipMemcpyDeviceToHost));
+    ctx->dt_valid = cfl == ctx->cfg.cfl;
     if (!(*dt > 0)) return mst::fail(MST_E_STATE, "non-physical state: no positive CFL time step");
     return MST_OK;
 }
@@ -1187,7 +1295,18 @@ int mst_monitor_maxrel(mst_ctx *ctx, double *out) {
                           : launch(ctx, -1, k_maxrel<5>, ctx->nc, ctx->nc, (const double *)ctx->Q[ctx->inew],
                                    (const double *)ctx->Q[ctx->iold], ctx->part);
     if (rc) return rc;
-    rc = launch(ctx, -1, k_finish, BLOCK, (const double *)ctx->part, ctx->nblk_c, ctx->nv, 2, 0, 1.0, ctx->scal + 6);
+    FinArgs f;
+    memset(&f, 0, sizeof(f));
+    f.nblk = ctx->nblk_c;
+    f.count = 1.0;
+    for (int k = 0; k < ctx->nv; k++) {
+        f.src[k] = ctx->part + k * ctx->nblk_c;
+        f.kind[k] = 2;
+        f.dst[k] = ctx->scal + 6 + k;
+    }
+    f.n = ctx->nv;
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(FIN), 0, ctx->stream, f);
+    HIPCHK(hipGetLastError());
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(out, ctx->scal + 6, sizeof(double) * ctx->nv, hipMemcpyDeviceToHost));
@@ -1282,6 +1401,35 @@ int mst_riemann_flux(const mst_config *cfg, int dim, int64_t n, const double *QL
     (void)hipFree(dF);
     (void)hipFree(dd);
     if (e != hipSuccess) return mst::fail(MST_E_HIP, std::string("riemann kernel: ") + hipGetErrorString(e));
+    return MST_OK;
+}
+
+int mst_copy_probe(int64_t nbytes, double *ms) {
+    if (nbytes < 16 || nbytes % 16) return mst::fail(MST_E_ARG, "nbytes must be a positive multiple of 16");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return mst::fail(MST_E_HIP, "no HIP device available");
+    double2 *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc((void **)&a, nbytes));
+    HIPCHK(hipMalloc((void **)&b, nbytes));
+    HIPCHK(hipMemset(a, 0, nbytes));
+    const int64_t n2 = nbytes / 16;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipEventRecord(e0, 0));
+    hipLaunchKernelGGL(k_copy, dim3((uint32_t)((n2 + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, 0, n2,
+                       (const double2 *)a, b);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e1, 0));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
     return MST_OK;
 }
 

@@ -32,7 +32,7 @@ EXPORTS = [
     "mst_get_state", "mst_get_new_state", "mst_set_dt", "mst_face_flux", "mst_residual_update", "mst_solve",
     "mst_update_new_to_old", "mst_step", "mst_compute_dt", "mst_residual_norms", "mst_monitor_maxrel",
     "mst_get_residual", "mst_get_face_flux", "mst_get_gradient", "mst_riemann_flux", "mst_timing_enable",
-    "mst_timing_get", "mst_timing_reset", "mst_sync", "mst_algorithmic_bytes",
+    "mst_timing_get", "mst_timing_reset", "mst_sync", "mst_algorithmic_bytes", "mst_copy_probe",
 ]
 
 
@@ -132,6 +132,7 @@ def lib():
         "mst_timing_reset": (C.c_int, [vp]),
         "mst_sync": (C.c_int, [vp]),
         "mst_algorithmic_bytes": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double)]),
+        "mst_copy_probe": (C.c_int, [C.c_int64, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -367,6 +368,13 @@ class Solver:
         b = C.c_double()
         _check(lib().mst_algorithmic_bytes(self.h, int(kernel), C.byref(b)), "mst_algorithmic_bytes")
         return b.value
+
+
+def copy_probe(nbytes):
+    """One launch of a plain device copy of nbytes; returns its duration in ms."""
+    ms = C.c_double()
+    _check(lib().mst_copy_probe(int(nbytes), C.byref(ms)), "mst_copy_probe")
+    return ms.value
 
 
 def riemann_flux(cfg, QL, QR, dirs, dim=2):
