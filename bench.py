@@ -39,14 +39,62 @@ def vortex_state(ccen, beta=5.0, mach=0.5, gamma=1.4):
     return np.ascontiguousarray(Q)
 
 
+def pulse_state(ccen, amp=0.1, sigma=0.1, mach=0.8, gamma=1.4):
+    """3D Gaussian density/pressure pulse at the box centre on a Mach-0.8 x-flow (rho=1, p=1/gamma)."""
+    r2 = ((ccen - 0.5) ** 2).sum(1)
+    b = amp * np.exp(-r2 / (2 * sigma ** 2))
+    rho, p, u = 1 + b, (1 + b) / gamma, mach
+    E = p / (gamma - 1) + 0.5 * rho * u * u
+    return np.ascontiguousarray(np.stack([rho, rho * u, 0 * rho, 0 * rho, E], axis=1))
+
+
 def build_workload(M, name):
     if name == "vortex2d":
         mesh = M.Mesh.gen_tri(1000, 1000, -10.0, 10.0, -10.0, 10.0, 0.1, 12345, (M.BC_PERIODIC,) * 4)
         cfg = dict(scheme=M.ROE, order=2, limiter=M.LIM_VANLEER, rk=3, cfl=0.5)
         desc = dict(workload="isentropic vortex 2D, 1000x1000x2 jittered triangles, periodic",
                     cells=mesh.n_cells, faces=mesh.n_faces, scheme="Roe", limiter="van Leer", rk=3, cfl=0.5)
-        return mesh, cfg, desc
+        return mesh, cfg, desc, vortex_state
+    if name == "pulse3d":
+        bc = (M.BC_INLET, M.BC_OUTLET) + (M.BC_SYMMETRY,) * 4
+        mesh = M.Mesh.gen_tet(160, 160, 160, (0, 0, 0), (1, 1, 1), 0.15, 12345, bc)
+        E = (1 / 1.4) / 0.4 + 0.5 * 0.8 * 0.8
+        cfg = dict(scheme=M.AUSM, order=2, limiter=M.LIM_MINMOD, rk=2, cfl=0.5, dim=3,
+                   q_inlet=[1.0, 0.8, 0.0, 0.0, E])
+        desc = dict(workload="3D Gaussian pulse in a Mach-0.8 box flow, 160^3 hexes x 6 jittered tets",
+                    cells=mesh.n_cells, faces=mesh.n_faces, scheme="AUSM", limiter="minmod", rk=2, cfl=0.5)
+        return mesh, cfg, desc, pulse_state
     raise SystemExit(f"unknown workload {name}")
+
+
+def measure(M, solver, steps, warmup):
+    """Run warmup steps, then time exactly `steps` steps (stream-synchronised on both sides) with the
+    per-kernel HIP events recorded on the solver's stream."""
+    solver.step(warmup)
+    solver.sync()
+    solver.timing_reset()
+    solver.timing(True)
+    solver.sync()
+    t0 = time.perf_counter()
+    solver.step(steps)
+    solver.sync()
+    t1 = time.perf_counter()
+    solver.timing(False)
+    kern = {}
+    for k, name in M.KERNEL_NAMES.items():
+        n, ms = solver.timing_get(k)
+        if n:
+            b = solver.algorithmic_bytes(k)
+            kern[name] = dict(launches=n, avg_us=1e3 * ms / n, total_ms=ms, alg_bytes=b,
+                              gbs=b / (1e-3 * ms / n) / 1e9)
+    return t1 - t0, kern
+
+
+def field_errors(Qm, Qd):
+    sc = np.abs(Qd).max(0)
+    sc[1:-1] = sc[1:-1].max()
+    d = np.abs(Qm - Qd) / sc
+    return np.sqrt((d ** 2).mean(0)).tolist(), d.max(0).tolist()
 
 
 def cpu_baseline(mesh, cfg_kw, Q, threads, budget_s=10.0):
@@ -77,6 +125,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="vortex2d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison")
+    ap.add_argument("--compare-steps", type=int, default=200,
+                    help="steps of the FP64 vs mixed state comparison")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="first launch one plain copy kernel of BYTES (scale for hardware byte counters)")
     args = ap.parse_args()
@@ -90,35 +141,17 @@ def main():
     if args.calibrate_copy:
         ms = M.copy_probe(args.calibrate_copy)
         copy = dict(bytes=args.calibrate_copy, ms=ms, gbs=2 * args.calibrate_copy / (ms * 1e-3) / 1e9)
-    mesh, cfg_kw, desc = build_workload(M, args.workload)
+    mesh, cfg_kw, desc, init = build_workload(M, args.workload)
     solver = M.Solver(mesh, M.make_config(**cfg_kw))
     geo = solver.prep.geometry()
-    Q0 = vortex_state(geo["ccen"])
+    Q0 = init(geo["ccen"])
     solver.set_state(Q0)
-    solver.step(args.warmup)
-    solver.sync()
-    solver.timing_reset()
-    solver.timing(True)
-    solver.sync()
-    t0 = time.perf_counter()
-    solver.step(args.steps)
-    solver.sync()
-    t1 = time.perf_counter()
-    solver.timing(False)
-    wall = t1 - t0
+    wall, kern = measure(M, solver, args.steps, args.warmup)
     stages = cfg_kw["rk"]
     value = mesh.n_faces * stages * args.steps / wall
-    kern = {}
-    for k, name in M.KERNEL_NAMES.items():
-        n, ms = solver.timing_get(k)
-        if n:
-            kern[name] = dict(launches=n, avg_us=1e3 * ms / n, total_ms=ms,
-                              alg_bytes=solver.algorithmic_bytes(k))
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     d = kern[dom]
     achieved = d["alg_bytes"] / (d["avg_us"] * 1e-6) / 1e9
-    for k in kern:
-        kern[k]["gbs"] = kern[k]["alg_bytes"] / (kern[k]["avg_us"] * 1e-6) / 1e9
     out = dict(
         metric="face-flux updates/s (FP64 residual path); %HBM peak of the dominant kernel",
         value=value, unit="face-updates/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
@@ -130,6 +163,33 @@ def main():
     )
     if copy:
         out["copy_probe"] = copy
+    if not args.no_mixed:
+        # mixed form on the same context: throughput, then the state difference after `compare_steps`
+        # steps of both forms from the same initial field with the same dt sequence
+        solver.set_config(M.make_config(precision=M.PREC_MIXED, **cfg_kw))
+        solver.set_state(Q0)
+        wall_m, kern_m = measure(M, solver, args.steps, args.warmup)
+        value_m = mesh.n_faces * stages * args.steps / wall_m
+        nsteps = args.compare_steps
+        fixed = dict(cfg_kw, cfl=0.0)
+        solver.set_config(M.make_config(**fixed))
+        solver.set_state(Q0)
+        dts = []
+        for _ in range(nsteps):
+            dts.append(solver.compute_dt(cfg_kw["cfl"]))
+            solver.set_dt(dts[-1])
+            solver.step(1)
+        Qd = solver.get_state()
+        solver.set_config(M.make_config(precision=M.PREC_MIXED, **fixed))
+        solver.set_state(Q0)
+        for dt in dts:
+            solver.set_dt(dt)
+            solver.step(1)
+        l2, linf = field_errors(solver.get_state(), Qd)
+        out["mixed"] = dict(value=value_m, ms_per_step=1e3 * wall_m / args.steps, speedup=value_m / value,
+                            dtype="fp32 flux+limiter / fp64 gather+update", kernels=kern_m,
+                            compare_steps=nsteps, rel_l2_per_field=l2, rel_linf_per_field=linf)
+        solver.set_config(M.make_config(**cfg_kw))
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(mesh, cfg_kw, Q0, threads)

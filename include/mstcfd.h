@@ -44,6 +44,9 @@ extern "C" {
 #define MST_LIM_MINMOD 1
 #define MST_LIM_VANLEER 2
 
+#define MST_PREC_FP64 0
+#define MST_PREC_MIXED 1
+
 /* reference behaviours behind switches (defaults: MST_COMPAT_REFERENCE) */
 #define MST_COMPAT_BRANCH2_LR 1      /* literal inverted L/R of mesh/MshBlock.cpp:294-303 */
 #define MST_COMPAT_GRAD_LASTINT 2    /* rhoSolver/RhoSolver.cpp:437-440 loop start N_int-1 */
@@ -59,7 +62,7 @@ typedef struct {
     int32_t limiter;       /* MST_LIM_* (order 2) */
     int32_t rk;            /* 1 forward Euler (time/Time.cpp:62-65), 2 SSP-RK2, 3 SSP-RK3 */
     int32_t compat;        /* MST_COMPAT_* */
-    int32_t reserved;      /* must be 0 */
+    int32_t precision;     /* MST_PREC_FP64 (plain) or MST_PREC_MIXED (FP32 flux/limiter, FP64 gather+update) */
     double gamma;          /* 1.4 */
     double eor;            /* 1e-10 */
     double entropy_delta;  /* 0.125 (rhoSolver/SolverRoe.cpp:115) */

@@ -1,0 +1,563 @@
+// kernels.hpp -- gfx950 kernels of the density-based explicit residual path.
+//
+// Per residual evaluation (one RK stage), on the context's stream:
+//   k_grad    (order 2)  one lane per cell: Green-Gauss gradient + limiter -> limited gradient
+//   k_flux               one lane per face: reconstruct, ghost state, dimension-split Roe/AUSM,
+//                        G_f = sum_d n_d F_d  (c0 outward normal)
+//   k_update             one lane per cell: R_c = sum +-G_f over its faces in a fixed order (no float
+//                        atomics -> bit-reproducible), then the Euler / SSP-RK stage update
+// plus k_cfl (CFL time step) and one-block reductions (k_finish) of per-block partials.
+//
+// Precision forms: T = double is the plain FP64 path (the reference's arithmetic); T = float is the
+// mixed form: gradient, limiter and Riemann arithmetic in FP32 on an FP32 shadow of the state and FP32
+// limited gradients; the per-axis fluxes are projected on the FP64 normals into FP64 face fluxes (so a
+// uniform state cancels exactly around a closed cell), then a compensated FP64 gather and the FP64
+// state update.  A periodic pair is one face (its second cell sees the partner face's centre), so the
+// two sides always exchange exactly opposite fluxes.
+#pragma once
+#include <type_traits>
+
+#include "device_common.hpp"
+#include "prep.hpp"
+#include "riemann.hpp"
+
+namespace mstk {
+
+// ------------------------------------------------------------------------------------ face flux
+template <typename T>
+struct FluxArgs {
+    int64_t nf, pstart;
+    const int32_t *__restrict__ fc0;
+    const int32_t *__restrict__ fc1;
+    const int32_t *__restrict__ finfo;  // kind | (L/R mask << 4)
+    const double *__restrict__ fn;      // c0 outward normal
+    // FP64: face centre, partner face centre (periodic) and cell centres, as the reference
+    const double *__restrict__ fxf;
+    const double *__restrict__ pxf;
+    const double *__restrict__ cxc;
+    // FP32: offsets x_f - x_c0 and x_f' - x_c1, formed in FP64 on the host
+    const float *__restrict__ fd0;
+    const float *__restrict__ fd1;
+    const T *__restrict__ Q;
+    const T *__restrict__ lg;
+    double *__restrict__ G;
+    Params<T> P;
+};
+
+// RhoSolver::updateFaceFlux (RhoSolver.cpp:90-233) / updateFaceFlux2ndOrder (:234-369), one lane per face
+template <int DIM, int SCHEME, bool SECOND, typename T>
+__global__ __launch_bounds__(BLOCK) void k_flux(const FluxArgs<T> a) {
+    constexpr int NV = DIM + 2;
+    constexpr bool OFFS = std::is_same<T, float>::value;
+    const int64_t f = block_id() * BLOCK + threadIdx.x;
+    if (f >= a.nf) return;
+    const int info = a.finfo[f];
+    const int kind = info & 15, mask = info >> 4;
+    double g[NV];
+    if (kind >= mst::KIND_NONE) {  // unknown type (zero flux) or the merged half of a periodic pair
+#pragma unroll
+        for (int k = 0; k < NV; k++) g[k] = 0;
+        stq<NV>(a.G + f * NV, g);
+        return;
+    }
+    const int64_t c0 = a.fc0[f];
+    double n64[DIM];
+    T n[DIM], A[NV], B[NV];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        n64[d] = a.fn[f * DIM + d];
+        n[d] = (T)n64[d];
+    }
+    ldq<NV>(a.Q + c0 * NV, A);
+    T dx0[DIM];
+    if constexpr (SECOND) {
+#pragma unroll
+        for (int d = 0; d < DIM; d++)
+            dx0[d] = OFFS ? (T)a.fd0[f * DIM + d] : (T)(a.fxf[f * DIM + d] - a.cxc[c0 * DIM + d]);
+    }
+    if (kind == mst::KIND_INTERIOR || kind == mst::KIND_PERIODIC) {
+        const int64_t c1 = a.fc1[f];
+        ldq<NV>(a.Q + c1 * NV, B);
+        if constexpr (SECOND) {
+            T dx1[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                if constexpr (OFFS) {
+                    dx1[d] = (T)a.fd1[f * DIM + d];
+                } else {
+                    const double xf = kind == mst::KIND_PERIODIC ? a.pxf[(f - a.pstart) * DIM + d] : a.fxf[f * DIM + d];
+                    dx1[d] = (T)(xf - a.cxc[c1 * DIM + d]);
+                }
+            }
+            recon<DIM>(A, a.lg + c0 * NV * DIM, dx0);
+            recon<DIM>(B, a.lg + c1 * NV * DIM, dx1);
+        }
+    } else {
+        T Rc[NV];
+#pragma unroll
+        for (int k = 0; k < NV; k++) Rc[k] = A[k];
+        if constexpr (SECOND) recon<DIM>(Rc, a.lg + c0 * NV * DIM, dx0);
+        if (kind == mst::KIND_WALL) {  // RhoSolver.cpp:142-179 / 285-319
+#pragma unroll
+            for (int k = 0; k < NV; k++) A[k] = B[k] = Rc[k];
+            reflect<DIM>(B, A, n);
+        } else if (kind == mst::KIND_SYMMETRY) {  // :184-211 / 322-350
+#pragma unroll
+            for (int k = 0; k < NV; k++) B[k] = Rc[k];
+            reflect<DIM>(B, A, n);
+        } else if (kind == mst::KIND_INLET) {  // :121-137 / 265-281
+#pragma unroll
+            for (int k = 0; k < NV; k++) {
+                A[k] = Rc[k];
+                B[k] = a.P.qin[k];
+            }
+        } else {  // outlet :216-226 / 353-363
+#pragma unroll
+            for (int k = 0; k < NV; k++) B[k] = Rc[k];
+        }
+    }
+    riemann_face<DIM, SCHEME, T>(A, B, n64, mask, a.P, g);
+    stq<NV>(a.G + f * NV, g);
+}
+
+// ------------------------------------------------------------------------------------ gradient + limiter
+template <typename T>
+struct GradArgs {
+    int64_t nc, pstart, pcount;
+    const double *__restrict__ pxf;  // FP64 form: partner face centres of the periodic pairs
+    const int32_t *__restrict__ ell;
+    const int32_t *__restrict__ fc0;
+    const int32_t *__restrict__ fc1;
+    const T *__restrict__ fn;
+    const T *__restrict__ feta;
+    const double *__restrict__ fxf;  // FP64 form: face centres and cell centres
+    const double *__restrict__ cxc;
+    const float *__restrict__ fd0;   // FP32 form: offsets x_f - x_c0, x_f' - x_c1
+    const float *__restrict__ fd1;
+    const T *__restrict__ Q;
+    const double *__restrict__ vol;
+    T *__restrict__ lg;
+    double *__restrict__ graw;
+    double *__restrict__ phi;
+};
+
+// RhoSolver::updateGradFlux (RhoSolver.cpp:430-454) fused with the (new) per-cell limiter.
+// FP64: grad = (1/V) sum_f Q_f n_out as the reference; FP32: the same sum written with differences,
+// (1/V) sum_f w_f (Q_nb - Q_c) n_out (w = 1-eta on the c0 side, eta on the c1 side), equal because
+// the outward normals of a closed cell sum to zero, and free of the cancellation an FP32 sum of
+// whole states would suffer.
+template <int DIM, int MAXF, int LIM, typename T>
+__global__ __launch_bounds__(BLOCK) void k_grad(const GradArgs<T> a) {
+    constexpr int NV = DIM + 2;
+    constexpr bool F32 = std::is_same<T, float>::value;
+    const int64_t c = block_id() * BLOCK + threadIdx.x;
+    if (c >= a.nc) return;
+    T qc[NV], qmax[NV], qmin[NV], t[NV][DIM];
+    ldq<NV>(a.Q + c * NV, qc);
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        qmax[k] = qmin[k] = qc[k];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) t[k][d] = 0;
+    }
+    T dxs[MAXF][DIM];
+    double xf[MAXF][DIM];
+    int nfc = 0;
+#pragma unroll
+    for (int s = 0; s < MAXF; s++) {
+        const int32_t e = a.ell[(int64_t)s * a.nc + c];
+        if (e < 0) break;
+        const int64_t f = e >> 1;
+        const bool is1 = e & 1;
+        const int64_t nb = is1 ? a.fc0[f] : a.fc1[f];
+        T n[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            const T v = a.fn[f * DIM + d];
+            n[d] = is1 ? -v : v;
+            if constexpr (F32)
+                dxs[s][d] = (T)(is1 ? a.fd1[f * DIM + d] : a.fd0[f * DIM + d]);
+            else  // the c1 side of a periodic pair sees the partner face's centre
+                xf[s][d] = (is1 && f >= a.pstart && f < a.pstart + a.pcount) ? a.pxf[(f - a.pstart) * DIM + d]
+                                                                             : a.fxf[f * DIM + d];
+        }
+        T qf[NV];
+        if (nb >= 0) {
+            T qn[NV];
+            ldq<NV>(a.Q + nb * NV, qn);
+            const T eta = a.feta[f];
+            if constexpr (F32) {
+                const T w = is1 ? eta : T(1) - eta;
+#pragma unroll
+                for (int k = 0; k < NV; k++) qf[k] = w * (qn[k] - qc[k]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < NV; k++) qf[k] = is1 ? eta * qn[k] + (1 - eta) * qc[k] : eta * qc[k] + (1 - eta) * qn[k];
+            }
+#pragma unroll
+            for (int k = 0; k < NV; k++) {
+                qmax[k] = fmax(qmax[k], qn[k]);
+                qmin[k] = fmin(qmin[k], qn[k]);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < NV; k++) qf[k] = F32 ? T(0) : qc[k];
+        }
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+#pragma unroll
+            for (int d = 0; d < DIM; d++) t[k][d] += qf[k] * n[d];
+        nfc = s + 1;
+    }
+    const T iV = (T)frcp(a.vol[c]);
+    if constexpr (!F32) {
+#pragma unroll
+        for (int s = 0; s < MAXF; s++) {
+            if (s >= nfc) break;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) dxs[s][d] = (T)(xf[s][d] - a.cxc[c * DIM + d]);
+        }
+    }
+    T ph[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        ph[k] = 1;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) t[k][d] *= iV;
+    }
+    if constexpr (LIM != MST_LIM_NONE) {
+        // Phi = min(1, min_f psi(r_f)) = min(1, psi(min_f r_f)) since psi is increasing: the smallest
+        // ratio r = |a|/|delta| is tracked as a fraction (bn/bd) and psi is applied once per field
+        T bn[NV], bd[NV];
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            bn[k] = 1;
+            bd[k] = 0;  // "infinite" ratio: no face limits
+        }
+#pragma unroll
+        for (int s = 0; s < MAXF; s++) {
+            if (s >= nfc) break;
+#pragma unroll
+            for (int k = 0; k < NV; k++) {
+                T delta = t[k][0] * dxs[s][0] + t[k][1] * dxs[s][1];
+                if constexpr (DIM == 3) delta += t[k][2] * dxs[s][2];
+                const T eps = T(1e-12) * (fabs(qmax[k]) + fabs(qmin[k]));
+                const T ext = delta > 0 ? qmax[k] : qmin[k];
+                const T an = fabs(ext - qc[k]), ad = fabs(delta);
+                if (fabs(delta) > eps && (bd[k] == T(0) || an * bd[k] < bn[k] * ad)) {
+                    bn[k] = an;
+                    bd[k] = ad;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            if (bd[k] == T(0)) continue;
+            if constexpr (LIM == MST_LIM_MINMOD)
+                ph[k] = bn[k] < bd[k] ? fdiv(bn[k], bd[k]) : T(1);
+            else
+                ph[k] = fmin(T(1), fdiv(T(2) * bn[k], bd[k] + bn[k]));  // (r+|r|)/(1+|r|) at r = bn/bd >= 0
+        }
+    }
+    T out[NV * DIM];
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+#pragma unroll
+        for (int d = 0; d < DIM; d++) out[k * DIM + d] = ph[k] * t[k][d];
+#pragma unroll
+    for (int i = 0; i < NV * DIM; i++) a.lg[c * NV * DIM + i] = out[i];
+    if (a.graw) {
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+#pragma unroll
+            for (int d = 0; d < DIM; d++) a.graw[c * NV * DIM + k * DIM + d] = (double)t[k][d];
+    }
+    if (a.phi) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) a.phi[c * NV + k] = (double)ph[k];
+    }
+}
+
+// ------------------------------------------------------------------------------------ gather + update
+struct UpdArgs {
+    int64_t nc;
+    const int32_t *__restrict__ ell;
+    const double *__restrict__ G;
+    const double *__restrict__ Qs;
+    const double *__restrict__ Qn;
+    const double *__restrict__ vol;
+    const double *__restrict__ sh;  // half sums of |S_d| per cell (CFL)
+    const double *__restrict__ dt;  // [0] dt of this stage
+    double ca, cb;
+    int stage0;
+    double gamma, cfl;
+    double *__restrict__ Qout;
+    float *__restrict__ Qout32;     // FP32 shadow of Qout (mixed form), else null
+    double *__restrict__ R;
+    double *__restrict__ part_l2;   // per-block partial sums of (R/V)^2: [field][block]
+    double *__restrict__ part_dt;   // per-block minimum CFL time step
+    int *__restrict__ bad;          // set to 1 when an updated state is not physical
+};
+
+// mode bits: UPD_STAGE writes the stage update (else only R), UPD_L2 reduces |R/V|^2,
+// UPD_DT reduces the CFL time step of the updated state (for the next step)
+enum { UPD_RESIDUAL = 0, UPD_STAGE = 1, UPD_L2 = 2, UPD_DT = 4 };
+
+// Block-level reduction of W values (fixed tree order): sums, or a min when MIN is set; thread 0..W-1
+// of the block write the results.  The per-block partials are combined by k_finish in block order,
+// so every reduction is bit-reproducible (no float atomics, no cross-block fences).
+template <int W, bool MIN>
+__device__ __forceinline__ void block_partial(double (&v)[W], double *__restrict__ out, int64_t stride) {
+    __shared__ double sh[W][BLOCK];
+#pragma unroll
+    for (int k = 0; k < W; k++) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int k = 0; k < W; k++) {
+                const double y = sh[k][threadIdx.x + w];
+                sh[k][threadIdx.x] = MIN ? fmin(sh[k][threadIdx.x], y) : sh[k][threadIdx.x] + y;
+            }
+        }
+        __syncthreads();
+    }
+    if ((int)threadIdx.x < W) out[threadIdx.x * stride] = sh[threadIdx.x][0];
+}
+
+template <int DIM>
+__device__ __forceinline__ double local_dt(const double *q, const double *__restrict__ sh, int64_t c, double V,
+                                           double gamma, double cfl) {
+    constexpr int NV = DIM + 2;
+    double ke = 0;
+#pragma unroll
+    for (int k = 1; k <= DIM; k++) ke += q[k] * q[k];
+    const double p = (q[NV - 1] - 0.5 * ke / q[0]) * (gamma - 1);
+    const double snd = sqrt(gamma * p / q[0]);
+    double lam = 0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) lam += (fabs(q[1 + d] / q[0]) + snd) * sh[c * DIM + d];
+    const double v = cfl * V / lam;
+    return v == v ? v : -1.0;  // a NaN state reports a negative time step
+}
+
+// RhoSolver::solve gather + update (RhoSolver.cpp:45-68), SSP-RK stage form (new).  In the mixed form
+// (F32) the face fluxes from FP32 Riemann solves are summed with a compensation term (Neumaier) and
+// an FP32 shadow of the updated state is written for the next stage's FP32 kernels.
+template <int NV, int MAXF, int MODE, bool F32>
+__global__ __launch_bounds__(BLOCK) void k_update(const UpdArgs a) {
+    constexpr int DIM = NV - 2;
+    constexpr bool STAGE = MODE & UPD_STAGE, L2 = MODE & UPD_L2, DT = MODE & UPD_DT;
+    const int64_t blk = block_id();
+    const int64_t c = blk * BLOCK + threadIdx.x;
+    const bool valid = c < a.nc;
+    double r[NV], cmp[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) r[k] = cmp[k] = 0;
+    double red[NV + 1];
+#pragma unroll
+    for (int k = 0; k < NV; k++) red[k] = 0;
+    red[NV] = INFINITY;
+    if (valid) {
+#pragma unroll
+        for (int s = 0; s < MAXF; s++) {
+            const int32_t e = a.ell[(int64_t)s * a.nc + c];
+            if (e < 0) break;
+            double gf[NV];
+            ldq<NV>(a.G + (int64_t)(e >> 1) * NV, gf);
+#pragma unroll
+            for (int k = 0; k < NV; k++) {
+                const double x = (e & 1) ? -gf[k] : gf[k];
+                if constexpr (F32) {  // Neumaier: keep the rounding error of every addition
+                    const double t = r[k] + x;
+                    cmp[k] += fabs(r[k]) >= fabs(x) ? (r[k] - t) + x : (x - t) + r[k];
+                    r[k] = t;
+                } else {
+                    r[k] += x;
+                }
+            }
+        }
+        if constexpr (F32) {
+#pragma unroll
+            for (int k = 0; k < NV; k++) r[k] += cmp[k];
+        }
+        if constexpr (!STAGE) {
+            stq<NV>(a.R + c * NV, r);
+            return;
+        } else {
+            const double V = a.vol[c];
+            const double dtv = a.dt[0] / V;
+            double qs[NV], out[NV];
+            ldq<NV>(a.Qs + c * NV, qs);
+            if (a.stage0) {
+#pragma unroll
+                for (int k = 0; k < NV; k++) out[k] = qs[k] - dtv * r[k];
+            } else {
+                double qn[NV];
+                ldq<NV>(a.Qn + c * NV, qn);
+#pragma unroll
+                for (int k = 0; k < NV; k++) out[k] = a.ca * qn[k] + a.cb * (qs[k] - dtv * r[k]);
+            }
+            stq<NV>(a.Qout + c * NV, out);
+            if constexpr (F32) stq<NV>(a.Qout32 + c * NV, out);
+            double ke = 0;
+#pragma unroll
+            for (int k = 1; k < NV - 1; k++) ke += out[k] * out[k];
+            const double p = (out[NV - 1] - 0.5 * ke / out[0]) * (a.gamma - 1);
+            if (!(out[0] > 0) || !(p > 0)) a.bad[0] = 1;
+            if constexpr (L2) {
+                const double iV = frcp(V);
+#pragma unroll
+                for (int k = 0; k < NV; k++) {
+                    const double x = r[k] * iV;
+                    red[k] = x * x;
+                }
+            }
+            if constexpr (DT) red[NV] = local_dt<DIM>(out, a.sh, c, V, a.gamma, a.cfl);
+        }
+    }
+    if constexpr (STAGE && L2) {
+        double v[NV];
+#pragma unroll
+        for (int k = 0; k < NV; k++) v[k] = red[k];
+        block_partial<NV, false>(v, a.part_l2 + blk, gridDim.x);
+    }
+    if constexpr (STAGE && DT) {
+        double v[1] = {red[NV]};
+        block_partial<1, true>(v, a.part_dt + blk, 1);
+    }
+}
+
+// ------------------------------------------------------------------------------------ reductions
+// CFL: dt_c = cfl V / sum_d (|u_d| + a) Shalf_d (new; the reference's DT is fixed, time/Time.cpp:62)
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_cfl(int64_t nc, const double *__restrict__ Q,
+                                               const double *__restrict__ vol, const double *__restrict__ sh,
+                                               double gamma, double cfl, double *__restrict__ part_dt) {
+    constexpr int NV = DIM + 2;
+    const int64_t blk = block_id();
+    const int64_t c = blk * BLOCK + threadIdx.x;
+    double v[1] = {INFINITY};
+    if (c < nc) {
+        double q[NV];
+        ldq<NV>(Q + c * NV, q);
+        v[0] = local_dt<DIM>(q, sh, c, vol[c], gamma, cfl);
+    }
+    block_partial<1, true>(v, part_dt + blk, 1);
+}
+
+// Time::goNextTimeStep monitor (time/Time.cpp:69-76): max |new-old|/old, values that do not compare
+// greater are skipped
+template <int NV>
+__global__ __launch_bounds__(BLOCK) void k_maxrel(int64_t nc, const double *__restrict__ Qnew,
+                                                  const double *__restrict__ Qold, double *__restrict__ part) {
+    const int64_t blk = block_id();
+    const int64_t c = blk * BLOCK + threadIdx.x;
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = 0;
+    if (c < nc) {
+        double a[NV], b[NV];
+        ldq<NV>(Qnew + c * NV, a);
+        ldq<NV>(Qold + c * NV, b);
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            const double x = fabs(a[k] - b[k]) / b[k];
+            if (v[k] < x) v[k] = x;
+        }
+    }
+    __shared__ double s[NV][BLOCK];
+#pragma unroll
+    for (int k = 0; k < NV; k++) s[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+#pragma unroll
+            for (int k = 0; k < NV; k++)
+                if (s[k][threadIdx.x] < s[k][threadIdx.x + w]) s[k][threadIdx.x] = s[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < NV) part[threadIdx.x * (int64_t)gridDim.x + blk] = s[threadIdx.x][0];
+}
+
+// One-block finish of the per-block partials.  Slot k is a sum (kind 0), a min (kind 1) or a max
+// (kind 2); every thread walks the blocks with a fixed stride, then a fixed butterfly inside each
+// wave and a fixed pass over the waves, so the result is bit-reproducible.
+constexpr int FIN = 1024;
+struct FinArgs {
+    const double *src[12];
+    int kind[12];
+    double *dst[12];
+    int post[12];  // 1: sqrt(v / count)
+    int n;
+    int64_t nblk;
+    double count;
+};
+
+__device__ __forceinline__ double fin_op(int kind, double a, double b) {
+    return kind == 0 ? a + b : (kind == 1 ? fmin(a, b) : (a < b ? b : a));
+}
+
+__global__ __launch_bounds__(FIN) void k_finish(const FinArgs a) {
+    constexpr int MAXW = 12;
+    __shared__ double s[MAXW][FIN / 64];
+    double acc[MAXW];
+#pragma unroll
+    for (int k = 0; k < MAXW; k++) acc[k] = (k < a.n && a.kind[k] == 1) ? INFINITY : 0.0;
+    for (int64_t i = threadIdx.x; i < a.nblk; i += FIN) {
+#pragma unroll
+        for (int k = 0; k < MAXW; k++)
+            if (k < a.n) acc[k] = fin_op(a.kind[k], acc[k], a.src[k][i]);
+    }
+#pragma unroll
+    for (int k = 0; k < MAXW; k++) {
+        if (k < a.n) {
+            for (int m = 32; m > 0; m >>= 1) acc[k] = fin_op(a.kind[k], acc[k], __shfl_xor(acc[k], m, 64));
+            if ((threadIdx.x & 63) == 0) s[k][threadIdx.x >> 6] = acc[k];
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < a.n) {
+        const int k = threadIdx.x;
+        double v = s[k][0];
+        for (int w = 1; w < FIN / 64; w++) v = fin_op(a.kind[k], v, s[k][w]);
+        a.dst[k][0] = a.post[k] ? sqrt(v / a.count) : v;
+    }
+}
+
+// FP32 shadow of a state (mixed form), after the host writes the FP64 state
+__global__ __launch_bounds__(BLOCK) void k_shadow(int64_t n, const double *__restrict__ Q, float *__restrict__ Q32) {
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) Q32[i] = (float)Q[i];
+}
+
+// plain streaming copy: the reference point that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes
+__global__ __launch_bounds__(BLOCK) void k_copy(int64_t n2, const double2 *__restrict__ src,
+                                                double2 *__restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n2) dst[i] = src[i];
+}
+
+// batched flux-scheme plugin: F = solverAll(dir) after set(QL, QR)
+template <int DIM, int SCHEME>
+__global__ __launch_bounds__(BLOCK) void k_riemann(int64_t n, const double *__restrict__ QL,
+                                                   const double *__restrict__ QR, const int32_t *__restrict__ dir,
+                                                   double *__restrict__ F, Params<double> P) {
+    constexpr int NV = DIM + 2;
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double A[NV], B[NV], nrm[DIM], g[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        A[k] = QL[i * NV + k];
+        B[k] = QR[i * NV + k];
+    }
+    const int d0 = dir[i];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) nrm[d] = d == d0 ? 1.0 : 0.0;
+    riemann_face<DIM, SCHEME, double>(A, B, nrm, (1 << DIM) - 1, P, g);
+#pragma unroll
+    for (int k = 0; k < NV; k++) F[i * NV + k] = g[k];
+}
+
+}  // namespace mstk

@@ -307,7 +307,9 @@ int prepare(const mst_mesh &m, Prepared &P) {
             k1 = std::min(a, b);
             k2 = std::max(a, b);
         } else {
-            k0 = 1 + kind_of(m.ftype[f]);
+            int32_t kd = kind_of(m.ftype[f]);
+            if (kd == KIND_PERIODIC && m.partner[f] >= 0 && m.partner[f] < f) kd = KIND_MERGED;
+            k0 = 1 + kd;
             k1 = a;
             k2 = 0;
         }
@@ -344,10 +346,14 @@ int prepare(const mst_mesh &m, Prepared &P) {
             P.fc1[i] = P.cell_new_of_old[m.c1[f]];
             k = KIND_INTERIOR;
             P.n_interior = i + 1;
-        } else if (k == KIND_PERIODIC && m.partner[f] >= 0) {
+        } else if (k == KIND_PERIODIC && m.partner[f] >= 0 && m.partner[f] > f) {
+            // a periodic pair is one face: c0 on this side, c1 = the partner face's cell
             P.fc1[i] = P.cell_new_of_old[m.c0[m.partner[f]]];
             if (P.pstart < 0) P.pstart = i;
             P.pcount++;
+        } else if (k == KIND_PERIODIC && m.partner[f] >= 0) {
+            P.fc1[i] = -1;
+            k = KIND_MERGED;
         } else {
             P.fc1[i] = -1;
             if (k == KIND_PERIODIC || k == KIND_INTERIOR) k = KIND_NONE;
@@ -362,6 +368,9 @@ int prepare(const mst_mesh &m, Prepared &P) {
     }
     if (P.pstart < 0) P.pstart = 0;
     if (P.pcount > 0 && P.pstart + P.pcount > nf) return fail(MST_E_MESH, "periodic faces not contiguous");
+    P.fmerged_into.assign(nf, -1);
+    for (int64_t i = 0; i < nf; i++)
+        if (P.fkind[i] == KIND_MERGED) P.fmerged_into[i] = P.face_new_of_old[m.partner[fo[i]]];
     P.pxf.resize(P.pcount * dim);
     for (int64_t i = 0; i < P.pcount; i++) {
         int32_t f = fo[P.pstart + i];
@@ -384,11 +393,14 @@ int prepare(const mst_mesh &m, Prepared &P) {
     }
     P.cshalf.swap(sh);
 
-    // ---- cell -> face slots (new ids, ascending); periodic faces only on their own cell
+    // ---- cell -> face slots (new ids, ascending); a periodic pair is listed on both of its cells
+    // through its first face, the merged second face on none
+    auto two_sided = [&](int64_t i) { return P.fkind[i] == KIND_INTERIOR || P.fkind[i] == KIND_PERIODIC; };
     std::vector<int32_t> cnt(nc, 0);
     for (int64_t i = 0; i < nf; i++) {
+        if (P.fkind[i] == KIND_MERGED) continue;
         cnt[P.fc0[i]]++;
-        if (P.fkind[i] == KIND_INTERIOR) cnt[P.fc1[i]]++;
+        if (two_sided(i)) cnt[P.fc1[i]]++;
     }
     P.maxf = 0;
     for (int64_t c = 0; c < nc; c++) P.maxf = std::max(P.maxf, cnt[c]);
@@ -396,9 +408,10 @@ int prepare(const mst_mesh &m, Prepared &P) {
     P.ell.assign((int64_t)P.maxf * nc, -1);
     std::fill(cnt.begin(), cnt.end(), 0);
     for (int64_t i = 0; i < nf; i++) {
+        if (P.fkind[i] == KIND_MERGED) continue;
         int32_t a = P.fc0[i];
         P.ell[(int64_t)cnt[a]++ * nc + a] = (int32_t)(i << 1);
-        if (P.fkind[i] == KIND_INTERIOR) {
+        if (two_sided(i)) {
             int32_t b = P.fc1[i];
             P.ell[(int64_t)cnt[b]++ * nc + b] = (int32_t)((i << 1) | 1);
         }
@@ -410,13 +423,13 @@ int prepare(const mst_mesh &m, Prepared &P) {
     P.ncolours = 0;
     for (int64_t i = 0; i < nf; i++) {
         uint32_t u = used[P.fc0[i]];
-        if (P.fkind[i] == KIND_INTERIOR) u |= used[P.fc1[i]];
+        if (two_sided(i)) u |= used[P.fc1[i]];
         int col = 0;
         while (col < 32 && (u >> col) & 1u) col++;
         if (col >= 32) return fail(MST_E_UNSUPPORTED, "face colouring needs more than 32 colours");
         P.colour_new[i] = col;
         used[P.fc0[i]] |= 1u << col;
-        if (P.fkind[i] == KIND_INTERIOR) used[P.fc1[i]] |= 1u << col;
+        if (two_sided(i)) used[P.fc1[i]] |= 1u << col;
         P.ncolours = std::max(P.ncolours, col + 1);
     }
     return MST_OK;
@@ -428,7 +441,9 @@ void lr_masks(const Prepared &P, int compat, std::vector<int32_t> &mask) {
     for (int64_t i = 0; i < P.nf; i++) {
         int k = P.fkind[i];
         int32_t mk = 0;
-        if (k == KIND_OUTLET) {
+        if (k == KIND_MERGED) {
+            mk = 0;
+        } else if (k == KIND_OUTLET) {
             mk = (1 << dim) - 1;  // outlet: set(Q_c0 side first) in both directions (RhoSolver.cpp:219-224,358-361)
         } else if (k != KIND_NONE) {
             for (int d = 0; d < dim; d++) {

@@ -15,7 +15,8 @@ enum FaceKind : int32_t {
     KIND_INLET = 3,
     KIND_OUTLET = 4,
     KIND_PERIODIC = 5,
-    KIND_NONE = 6  // unknown Fluent type: the reference skips it, the flux stays zero
+    KIND_NONE = 6,   // unknown Fluent type: the reference skips it, the flux stays zero
+    KIND_MERGED = 7  // second face of a periodic pair: its flux is minus the first face's
 };
 
 struct Prepared {
@@ -45,6 +46,7 @@ struct Prepared {
     std::vector<double> fxf;            // nf*dim: face centre
     std::vector<double> feta;           // nf: eta0 (Face.cpp:62-69)
     std::vector<double> pxf;            // pcount*dim: centre of the partner face (periodic)
+    std::vector<int32_t> fmerged_into;  // nf: for KIND_MERGED faces the new index of their pair's first face
     std::vector<double> cvol, cxc, cshalf;  // nc, nc*dim, nc*dim (half sums of |S_d|)
     std::vector<int32_t> ell;           // maxf*nc, slot-major: (face << 1) | (cell is c1); -1 padding
 };

@@ -18,6 +18,7 @@ F64P = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 BC_INTERIOR, BC_WALL, BC_OUTLET, BC_SYMMETRY, BC_INLET, BC_PERIODIC = 2, 3, 5, 7, 10, 12
 ROE, AUSM = 0, 1
 LIM_NONE, LIM_MINMOD, LIM_VANLEER = 0, 1, 2
+PREC_FP64, PREC_MIXED = 0, 1
 COMPAT_BRANCH2_LR, COMPAT_GRAD_LASTINT, COMPAT_AUSM_LOCAL_A, COMPAT_AUSM_P_UNSCALED = 1, 2, 4, 8
 COMPAT_REFERENCE = COMPAT_GRAD_LASTINT | COMPAT_AUSM_LOCAL_A | COMPAT_AUSM_P_UNSCALED
 K_GRAD, K_FLUX, K_UPDATE, K_DT = 0, 1, 2, 3
@@ -49,7 +50,7 @@ class Config(C.Structure):
         ("limiter", C.c_int32),
         ("rk", C.c_int32),
         ("compat", C.c_int32),
-        ("reserved", C.c_int32),
+        ("precision", C.c_int32),
         ("gamma", C.c_double),
         ("eor", C.c_double),
         ("entropy_delta", C.c_double),
@@ -65,9 +66,9 @@ def reference_inlet(dim=2):
 
 
 def make_config(scheme=ROE, order=1, limiter=LIM_NONE, rk=1, compat=COMPAT_REFERENCE, gamma=1.4, eor=1e-10,
-                entropy_delta=0.125, cfl=0.0, q_inlet=None, dim=2):
+                entropy_delta=0.125, cfl=0.0, q_inlet=None, dim=2, precision=PREC_FP64):
     c = Config()
-    c.scheme, c.order, c.limiter, c.rk, c.compat, c.reserved = scheme, order, limiter, rk, compat, 0
+    c.scheme, c.order, c.limiter, c.rk, c.compat, c.precision = scheme, order, limiter, rk, compat, precision
     c.gamma, c.eor, c.entropy_delta, c.cfl = gamma, eor, entropy_delta, cfl
     q = list(reference_inlet(dim) if q_inlet is None else q_inlet)
     q = q + [0.0] * (5 - len(q))

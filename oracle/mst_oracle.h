@@ -44,7 +44,7 @@ typedef struct {
     int32_t limiter;   /* ORC_LIM_* (order 2 only) */
     int32_t rk;        /* 1 forward Euler, 2 SSP-RK2, 3 SSP-RK3 */
     int32_t compat;    /* ORC_COMPAT_* bits */
-    int32_t reserved;
+    int32_t precision; /* ignored: the oracle is the FP64 reference */
     double gamma;          /* 1.4 */
     double eor;            /* 1e-10: rho + EOR in Roe's momentum fluxes */
     double entropy_delta;  /* 0.125: Harten fix */

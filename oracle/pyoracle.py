@@ -31,7 +31,7 @@ class Config(C.Structure):
         ("limiter", C.c_int32),
         ("rk", C.c_int32),
         ("compat", C.c_int32),
-        ("reserved", C.c_int32),
+        ("precision", C.c_int32),
         ("gamma", C.c_double),
         ("eor", C.c_double),
         ("entropy_delta", C.c_double),

@@ -1,0 +1,107 @@
+"""Mixed-precision form: FP32 gradient/limiter/flux arithmetic, FP32 face fluxes, compensated FP64
+gather and FP64 update.  Checked against the FP64 oracle and the plain FP64 GPU path."""
+import math
+
+import numpy as np
+import pytest
+
+import mstcfd as M
+from exact import vortex_exact
+from helpers import cons, flux_scale, normal_sums, residual_parity_error, smooth_state
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def pulse_state(ccen, amp=0.1, sigma=0.1, mach=0.8, gamma=1.4):
+    """3D Gaussian density/pressure pulse at the box centre on a Mach-0.8 x-flow (rho=1, p=1/gamma)."""
+    r2 = ((ccen - 0.5) ** 2).sum(1)
+    b = amp * np.exp(-r2 / (2 * sigma ** 2))
+    rho = 1 + b
+    p = (1 + b) / gamma
+    vel = np.zeros((ccen.shape[0], 3))
+    vel[:, 0] = mach
+    return cons(rho, vel, p)
+
+
+def field_errors(Qm, Qd):
+    """relative L2 and Linf difference per field (scale: max |Q_fp64| of the field, momentum: rho(|u|+a))."""
+    sc = np.abs(Qd).max(0)
+    sc[1:-1] = sc[1:-1].max()
+    d = np.abs(Qm - Qd) / sc
+    return np.sqrt((d ** 2).mean(0)), d.max(0)
+
+
+@pytest.mark.parametrize("scheme,order,lim", [(M.ROE, 2, M.LIM_VANLEER), (M.AUSM, 2, M.LIM_MINMOD), (M.ROE, 1, 0)])
+def test_mixed_residual_close_to_oracle(scheme, order, lim):
+    m = M.Mesh.gen_tri(60, 50, -10, 10, -10, 10, 0.1, 3, (M.BC_PERIODIC,) * 4)
+    om = O.OracleMesh(m.raw())
+    geo = om.geometry()
+    Q = smooth_state(geo["ccen"], seed=3, mach=0.8)
+    kw = dict(scheme=scheme, order=order, limiter=lim)
+    s = M.Solver(m, M.make_config(precision=M.PREC_MIXED, **kw))
+    s.set_state(Q)
+    Rg = s.residual()
+    Rr, Sr = om.residual(O.make_config(**kw), Q)
+    err = residual_parity_error(Rg, Rr, Sr, flux_scale(Q) * normal_sums(m.raw(), geo["S"])[:, None])
+    print(f"mixed residual scheme={scheme} order={order} lim={lim}: err/S = {err:.2e}", flush=True)
+    assert 1e-9 < err <= 2e-6  # FP32 flux arithmetic: well above FP64 rounding, far below 1e-5
+
+
+def test_mixed_conserves_exactly_and_tracks_fp64():
+    """Each FP32 face flux enters both cells with opposite signs in FP64, so periodic totals stay
+    conserved to FP64 rounding; the state follows the FP64 run to FP32 accuracy."""
+    m = M.Mesh.gen_tri(200, 200, -10, 10, -10, 10, 0.1, 12345, (M.BC_PERIODIC,) * 4)
+    kw = dict(scheme=M.ROE, order=2, limiter=M.LIM_VANLEER, rk=3)
+    s64 = M.Solver(m, M.make_config(**kw))
+    s32 = M.Solver(m, M.make_config(precision=M.PREC_MIXED, **kw))
+    geo = s64.prep.geometry()
+    Q0 = vortex_exact(geo["ccen"])
+    for s in (s64, s32):
+        s.set_state(Q0)
+    for _ in range(50):
+        dt = s64.compute_dt(0.5)
+        for s in (s64, s32):
+            s.set_dt(dt)
+            s.step(1)
+    Qd, Qm = s64.get_state(), s32.get_state()
+    V = geo["vol"][:, None]
+    tot0, tot = (V * Q0).sum(0), (V * Qm).sum(0)
+    assert (np.abs(tot - tot0) <= 1e-13 * np.abs(V * Q0).sum(0)).all()
+    l2, linf = field_errors(Qm, Qd)
+    print(f"vortex 80k cells, 50 RK3 steps: mixed vs FP64 rel L2 {l2}, Linf {linf}", flush=True)
+    assert linf[0] < 1e-6
+
+
+def test_mixed_pulse3d_density_error():
+    """C3 construction at reduced size (20^3 hexes -> 48,000 tets): AUSM + minmod, RK2, 200 steps."""
+    bc = (M.BC_INLET, M.BC_OUTLET) + (M.BC_SYMMETRY,) * 4
+    m = M.Mesh.gen_tet(20, 20, 20, (0, 0, 0), (1, 1, 1), 0.15, 12345, bc)
+    qin = list(cons([1.0], [[0.8, 0, 0]], [1 / 1.4])[0])
+    kw = dict(scheme=M.AUSM, order=2, limiter=M.LIM_MINMOD, rk=2, q_inlet=qin, dim=3)
+    s64 = M.Solver(m, M.make_config(**kw))
+    s32 = M.Solver(m, M.make_config(precision=M.PREC_MIXED, **kw))
+    Q0 = pulse_state(s64.prep.geometry()["ccen"])
+    for s in (s64, s32):
+        s.set_state(Q0)
+    for _ in range(200):
+        dt = s64.compute_dt(0.5)
+        for s in (s64, s32):
+            s.set_dt(dt)
+            s.step(1)
+    l2, linf = field_errors(s32.get_state(), s64.get_state())
+    print(f"pulse3d 48k tets, 200 RK2 steps: mixed vs FP64 rel L2 {l2}, Linf {linf}", flush=True)
+    assert linf[0] <= 1e-6
+
+
+def test_mixed_freestream_and_switching():
+    m = M.Mesh.gen_tri(40, 40, -1, 1, -1, 1, 0.2, 4, (M.BC_PERIODIC,) * 4)
+    s = M.Solver(m, M.make_config(order=2, limiter=M.LIM_MINMOD, precision=M.PREC_MIXED))
+    q = cons([1.2], [[0.4, -0.3]], [0.9])[0]
+    s.set_state(np.tile(q, (m.n_cells, 1)))
+    R = s.residual()
+    assert np.abs(R).max() <= 1e-6 * 2.0 * (2 / 40) * 3  # FP32 normals: closure to ~1e-7
+    # switching the precision of a live context keeps the state and agrees with a fresh FP64 context
+    s.set_config(M.make_config(order=2, limiter=M.LIM_MINMOD))
+    R64 = s.residual()
+    assert np.abs(R64).max() <= 1e-13

@@ -79,7 +79,7 @@ def test_generated_counts_match_configs():
     m = M.Mesh.gen_tri(1000, 1000, -10, 10, -10, 10, 0.1, 12345, (M.BC_PERIODIC,) * 4)
     assert (m.n_cells, m.n_faces, m.n_int, m.n_nodes) == (2000000, 3002000, 2998000, 1002001)
     p = M.Prep(m)
-    assert p.n_periodic == 4000 and p.maxf == 3
+    assert p.n_periodic == 2000 and p.maxf == 3  # 4000 periodic faces form 2000 pairs
     # C3 construction at n=8: 6 n^3 tets, 12 n^3 - 6 n^2 interior faces, 12 n^2 boundary faces
     n = 8
     m = M.Mesh.gen_tet(n, n, n, (0, 0, 0), (1, 1, 1), 0.15, 12345, (M.BC_INLET, M.BC_OUTLET) + (M.BC_SYMMETRY,) * 4)
@@ -165,14 +165,19 @@ def test_rcm_permutation_colouring_tables(shuffle):
     assert np.array_equal(fc0[fnw], cn[raw["c0"]])
     inter = raw["c1"] >= 0
     assert np.array_equal(fc1[fnw][inter], cn[raw["c1"][inter]])
-    per = raw["ftype"] == M.BC_PERIODIC
+    per = (raw["ftype"] == M.BC_PERIODIC) & (raw["partner"] > np.arange(m.n_faces))  # first face of a pair
+    merged = (raw["ftype"] == M.BC_PERIODIC) & ~per
     assert np.array_equal(fc1[fnw][per], cn[raw["c0"][raw["partner"][per]]])
     assert (fc1[fnw][~inter & ~per] == -1).all()
-    # cell->face slots list exactly the faces each cell accumulates (c0 side, and c1 side of interior)
+    assert (kind[fnw][per] == 5).all() and (kind[fnw][merged] == 7).all()
+    # cell->face slots list exactly the faces each cell accumulates: c0 side of every face but the
+    # merged halves of periodic pairs, c1 side of interior faces and of periodic pairs
     inc = {}
     for f in range(m.n_faces):
+        if kind[f] == 7:
+            continue
         inc.setdefault(fc0[f], set()).add(2 * f)
-        if kind[f] == 0:
+        if kind[f] in (0, 5):
             inc.setdefault(fc1[f], set()).add(2 * f + 1)
     for c in range(m.n_cells):
         s = [e for e in ell[:, c] if e >= 0]
@@ -182,7 +187,9 @@ def test_rcm_permutation_colouring_tables(shuffle):
     seen = {}
     for f in range(m.n_faces):
         nf = fnw[f]
-        for c in ([fc0[nf], fc1[nf]] if kind[nf] == 0 else [fc0[nf]]):
+        if kind[nf] == 7:
+            continue
+        for c in ([fc0[nf], fc1[nf]] if kind[nf] in (0, 5) else [fc0[nf]]):
             key = (c, col[f])
             assert key not in seen
             seen[key] = f
