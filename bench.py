@@ -91,10 +91,15 @@ def measure(M, solver, steps, warmup):
 
 
 def field_errors(Qm, Qd):
-    sc = np.abs(Qd).max(0)
-    sc[1:-1] = sc[1:-1].max()
-    d = np.abs(Qm - Qd) / sc
-    return np.sqrt((d ** 2).mean(0)).tolist(), d.max(0).tolist()
+    """Per-field relative difference of two states: L2 = ||dQ_k||_2 / ||Q_k||_2 and
+    Linf = max|dQ_k| / max|Q_k|; the momentum components share the norm of the momentum field."""
+    nv = Qd.shape[1]
+    d = Qm - Qd
+    n2 = np.sqrt((Qd ** 2).sum(0))
+    n2[1:nv - 1] = np.sqrt((Qd[:, 1:nv - 1] ** 2).sum())
+    ninf = np.abs(Qd).max(0)
+    ninf[1:nv - 1] = ninf[1:nv - 1].max()
+    return (np.sqrt((d ** 2).sum(0)) / n2).tolist(), (np.abs(d).max(0) / ninf).tolist()
 
 
 def cpu_baseline(mesh, cfg_kw, Q, threads, budget_s=10.0):
@@ -125,6 +130,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="vortex2d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--strategy", choices=("face", "cell"), default="face",
+                    help="face: flux kernel + gather/update kernel; cell: fused cell kernel")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison")
     ap.add_argument("--compare-steps", type=int, default=200,
                     help="steps of the FP64 vs mixed state comparison")
@@ -143,6 +150,8 @@ def main():
         copy = dict(bytes=args.calibrate_copy, ms=ms, gbs=2 * args.calibrate_copy / (ms * 1e-3) / 1e9)
     mesh, cfg_kw, desc, init = build_workload(M, args.workload)
     solver = M.Solver(mesh, M.make_config(**cfg_kw))
+    solver.strategy = M.STRATEGY_CELL if args.strategy == "cell" else M.STRATEGY_FACE
+    desc = dict(desc, strategy=args.strategy)
     geo = solver.prep.geometry()
     Q0 = init(geo["ccen"])
     solver.set_state(Q0)

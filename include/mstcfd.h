@@ -160,6 +160,15 @@ int mst_update_new_to_old(mst_ctx *ctx);
 /* Time::goNextTimeStep (time/Time.cpp:54-80) generalised: nsteps steps of cfg->rk stages, dt from cfl
  * (device-side reduction) when cfg->cfl > 0 else the last mst_set_dt value.  Old holds the result. */
 int mst_step(mst_ctx *ctx, int nsteps);
+/* NEW: kernel strategy of mst_step and mst_solve.  FACE: a face kernel writes every face flux once,
+ * a cell kernel gathers them and updates (the reference's updateFaceFlux + solve split).  CELL: one
+ * fused cell kernel solves each cell's own faces (every interior face twice, bitwise the same flux
+ * on both sides) and updates, with no face-flux array in memory.  Results agree to rounding; the
+ * default is FACE.  mst_face_flux / mst_residual_update keep the face split whatever the strategy. */
+#define MST_STRATEGY_FACE 0
+#define MST_STRATEGY_CELL 1
+int mst_set_strategy(mst_ctx *ctx, int strategy);
+int mst_get_strategy(const mst_ctx *ctx, int *strategy);
 /* NEW: dt = cfl * min_c V_c / sum_d (|u_d| + a) Shalf_c,d on the Old state */
 int mst_compute_dt(mst_ctx *ctx, double cfl, double *dt);
 /* NEW: L2 norm over cells of R(Q_old)/V per field (nv values) from the first stage of the last step */
@@ -186,7 +195,8 @@ int mst_riemann_flux(const mst_config *cfg, int dim, int64_t n, const double *QL
 #define MST_K_FLUX 1
 #define MST_K_UPDATE 2
 #define MST_K_DT 3
-#define MST_K_COUNT 4
+#define MST_K_CELL 4 /* fused flux + gather + update (MST_STRATEGY_CELL) */
+#define MST_K_COUNT 5
 int mst_timing_enable(mst_ctx *ctx, int on);
 /* per kernel class: launches timed and total milliseconds since the last reset */
 int mst_timing_get(mst_ctx *ctx, int kernel, int64_t *count, double *total_ms);

@@ -6,6 +6,8 @@
 //                        G_f = sum_d n_d F_d  (c0 outward normal)
 //   k_update             one lane per cell: R_c = sum +-G_f over its faces in a fixed order (no float
 //                        atomics -> bit-reproducible), then the Euler / SSP-RK stage update
+//   k_cell               (cell strategy, replaces k_flux + k_update) one lane per cell: solves its own
+//                        faces' Riemann problems, sums them and updates, without a face-flux array
 // plus k_cfl (CFL time step) and one-block reductions (k_finish) of per-block partials.
 //
 // Precision forms: T = double is the plain FP64 path (the reference's arithmetic); T = float is the
@@ -44,30 +46,16 @@ struct FluxArgs {
     Params<T> P;
 };
 
-// RhoSolver::updateFaceFlux (RhoSolver.cpp:90-233) / updateFaceFlux2ndOrder (:234-369), one lane per face
-template <int DIM, int SCHEME, bool SECOND, typename T>
-__global__ __launch_bounds__(BLOCK) void k_flux(const FluxArgs<T> a) {
+// Riemann left (A: c0 side) and right (B: c1 side or ghost) states of face f: 2nd-order reconstruction
+// with the limited gradients, ghost states of the boundary faces (RhoSolver::updateFaceFlux,
+// RhoSolver.cpp:90-233, and updateFaceFlux2ndOrder, :234-369).  Every input is read from memory by
+// face index, so the lanes of the two cells of a face (cell strategy) compute bit-identical states.
+template <int DIM, bool SECOND, typename T>
+__device__ __forceinline__ void face_states(const FluxArgs<T> &a, int64_t f, int kind, const T *n, T (&A)[DIM + 2],
+                                            T (&B)[DIM + 2]) {
     constexpr int NV = DIM + 2;
     constexpr bool OFFS = std::is_same<T, float>::value;
-    const int64_t f = block_id() * BLOCK + threadIdx.x;
-    if (f >= a.nf) return;
-    const int info = a.finfo[f];
-    const int kind = info & 15, mask = info >> 4;
-    double g[NV];
-    if (kind >= mst::KIND_NONE) {  // unknown type (zero flux) or the merged half of a periodic pair
-#pragma unroll
-        for (int k = 0; k < NV; k++) g[k] = 0;
-        stq<NV>(a.G + f * NV, g);
-        return;
-    }
     const int64_t c0 = a.fc0[f];
-    double n64[DIM];
-    T n[DIM], A[NV], B[NV];
-#pragma unroll
-    for (int d = 0; d < DIM; d++) {
-        n64[d] = a.fn[f * DIM + d];
-        n[d] = (T)n64[d];
-    }
     ldq<NV>(a.Q + c0 * NV, A);
     T dx0[DIM];
     if constexpr (SECOND) {
@@ -116,7 +104,38 @@ __global__ __launch_bounds__(BLOCK) void k_flux(const FluxArgs<T> a) {
             for (int k = 0; k < NV; k++) B[k] = Rc[k];
         }
     }
+}
+
+// G_f (c0 outward) of one face: states, dimension-split Riemann solves, FP64 projection
+template <int DIM, int SCHEME, bool SECOND, typename T>
+__device__ __forceinline__ void face_flux(const FluxArgs<T> &a, int64_t f, int kind, int mask, double (&g)[DIM + 2]) {
+    constexpr int NV = DIM + 2;
+    double n64[DIM];
+    T n[DIM], A[NV], B[NV];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        n64[d] = a.fn[f * DIM + d];
+        n[d] = (T)n64[d];
+    }
+    face_states<DIM, SECOND, T>(a, f, kind, n, A, B);
     riemann_face<DIM, SCHEME, T>(A, B, n64, mask, a.P, g);
+}
+
+// face strategy: one lane per face writes G_f (RhoSolver::updateFaceFlux[2ndOrder])
+template <int DIM, int SCHEME, bool SECOND, typename T>
+__global__ __launch_bounds__(BLOCK) void k_flux(const FluxArgs<T> a) {
+    constexpr int NV = DIM + 2;
+    const int64_t f = block_id() * BLOCK + threadIdx.x;
+    if (f >= a.nf) return;
+    const int info = a.finfo[f];
+    const int kind = info & 15, mask = info >> 4;
+    double g[NV];
+    if (kind >= mst::KIND_NONE) {  // unknown type (zero flux) or the merged half of a periodic pair
+#pragma unroll
+        for (int k = 0; k < NV; k++) g[k] = 0;
+    } else {
+        face_flux<DIM, SCHEME, SECOND, T>(a, f, kind, mask, g);
+    }
     stq<NV>(a.G + f * NV, g);
 }
 
@@ -290,6 +309,8 @@ struct UpdArgs {
     const double *__restrict__ dt;  // [0] dt of this stage
     double ca, cb;
     int stage0;
+    int maxf;       // ELL slots (cell strategy)
+    int l2, dtred;  // cell strategy: reduce |R/V|^2 / the CFL time step in this launch
     double gamma, cfl;
     double *__restrict__ Qout;
     float *__restrict__ Qout32;     // FP32 shadow of Qout (mixed form), else null
@@ -341,50 +362,39 @@ __device__ __forceinline__ double local_dt(const double *q, const double *__rest
     return v == v ? v : -1.0;  // a NaN state reports a negative time step
 }
 
-// RhoSolver::solve gather + update (RhoSolver.cpp:45-68), SSP-RK stage form (new).  In the mixed form
-// (F32) the face fluxes from FP32 Riemann solves are summed with a compensation term (Neumaier) and
-// an FP32 shadow of the updated state is written for the next stage's FP32 kernels.
-template <int NV, int MAXF, int MODE, bool F32>
-__global__ __launch_bounds__(BLOCK) void k_update(const UpdArgs a) {
-    constexpr int DIM = NV - 2;
-    constexpr bool STAGE = MODE & UPD_STAGE, L2 = MODE & UPD_L2, DT = MODE & UPD_DT;
-    const int64_t blk = block_id();
-    const int64_t c = blk * BLOCK + threadIdx.x;
-    const bool valid = c < a.nc;
-    double r[NV], cmp[NV];
+// add one face flux to a cell's residual; the mixed form (F32) keeps the rounding error of every
+// addition in a Neumaier compensation term
+template <int NV, bool F32>
+__device__ __forceinline__ void accumulate(double (&r)[NV], double (&cmp)[NV], const double (&g)[NV], bool neg) {
 #pragma unroll
-    for (int k = 0; k < NV; k++) r[k] = cmp[k] = 0;
-    double red[NV + 1];
-#pragma unroll
-    for (int k = 0; k < NV; k++) red[k] = 0;
-    red[NV] = INFINITY;
-    if (valid) {
-#pragma unroll
-        for (int s = 0; s < MAXF; s++) {
-            const int32_t e = a.ell[(int64_t)s * a.nc + c];
-            if (e < 0) break;
-            double gf[NV];
-            ldq<NV>(a.G + (int64_t)(e >> 1) * NV, gf);
-#pragma unroll
-            for (int k = 0; k < NV; k++) {
-                const double x = (e & 1) ? -gf[k] : gf[k];
-                if constexpr (F32) {  // Neumaier: keep the rounding error of every addition
-                    const double t = r[k] + x;
-                    cmp[k] += fabs(r[k]) >= fabs(x) ? (r[k] - t) + x : (x - t) + r[k];
-                    r[k] = t;
-                } else {
-                    r[k] += x;
-                }
-            }
-        }
+    for (int k = 0; k < NV; k++) {
+        const double x = neg ? -g[k] : g[k];
         if constexpr (F32) {
-#pragma unroll
-            for (int k = 0; k < NV; k++) r[k] += cmp[k];
-        }
-        if constexpr (!STAGE) {
-            stq<NV>(a.R + c * NV, r);
-            return;
+            const double t = r[k] + x;
+            cmp[k] += fabs(r[k]) >= fabs(x) ? (r[k] - t) + x : (x - t) + r[k];
+            r[k] = t;
         } else {
+            r[k] += x;
+        }
+    }
+}
+
+// Stage update of cell c from its residual r (RhoSolver::solve, RhoSolver.cpp:57-66, in SSP-RK stage
+// form), the non-physical-state flag, and the per-block partials of |R/V|^2 and the CFL time step.
+// Every lane of the block must call it (block-level reductions).
+template <int NV, bool STAGE, bool F32>
+__device__ __forceinline__ void stage_tail(const UpdArgs &a, int64_t c, bool valid, const double (&r)[NV],
+                                           int64_t blk, const bool L2, const bool DT) {
+    constexpr int DIM = NV - 2;
+    if constexpr (!STAGE) {
+        if (valid) stq<NV>(a.R + c * NV, r);
+        return;
+    } else {
+        double red[NV + 1];
+#pragma unroll
+        for (int k = 0; k < NV; k++) red[k] = 0;
+        red[NV] = INFINITY;
+        if (valid) {
             const double V = a.vol[c];
             const double dtv = a.dt[0] / V;
             double qs[NV], out[NV];
@@ -405,7 +415,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(const UpdArgs a) {
             for (int k = 1; k < NV - 1; k++) ke += out[k] * out[k];
             const double p = (out[NV - 1] - 0.5 * ke / out[0]) * (a.gamma - 1);
             if (!(out[0] > 0) || !(p > 0)) a.bad[0] = 1;
-            if constexpr (L2) {
+            if (L2) {
                 const double iV = frcp(V);
 #pragma unroll
                 for (int k = 0; k < NV; k++) {
@@ -413,19 +423,80 @@ __global__ __launch_bounds__(BLOCK) void k_update(const UpdArgs a) {
                     red[k] = x * x;
                 }
             }
-            if constexpr (DT) red[NV] = local_dt<DIM>(out, a.sh, c, V, a.gamma, a.cfl);
+            if (DT) red[NV] = local_dt<DIM>(out, a.sh, c, V, a.gamma, a.cfl);
+        }
+        if (L2) {  // uniform over the launch
+            double v[NV];
+#pragma unroll
+            for (int k = 0; k < NV; k++) v[k] = red[k];
+            block_partial<NV, false>(v, a.part_l2 + blk, gridDim.x);
+        }
+        if (DT) {
+            double v[1] = {red[NV]};
+            block_partial<1, true>(v, a.part_dt + blk, 1);
         }
     }
-    if constexpr (STAGE && L2) {
-        double v[NV];
+}
+
+// Face strategy, second kernel: RhoSolver::solve gather + update (RhoSolver.cpp:45-68).  R_c sums +-G_f
+// over the cell's faces in a fixed order (no float atomics -> bit-reproducible).
+template <int NV, int MAXF, int MODE, bool F32>
+__global__ __launch_bounds__(BLOCK) void k_update(const UpdArgs a) {
+    const int64_t blk = block_id();
+    const int64_t c = blk * BLOCK + threadIdx.x;
+    const bool valid = c < a.nc;
+    double r[NV], cmp[NV];
 #pragma unroll
-        for (int k = 0; k < NV; k++) v[k] = red[k];
-        block_partial<NV, false>(v, a.part_l2 + blk, gridDim.x);
+    for (int k = 0; k < NV; k++) r[k] = cmp[k] = 0;
+    if (valid) {
+#pragma unroll
+        for (int s = 0; s < MAXF; s++) {
+            const int32_t e = a.ell[(int64_t)s * a.nc + c];
+            if (e < 0) break;
+            double gf[NV];
+            ldq<NV>(a.G + (int64_t)(e >> 1) * NV, gf);
+            accumulate<NV, F32>(r, cmp, gf, e & 1);
+        }
+        if constexpr (F32) {
+#pragma unroll
+            for (int k = 0; k < NV; k++) r[k] += cmp[k];
+        }
     }
-    if constexpr (STAGE && DT) {
-        double v[1] = {red[NV]};
-        block_partial<1, true>(v, a.part_dt + blk, 1);
+    stage_tail<NV, (MODE & UPD_STAGE) != 0, F32>(a, c, valid, r, blk, (MODE & UPD_L2) != 0, (MODE & UPD_DT) != 0);
+}
+
+// Cell strategy: flux + gather + update fused, one lane per cell.  Each lane solves the Riemann
+// problems of its own faces (an interior face is solved by both of its cells, with bit-identical
+// inputs and instructions, so the two cells still receive exactly opposite fluxes) and never writes
+// face fluxes to memory: the face-flux array's write and re-read are traded for a second solve.
+template <int DIM, int SCHEME, bool SECOND, typename T>
+__global__ __launch_bounds__(BLOCK) void k_cell(const FluxArgs<T> fa, const UpdArgs a) {
+    constexpr int NV = DIM + 2;
+    constexpr bool F32 = std::is_same<T, float>::value;
+    const int64_t blk = block_id();
+    const int64_t c = blk * BLOCK + threadIdx.x;
+    const bool valid = c < a.nc;
+    double r[NV], cmp[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) r[k] = cmp[k] = 0;
+    if (valid) {
+        for (int s = 0; s < a.maxf; s++) {
+            const int32_t e = a.ell[(int64_t)s * a.nc + c];
+            if (e < 0) break;
+            const int64_t f = e >> 1;
+            const int info = fa.finfo[f];
+            const int kind = info & 15;
+            if (kind >= mst::KIND_NONE) continue;  // zero-flux face
+            double g[NV];
+            face_flux<DIM, SCHEME, SECOND, T>(fa, f, kind, info >> 4, g);
+            accumulate<NV, F32>(r, cmp, g, e & 1);
+        }
+        if constexpr (F32) {
+#pragma unroll
+            for (int k = 0; k < NV; k++) r[k] += cmp[k];
+        }
     }
+    stage_tail<NV, true, F32>(a, c, valid, r, blk, a.l2, a.dtred);
 }
 
 // ------------------------------------------------------------------------------------ reductions

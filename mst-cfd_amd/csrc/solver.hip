@@ -49,6 +49,7 @@ struct mst_ctx {
     Params<double> p64;
     Params<float> p32;
     bool mixed = false;
+    int strategy = MST_STRATEGY_FACE;
     hipStream_t stream = nullptr;
     int device = 0;
     int dim = 2, nv = 4, maxf = 3;
@@ -160,8 +161,8 @@ int run_grad(mst_ctx *x, int isrc, bool debug) {
     return x->dim == 2 ? run_grad_t<2, double>(x, isrc, debug) : run_grad_t<3, double>(x, isrc, debug);
 }
 
-template <int DIM, typename T>
-int run_flux_t(mst_ctx *x, int isrc) {
+template <typename T>
+FluxArgs<T> flux_args(const mst_ctx *x, int isrc) {
     FluxArgs<T> a;
     a.nf = x->nf;
     a.pstart = x->prep.P.pstart;
@@ -184,6 +185,12 @@ int run_flux_t(mst_ctx *x, int isrc) {
         a.lg = x->lg;
         a.P = x->p64;
     }
+    return a;
+}
+
+template <int DIM, typename T>
+int run_flux_t(mst_ctx *x, int isrc) {
+    const FluxArgs<T> a = flux_args<T>(x, isrc);
     const bool second = x->cfg.order >= 2;
     if (x->cfg.scheme == MST_SCHEME_ROE)
         return second ? launch(x, MST_K_FLUX, k_flux<DIM, MST_SCHEME_ROE, true, T>, x->nf, a)
@@ -204,8 +211,7 @@ int run_face_flux(mst_ctx *x, int isrc) {
     return x->dim == 2 ? run_flux_t<2, double>(x, isrc) : run_flux_t<3, double>(x, isrc);
 }
 
-template <int NV, bool GT>
-int run_update_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
+UpdArgs upd_args(const mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
     UpdArgs a;
     a.nc = x->nc;
     a.ell = x->ell;
@@ -218,6 +224,9 @@ int run_update_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int
     a.ca = ca;
     a.cb = cb;
     a.stage0 = stage0;
+    a.maxf = x->maxf;
+    a.l2 = (mode & UPD_L2) != 0;
+    a.dtred = (mode & UPD_DT) != 0;
     a.gamma = x->cfg.gamma;
     a.cfl = x->cfg.cfl;
     a.Qout = iout >= 0 ? x->Q[iout] : nullptr;
@@ -226,6 +235,12 @@ int run_update_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int
     a.part_l2 = x->part;
     a.part_dt = x->part + x->nblk_c * x->nv;
     a.bad = x->bad;
+    return a;
+}
+
+template <int NV, bool GT>
+int run_update_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
+    const UpdArgs a = upd_args(x, mode, is, in, ca, cb, stage0, iout);
 #define MST_UPD(MF)                                                                                             \
     switch (mode) {                                                                                             \
         case UPD_RESIDUAL: return launch(x, MST_K_UPDATE, k_update<NV, MF, UPD_RESIDUAL, GT>, x->nc, a);        \
@@ -249,6 +264,42 @@ int run_update(mst_ctx *x, int mode, int is, int in, double ca, double cb, int s
     else
         rc = x->nv == 4 ? run_update_t<4, false>(x, mode, is, in, ca, cb, stage0, iout)
                         : run_update_t<5, false>(x, mode, is, in, ca, cb, stage0, iout);
+    if (rc == MST_OK && iout >= 0) x->shadow_ok[iout] = x->mixed;
+    return rc;
+}
+
+// cell strategy: gradient + limiter (order 2), then the fused flux + gather + stage update
+template <int DIM, typename T>
+int run_cell_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
+    const FluxArgs<T> fa = flux_args<T>(x, is);
+    const UpdArgs a = upd_args(x, mode, is, in, ca, cb, stage0, iout);
+    const bool second = x->cfg.order >= 2;
+    if (x->cfg.scheme == MST_SCHEME_ROE)
+        return second ? launch(x, MST_K_CELL, k_cell<DIM, MST_SCHEME_ROE, true, T>, x->nc, fa, a)
+                      : launch(x, MST_K_CELL, k_cell<DIM, MST_SCHEME_ROE, false, T>, x->nc, fa, a);
+    return second ? launch(x, MST_K_CELL, k_cell<DIM, MST_SCHEME_AUSM, true, T>, x->nc, fa, a)
+                  : launch(x, MST_K_CELL, k_cell<DIM, MST_SCHEME_AUSM, false, T>, x->nc, fa, a);
+}
+
+// one residual evaluation + stage update (mode includes UPD_STAGE) with the context's strategy
+int run_stage(mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
+    if (x->strategy == MST_STRATEGY_FACE) {
+        int rc = run_face_flux(x, is);
+        if (rc) return rc;
+        return run_update(x, mode, is, in, ca, cb, stage0, iout);
+    }
+    int rc = ensure_shadow(x, is);
+    if (rc) return rc;
+    if (x->cfg.order >= 2) {
+        rc = run_grad(x, is, false);
+        if (rc) return rc;
+    }
+    if (x->mixed)
+        rc = x->dim == 2 ? run_cell_t<2, float>(x, mode, is, in, ca, cb, stage0, iout)
+                         : run_cell_t<3, float>(x, mode, is, in, ca, cb, stage0, iout);
+    else
+        rc = x->dim == 2 ? run_cell_t<2, double>(x, mode, is, in, ca, cb, stage0, iout)
+                         : run_cell_t<3, double>(x, mode, is, in, ca, cb, stage0, iout);
     if (rc == MST_OK && iout >= 0) x->shadow_ok[iout] = x->mixed;
     return rc;
 }
@@ -465,7 +516,14 @@ int mst_ctx_create(const mst_mesh *m, const mst_config *cfg, mst_ctx **out) {
     TRY(dupload(&x->fc0, P.fc0));
     TRY(dupload(&x->fc1, P.fc1));
     TRY(dalloc(&x->finfo, x->nf));
-    TRY(dupload(&x->ell, P.ell));
+    {
+        // the gather loops are instantiated for 3, 4 or 8 slots: pad the slot-major table with empty
+        // rows up to that count so an unrolled loop over a full cell never reads past it
+        const int64_t slots = x->maxf <= 3 ? 3 : (x->maxf <= 4 ? 4 : 8);
+        std::vector<int32_t> ell(P.ell);
+        ell.resize(slots * x->nc, -1);
+        TRY(dupload(&x->ell, ell));
+    }
     TRY(dupload(&x->fn, P.fn));
     TRY(dupload(&x->fxf, P.fxf));
     TRY(dupload(&x->pxf, P.pxf));
@@ -556,9 +614,23 @@ int mst_residual_update(mst_ctx *ctx) {
 
 int mst_solve(mst_ctx *ctx) {
     CTX_GUARD(ctx);
-    int rc = run_face_flux(ctx, ctx->iold);
+    int rc = run_stage(ctx, UPD_STAGE, ctx->iold, ctx->iold, 0.0, 1.0, 1, ctx->inew);
     if (rc) return rc;
-    return mst_residual_update(ctx);
+    return check_bad(ctx);
+}
+
+int mst_set_strategy(mst_ctx *ctx, int strategy) {
+    CTX_GUARD(ctx);
+    if (strategy != MST_STRATEGY_FACE && strategy != MST_STRATEGY_CELL) return mst::fail(MST_E_ARG, "bad strategy");
+    ctx->strategy = strategy;
+    return MST_OK;
+}
+
+int mst_get_strategy(const mst_ctx *ctx, int *strategy) {
+    CTX_GUARD(ctx);
+    if (!strategy) return mst::fail(MST_E_ARG, "null argument");
+    *strategy = ctx->strategy;
+    return MST_OK;
 }
 
 int mst_update_new_to_old(mst_ctx *ctx) {
@@ -590,10 +662,8 @@ int mst_step(mst_ctx *ctx, int nsteps) {
         int src = ctx->iold;
         for (int s = 0; s < ns; s++) {
             const int dst = (s == ns - 1) ? ctx->inew : spare[s & 1];
-            rc = run_face_flux(ctx, src);
-            if (rc) return rc;
-            int mode = UPD_STAGE | (s == 0 ? UPD_L2 : 0) | ((cfl && s == ns - 1) ? UPD_DT : 0);
-            rc = run_update(ctx, mode, src, ctx->iold, A[s], B[s], s == 0 ? 1 : 0, dst);
+            const int mode = UPD_STAGE | (s == 0 ? UPD_L2 : 0) | ((cfl && s == ns - 1) ? UPD_DT : 0);
+            rc = run_stage(ctx, mode, src, ctx->iold, A[s], B[s], s == 0 ? 1 : 0, dst);
             if (rc) return rc;
             src = dst;
         }
@@ -815,26 +885,41 @@ int mst_sync(mst_ctx *ctx) {
     return MST_OK;
 }
 
-// Compulsory HBM traffic of one launch: every array the kernel must touch, once.
+// Compulsory HBM traffic of one launch: every array the kernel must touch, once.  r is the size of
+// the arithmetic real of the gradient/flux kernels (8 FP64, 4 mixed: FP32 state shadow, normals,
+// offsets, limited gradients); face fluxes, the stage states, volumes and centres stay FP64.
 int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
     CTX_GUARD(ctx);
     if (!bytes) return mst::fail(MST_E_ARG, "null argument");
     const double nc = (double)ctx->nc, nf = (double)ctx->nf, D = ctx->dim, NV = ctx->nv;
+    const double np = (double)ctx->prep.P.pcount;
+    const bool mixed = ctx->mixed, second = ctx->cfg.order >= 2;
+    const double r = mixed ? 4 : 8;
     const double ell = 4.0 * ctx->maxf * nc;
-    const bool second = ctx->cfg.order >= 2;
+    const double stages = ctx->cfg.rk;
+    // faces: c0, c1, info, FP64 normal; cells: the arithmetic state; order 2: face-cell offsets
+    // (FP64: face centres, partner centres and cell centres) and limited gradients
+    double flux_in = nf * (12 + 8 * D) + nc * r * NV;
+    if (second) flux_in += mixed ? nf * 8 * D + nc * r * NV * D : nf * 8 * D + np * 8 * D + nc * (8 * D + 8 * NV * D);
+    // update: cell faces, stage state, volume, new state (+ FP32 shadow), Q^n on stages after the
+    // first, the CFL half-sums on the last stage when the time step is reduced there
+    double upd = ell + nc * (8 * NV + 8 + 8 * NV) + nc * 8 * NV * (stages - 1) / stages;
+    if (mixed) upd += nc * 4 * NV;
+    if (ctx->cfg.cfl > 0) upd += nc * 8 * D / stages;
     switch (kernel) {
-        case MST_K_FLUX:  // c0,c1,info + normal + G out per face; Q per cell (+ centres, limited gradient)
-            *bytes = nf * (12 + 8 * D + 8 * NV) + nc * 8 * NV;
-            if (second) *bytes += nf * 8 * D + nc * (8 * D + 8 * NV * D) + (double)ctx->prep.P.pcount * 8 * D;
+        case MST_K_FLUX:
+            *bytes = flux_in + nf * 8 * NV;
             break;
-        case MST_K_GRAD:  // per face c0,c1, normal, centre, eta; per cell ell, Q, centre, V, limited gradient out
-            *bytes = nf * (8 + 16 * D + 8) + ell + nc * (8 * NV + 8 * D + 8 + 8 * NV * D);
+        case MST_K_GRAD:  // faces: c0, c1, normal, eta, offsets; cells: ell, state, volume, limited gradient out
+            *bytes = nf * (8 + r * D + r) + ell + nc * (r * NV + 8 + r * NV * D);
+            *bytes += mixed ? nf * 8 * D : nf * 8 * D + np * 8 * D + nc * 8 * D;
             break;
-        case MST_K_UPDATE: {  // ell, G per face, Q_stage, V, Q_out; Q^n on stages after the first
-            const double stages = ctx->cfg.rk;
-            *bytes = ell + nf * 8 * NV + nc * (8 * NV + 8 + 8 * NV) + nc * 8 * NV * (stages - 1) / stages;
+        case MST_K_UPDATE:
+            *bytes = upd + nf * 8 * NV;
             break;
-        }
+        case MST_K_CELL:  // flux inputs + update, no face-flux array; FP64 reads the stage state once
+            *bytes = flux_in + upd - (mixed ? 0 : nc * 8 * NV);
+            break;
         case MST_K_DT:
             *bytes = nc * (8 * NV + 8 + 8 * D);
             break;

@@ -21,8 +21,9 @@ LIM_NONE, LIM_MINMOD, LIM_VANLEER = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 COMPAT_BRANCH2_LR, COMPAT_GRAD_LASTINT, COMPAT_AUSM_LOCAL_A, COMPAT_AUSM_P_UNSCALED = 1, 2, 4, 8
 COMPAT_REFERENCE = COMPAT_GRAD_LASTINT | COMPAT_AUSM_LOCAL_A | COMPAT_AUSM_P_UNSCALED
-K_GRAD, K_FLUX, K_UPDATE, K_DT = 0, 1, 2, 3
-KERNEL_NAMES = {K_GRAD: "k_grad", K_FLUX: "k_flux", K_UPDATE: "k_update", K_DT: "k_cfl"}
+K_GRAD, K_FLUX, K_UPDATE, K_DT, K_CELL = 0, 1, 2, 3, 4
+KERNEL_NAMES = {K_GRAD: "k_grad", K_FLUX: "k_flux", K_UPDATE: "k_update", K_DT: "k_cfl", K_CELL: "k_cell"}
+STRATEGY_FACE, STRATEGY_CELL = 0, 1
 
 # public exported symbols of include/mstcfd.h (checked by the CPU tests)
 EXPORTS = [
@@ -34,6 +35,7 @@ EXPORTS = [
     "mst_update_new_to_old", "mst_step", "mst_compute_dt", "mst_residual_norms", "mst_monitor_maxrel",
     "mst_get_residual", "mst_get_face_flux", "mst_get_gradient", "mst_riemann_flux", "mst_timing_enable",
     "mst_timing_get", "mst_timing_reset", "mst_sync", "mst_algorithmic_bytes", "mst_copy_probe",
+    "mst_set_strategy", "mst_get_strategy",
 ]
 
 
@@ -111,6 +113,8 @@ def lib():
         "mst_ctx_create": (C.c_int, [vp, C.POINTER(Config), pp]),
         "mst_ctx_destroy": (C.c_int, [vp]),
         "mst_ctx_set_config": (C.c_int, [vp, C.POINTER(Config)]),
+        "mst_set_strategy": (C.c_int, [vp, C.c_int]),
+        "mst_get_strategy": (C.c_int, [vp, C.POINTER(C.c_int)]),
         "mst_ctx_prep": (vp, [vp]),
         "mst_set_state": (C.c_int, [vp, F64P, C.c_int64]),
         "mst_get_state": (C.c_int, [vp, F64P, C.c_int64]),
@@ -287,6 +291,16 @@ class Solver:
     def set_config(self, cfg):
         _check(lib().mst_ctx_set_config(self.h, C.byref(cfg)), "mst_ctx_set_config")
         self.cfg = cfg
+
+    @property
+    def strategy(self):
+        v = C.c_int()
+        _check(lib().mst_get_strategy(self.h, C.byref(v)), "mst_get_strategy")
+        return v.value
+
+    @strategy.setter
+    def strategy(self, value):
+        _check(lib().mst_set_strategy(self.h, int(value)), "mst_set_strategy")
 
     def set_state(self, Q):
         Q = np.ascontiguousarray(Q, np.float64).reshape(self.n_cells, self.nv)

@@ -48,6 +48,11 @@ def test_error_paths_return_codes():
         M.Mesh.gen_tri(4, 4, 0, 1, 0, 1, bc=(M.BC_PERIODIC, M.BC_WALL, M.BC_WALL, M.BC_WALL))
     with pytest.raises(M.MstError):
         M.Mesh.gen_tet(2, 2, 2, bc=(M.BC_PERIODIC,) * 6)
+    # context entry points reject a null context before touching a device
+    v = C.c_int()
+    assert M.lib().mst_set_strategy(None, M.STRATEGY_CELL) == -1
+    assert M.lib().mst_get_strategy(None, C.byref(v)) == -1
+    assert M.lib().mst_step(None, 1) == -1
 
 
 def test_malformed_mesh_is_rejected(tmp_path):

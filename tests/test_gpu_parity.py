@@ -133,8 +133,12 @@ def test_gradient_and_limiter_parity(mname, lim):
     assert perr <= 1e-9
 
 
+STRATEGIES = [M.STRATEGY_FACE, M.STRATEGY_CELL]
+
+
+@pytest.mark.parametrize("strategy", STRATEGIES)
 @pytest.mark.parametrize("mname,scheme", [("allbc", ROE), ("strip_wall", AUSM), ("tet", AUSM)])
-def test_euler_solve_matches_oracle(mname, scheme):
+def test_euler_solve_matches_oracle(mname, scheme, strategy):
     """RhoSolver::solve: Q_new = Q_old - DT/V R, then updateNewToOld."""
     m, om, geo = mesh(mname)
     dim = m.dim
@@ -142,6 +146,7 @@ def test_euler_solve_matches_oracle(mname, scheme):
     Q = smooth_state(geo["ccen"], seed=2, mach=0.5)
     dt = 1.0 / 4000.0 if mname != "allbc" else 1e-3
     s = M.Solver(m, cg)
+    s.strategy = strategy
     s.set_state(Q)
     s.set_dt(dt)
     s.solve()
@@ -152,15 +157,18 @@ def test_euler_solve_matches_oracle(mname, scheme):
     assert np.array_equal(s.get_state(), Qg)
 
 
+@pytest.mark.parametrize("strategy", STRATEGIES)
 @pytest.mark.parametrize("mname,scheme,order,lim,rk", [
     ("strip_wall", ROE, 2, M.LIM_MINMOD, 2), ("periodic", ROE, 2, M.LIM_VANLEER, 3),
-    ("allbc", AUSM, 2, M.LIM_MINMOD, 2), ("tet", AUSM, 2, M.LIM_MINMOD, 2), ("allbc", ROE, 1, 0, 1)])
-def test_rk_steps_match_oracle(mname, scheme, order, lim, rk):
+    ("allbc", AUSM, 2, M.LIM_MINMOD, 2), ("tet", AUSM, 2, M.LIM_MINMOD, 2), ("allbc", ROE, 1, 0, 1),
+    ("tet", ROE, 2, M.LIM_VANLEER, 3)])
+def test_rk_steps_match_oracle(mname, scheme, order, lim, rk, strategy):
     m, om, geo = mesh(mname)
     dim = m.dim
     cg, co = both_cfg(dim=dim, scheme=scheme, order=order, limiter=lim, rk=rk, cfl=0.4, q_inlet=inlet(dim))
     Q = smooth_state(geo["ccen"], seed=3, mach=0.6)
     s = M.Solver(m, cg)
+    s.strategy = strategy
     s.set_state(Q)
     Qr = Q.copy()
     for it in range(10):
@@ -173,7 +181,7 @@ def test_rk_steps_match_oracle(mname, scheme, order, lim, rk):
         assert np.allclose(ng, res, rtol=1e-10, atol=0)
     Qg = s.get_state()
     err = rel_state_error(Qg, Qr)
-    print(f"{mname} rk{rk}: 10-step state err {err:.2e}", flush=True)
+    print(f"{mname} rk{rk} strategy {strategy}: 10-step state err {err:.2e}", flush=True)
     assert err <= 1e-11
 
 
@@ -212,3 +220,28 @@ def test_monitor_maxrel_matches_reference_rule():
     got = s.monitor_maxrel()
     ref = O.monitor_maxrel(s.get_new_state(), s.get_state())
     assert np.allclose(got, ref, rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_MIXED])
+@pytest.mark.parametrize("mname,scheme,lim", [("periodic", ROE, M.LIM_VANLEER), ("allbc", AUSM, M.LIM_MINMOD),
+                                              ("tet", ROE, M.LIM_MINMOD)])
+def test_cell_strategy_matches_face_strategy(mname, scheme, lim, precision):
+    """The fused cell kernel solves each interior face twice from the same memory inputs with the same
+    instructions, so it reproduces the face split (same fluxes, same summation order)."""
+    m, om, geo = mesh(mname)
+    dim = m.dim
+    cg = M.make_config(dim=dim, scheme=scheme, order=2, limiter=lim, rk=3, cfl=0.45, q_inlet=inlet(dim),
+                       precision=precision)
+    Q = smooth_state(geo["ccen"], seed=6, mach=0.7)
+    out = []
+    for strategy in STRATEGIES:
+        s = M.Solver(m, cg)
+        s.strategy = strategy
+        assert s.strategy == strategy
+        s.set_state(Q)
+        s.step(20)
+        out.append((s.get_state(), s.residual_norms()))
+    err = rel_state_error(out[1][0], out[0][0])
+    print(f"{mname} prec={precision}: cell vs face after 20 RK3 steps {err:.2e}", flush=True)
+    assert err <= 1e-14
+    assert np.allclose(out[1][1], out[0][1], rtol=1e-12, atol=0)
