@@ -23,38 +23,13 @@ sys.path.insert(0, os.path.join(ROOT, "mst-cfd_amd"))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def vortex_state(ccen, beta=5.0, mach=0.5, gamma=1.4):
-    """Shu's isentropic vortex centred at the origin on a rho=1, p=1 freestream moving in +x."""
-    x, y = ccen[:, 0], ccen[:, 1]
-    r2 = x * x + y * y
-    uinf = mach * math.sqrt(gamma)
-    e = np.exp(0.5 * (1.0 - r2))
-    du = -beta / (2 * math.pi) * y * e
-    dv = beta / (2 * math.pi) * x * e
-    T = 1.0 - (gamma - 1) * beta ** 2 / (8 * gamma * math.pi ** 2) * np.exp(1.0 - r2)
-    rho = T ** (1.0 / (gamma - 1))
-    p = rho * T
-    u, v = uinf + du, dv
-    Q = np.stack([rho, rho * u, rho * v, p / (gamma - 1) + 0.5 * rho * (u * u + v * v)], axis=1)
-    return np.ascontiguousarray(Q)
-
-
-def pulse_state(ccen, amp=0.1, sigma=0.1, mach=0.8, gamma=1.4):
-    """3D Gaussian density/pressure pulse at the box centre on a Mach-0.8 x-flow (rho=1, p=1/gamma)."""
-    r2 = ((ccen - 0.5) ** 2).sum(1)
-    b = amp * np.exp(-r2 / (2 * sigma ** 2))
-    rho, p, u = 1 + b, (1 + b) / gamma, mach
-    E = p / (gamma - 1) + 0.5 * rho * u * u
-    return np.ascontiguousarray(np.stack([rho, rho * u, 0 * rho, 0 * rho, E], axis=1))
-
-
 def build_workload(M, name):
     if name == "vortex2d":
         mesh = M.Mesh.gen_tri(1000, 1000, -10.0, 10.0, -10.0, 10.0, 0.1, 12345, (M.BC_PERIODIC,) * 4)
         cfg = dict(scheme=M.ROE, order=2, limiter=M.LIM_VANLEER, rk=3, cfl=0.5)
         desc = dict(workload="isentropic vortex 2D, 1000x1000x2 jittered triangles, periodic",
                     cells=mesh.n_cells, faces=mesh.n_faces, scheme="Roe", limiter="van Leer", rk=3, cfl=0.5)
-        return mesh, cfg, desc, vortex_state
+        return mesh, cfg, desc, lambda: mesh.init_state(M.INIT_VORTEX, [1.4, 0.5, 5.0, 0.0, 0.0])
     if name == "pulse3d":
         bc = (M.BC_INLET, M.BC_OUTLET) + (M.BC_SYMMETRY,) * 4
         mesh = M.Mesh.gen_tet(160, 160, 160, (0, 0, 0), (1, 1, 1), 0.15, 12345, bc)
@@ -63,7 +38,7 @@ def build_workload(M, name):
                    q_inlet=[1.0, 0.8, 0.0, 0.0, E])
         desc = dict(workload="3D Gaussian pulse in a Mach-0.8 box flow, 160^3 hexes x 6 jittered tets",
                     cells=mesh.n_cells, faces=mesh.n_faces, scheme="AUSM", limiter="minmod", rk=2, cfl=0.5)
-        return mesh, cfg, desc, pulse_state
+        return mesh, cfg, desc, lambda: mesh.init_state(M.INIT_PULSE, [1.4, 0.1, 0.1, 0.5, 0.5, 0.5, 0.8])
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -123,6 +98,75 @@ def cpu_baseline(mesh, cfg_kw, Q, threads, budget_s=10.0):
                        f"{mesh.n_cells}-cell workload, oracle/mst_oracle.c with OpenMP, {t:.2f} s")
 
 
+def batch_cases(M, n=512, seed=777):
+    """512 Sod/Riemann variants on [0,1] x [0,0.01] strips of 2 x nx quads (N_c = 4 nx uniform in
+    [500, 50000]), rho_R = 0.125, p_R = 0.1, rho_L/rho_R ~ U[1, 10], Roe or AUSM+ with probability
+    1/2; p_L/p_R = 10^U[log10 2, 2] for Roe cases and 10^U[log10 2, log10 20] for AUSM+ cases (above
+    a ratio of about 25 AUSM+ loses positivity in the first steps on these strips, at any CFL number
+    tried down to 0.05, in the oracle as on the device); walls all round, minmod, SSP-RK2, CFL 0.5."""
+    rng = np.random.default_rng(seed)
+    meshes, cfgs, Q0 = [], [], []
+    for i in range(n):
+        nc = rng.uniform(500, 50000)
+        nx = max(2, int(round(nc / 4)))
+        roe = rng.uniform() < 0.5
+        hi = 2.0 if roe else math.log10(20.0)
+        pr = 10 ** (math.log10(2) + rng.uniform() * (hi - math.log10(2)))
+        dr = rng.uniform(1, 10)
+        m = M.Mesh.gen_tri(nx, 2, 0.0, 1.0, 0.0, 0.01, 0.1, seed * 1000 + i)
+        meshes.append(m)
+        cfgs.append(dict(scheme=M.ROE if roe else M.AUSM, order=2, limiter=M.LIM_MINMOD, rk=2, cfl=0.5,
+                         compat=M.COMPAT_REFERENCE if roe else M.COMPAT_GRAD_LASTINT))
+        Q0.append(m.init_state(M.INIT_RIEMANN, [1.4, 0, 0.5, 0.125 * dr, 0, 0, 0.1 * pr, 0.125, 0, 0, 0.1]))
+    return meshes, cfgs, Q0
+
+
+def bench_batch(M, args):
+    """Batch throughput: all cases advance one SSP-RK2 step per graph replay."""
+    t0 = time.perf_counter()
+    meshes, cfgs, Q0 = batch_cases(M)
+    cache = os.environ.get("MSTCFD_TUNE_CACHE") or os.path.join(
+        os.environ.get("XDG_CACHE_HOME") or os.path.join(os.path.expanduser("~"), ".cache"), "mstcfd", "batch_tune.json")
+    os.makedirs(os.path.dirname(cache), exist_ok=True)
+    out_forms = {}
+    for form, prec in (("fp64", M.PREC_FP64), ("mixed", M.PREC_MIXED)):
+        b = M.Batch(meshes, [M.make_config(precision=prec, **c) for c in cfgs])
+        for i, q in enumerate(Q0):
+            b.set_state(i, q)
+        t_setup = time.perf_counter() - t0
+        tune_ms = b.autotune(cache, force=args.retune, steps=5)
+        block, tiles = b.get_tuning()
+        res = {}
+        for graph in (True, False):
+            for i, q in enumerate(Q0):  # both timings advance the same steps from the initial field
+                b.set_state(i, q)
+            b.set_graph(graph)
+            b.step(args.warmup)  # mst_batch_step returns after the stream has drained
+            t1 = time.perf_counter()
+            b.step(args.steps)
+            wall = time.perf_counter() - t1
+            res["graph" if graph else "direct"] = wall
+        n_items = b.counts()
+        finite = all(np.isfinite(b.get_state(i)).all() for i in range(0, len(meshes), 37))
+        faces = b.n_faces
+        out_forms[form] = dict(value=faces * 2 * args.steps / res["graph"], ms_per_step=1e3 * res["graph"] / args.steps,
+                               direct_ms_per_step=1e3 * res["direct"] / args.steps, block=block, tiles=tiles,
+                               autotune_ms_per_step=tune_ms, cell_items=n_items[3], face_items=n_items[4],
+                               state_finite=bool(finite))
+        b.close()
+    v = out_forms["fp64"]
+    out = dict(metric="batch face-flux updates/s (512 uneven Riemann/Sod cases, graph-replayed steps)",
+               value=v["value"], unit="face-updates/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+               ms_per_step=v["ms_per_step"], higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
+               data="synthetic",
+               config=dict(workload="512 Sod/Riemann variants, N_c ~ U[500,50000] triangles, Roe/AUSM per case",
+                           cases=len(meshes), cells=int(sum(m.n_cells for m in meshes)),
+                           faces=int(sum(m.n_faces for m in meshes)), limiter="minmod", rk=2, cfl=0.5,
+                           setup_s=t_setup, tune_cache=cache),
+               fp64=v, mixed=out_forms["mixed"], mixed_speedup=out_forms["mixed"]["value"] / v["value"])
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +174,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="vortex2d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--retune", action="store_true", help="batch workload: ignore the tuning cache")
     ap.add_argument("--strategy", choices=("face", "cell"), default="face",
                     help="face: flux kernel + gather/update kernel; cell: fused cell kernel")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison")
@@ -144,6 +189,8 @@ def main():
         sys.exit(2)
     import mstcfd as M
 
+    if args.workload == "batch":
+        return bench_batch(M, args)
     copy = None
     if args.calibrate_copy:
         ms = M.copy_probe(args.calibrate_copy)
@@ -152,8 +199,7 @@ def main():
     solver = M.Solver(mesh, M.make_config(**cfg_kw))
     solver.strategy = M.STRATEGY_CELL if args.strategy == "cell" else M.STRATEGY_FACE
     desc = dict(desc, strategy=args.strategy)
-    geo = solver.prep.geometry()
-    Q0 = init(geo["ccen"])
+    Q0 = init()
     solver.set_state(Q0)
     wall, kern = measure(M, solver, args.steps, args.warmup)
     stages = cfg_kw["rk"]

@@ -52,7 +52,10 @@ extern "C" {
 #define MST_COMPAT_GRAD_LASTINT 2    /* rhoSolver/RhoSolver.cpp:437-440 loop start N_int-1 */
 #define MST_COMPAT_AUSM_LOCAL_A 4    /* rhoSolver/SolverAusm.cpp:112-113 */
 #define MST_COMPAT_AUSM_P_UNSCALED 8 /* rhoSolver/SolverAusm.cpp:130,136 operator precedence */
-#define MST_COMPAT_REFERENCE (MST_COMPAT_GRAD_LASTINT | MST_COMPAT_AUSM_LOCAL_A | MST_COMPAT_AUSM_P_UNSCALED)
+#define MST_COMPAT_AUSM_ONESIDED 16  /* rhoSolver/SolverAusm.cpp:116,122,129,135 abs(M <= 1): subsonic
+                                        polynomials for every M <= 1 (clear: |M| <= 1, AUSM+) */
+#define MST_COMPAT_REFERENCE                                                                      \
+    (MST_COMPAT_GRAD_LASTINT | MST_COMPAT_AUSM_LOCAL_A | MST_COMPAT_AUSM_P_UNSCALED | MST_COMPAT_AUSM_ONESIDED)
 
 /* Runtime replacement of the compile-time macros of include/CONST.h:3-83
  * (RHOSOLVER :10, ACCURACY :6, GAMMA :41, EOR :42, inlet state :78-83). */
@@ -104,8 +107,23 @@ void mst_mesh_destroy(mst_mesh *m);
 int mst_mesh_gen_tri(int nx, int ny, double x0, double x1, double y0, double y1, double jitter, uint64_t seed,
                      const int32_t bc[4], mst_mesh **out);
 /* 3D: nx x ny x nz hexes on the box lo..hi, each split into 6 tets (Kuhn split); bc[6] = x0,x1,y0,y1,z0,z1 */
+/* NEW (configuration C5): compression ramp, the gen_tri lattice mapped onto x in [x0, x1] (x0 < 0 < x1)
+ * with a wall corner at x = 0 (bottom flat for x < 0, rising at theta_deg for x > 0) and a straight
+ * top at y = height; bc = {left, right, bottom, top} */
+int mst_mesh_gen_ramp(int nx, int ny, double x0, double x1, double height, double theta_deg, double jitter,
+                      uint64_t seed, const int32_t bc[4], mst_mesh **out);
 int mst_mesh_gen_tet(int nx, int ny, int nz, const double lo[3], const double hi[3], double jitter, uint64_t seed,
                      const int32_t bc[6], mst_mesh **out);
+
+/* ---------------------------------------------------------------- initial conditions (file order)
+ * Time::initialization (time/Time.cpp:13-36) and, new, the synthetic workloads' fields.  Q: n_cells*nv. */
+#define MST_INIT_REFERENCE 0  /* params: Q0 (nv conserved); x_c > 0.5 gets rho*0.125, rho*E*0.1 (Time.cpp:26-36) */
+#define MST_INIT_UNIFORM 1    /* params: gamma, rho, u_1..u_dim, p */
+#define MST_INIT_RIEMANN 2    /* params: gamma, axis, x0, rhoL, uL(dim), pL, rhoR, uR(dim), pR (left: x_c[axis] <= x0) */
+#define MST_INIT_VORTEX 3     /* 2D, params: gamma, mach, beta, xc, yc (Shu vortex on rho_inf = p_inf = 1) */
+#define MST_INIT_PULSE 4      /* params: gamma, amp, sigma, centre(dim), mach: rho = 1+b, p = (1+b)/gamma, u = mach e_x */
+#define MST_INIT_FREESTREAM 5 /* params: gamma, mach, angle (deg): rho = 1, p = 1/gamma, |u| = mach */
+int mst_mesh_init_state(const mst_mesh *m, int kind, const double *params, int n_params, double *Q, int64_t n_cells);
 
 /* ---------------------------------------------------------------- host preprocessing (no device needed)
  * The one-time part of the reference's mesh set-up (mesh/MshBlock.cpp:272-334, mesh/Face.cpp:8-69,
@@ -189,6 +207,42 @@ int mst_get_gradient(mst_ctx *ctx, double *grad, double *phi, int64_t n_cells);
  * QL, QR: n*(dim+2); dir: n; F: n*(dim+2); host arrays. */
 int mst_riemann_flux(const mst_config *cfg, int dim, int64_t n, const double *QL, const double *QR,
                      const int32_t *dir, double *F);
+
+/* ---------------------------------------------------------------- batches (NEW, no reference counterpart:
+ * the reference runs one mesh per process, work/Work.cpp:53-89)
+ * Many independent cases advanced together.  A step of every active case is one pass of persistent
+ * kernels over a work list of (case, tile) items -- gradient + limiter, fluxes (Roe or AUSM per
+ * case), gather + stage update -- and one reduction block per case; the whole RK step is replayed
+ * from a hipGraph.  Cases share dim, order, limiter, rk and precision; scheme, compat, gamma, eor,
+ * entropy_delta, q_inlet and cfl (> 0: each case advances with its own CFL time step) are per case.
+ * Periodic faces are not supported.  States use each case's file cell order.  Results do not depend
+ * on the tiling and equal the single-case context's. */
+typedef struct mst_batch mst_batch;
+#define MST_BATCH_BUCKETS 4 /* cases binned by cell count: < 2048, < 8192, < 32768, >= 32768 */
+int mst_batch_create(int n_cases, const mst_mesh *const *meshes, const mst_config *cfgs, mst_batch **out);
+int mst_batch_destroy(mst_batch *b);
+/* out[6]: cases, total cells, total faces, cell work items, face work items, dim */
+int mst_batch_counts(const mst_batch *b, int64_t *out);
+int mst_batch_case_counts(const mst_batch *b, int case_id, int64_t *n_cells, int64_t *n_faces);
+int mst_batch_set_state(mst_batch *b, int case_id, const double *Q, int64_t n_cells);
+int mst_batch_get_state(mst_batch *b, int case_id, double *Q, int64_t n_cells);
+/* nsteps RK steps of every active case */
+int mst_batch_step(mst_batch *b, int nsteps);
+/* per case (n_cases*nv): L2 over cells of R/V from the first stage of the last step */
+int mst_batch_residual_norms(mst_batch *b, double *out);
+/* per case: the CFL time step the next step will take */
+int mst_batch_dt(mst_batch *b, double *out);
+/* active[n_cases]: inactive cases keep their state and are left out of the work list */
+int mst_batch_set_active(mst_batch *b, const uint8_t *active);
+/* launch shape: block size (64, 128, 256) and cell tile per case-size bucket (faces: scaled by nf/nc) */
+int mst_batch_set_tuning(mst_batch *b, int block, const int32_t *tiles);
+int mst_batch_get_tuning(const mst_batch *b, int *block, int32_t *tiles);
+/* 1: replay the captured step graph (default); 0: launch the kernels one by one */
+int mst_batch_set_graph(mst_batch *b, int on);
+/* measured search over block size and per-bucket tiles (steps timed per trial); the best shape is
+ * applied and stored in the JSON file cache_path under a key describing the batch; a cached entry
+ * is applied without measuring unless force.  The state is restored afterwards. */
+int mst_batch_autotune(mst_batch *b, const char *cache_path, int force, int steps, double *ms_per_step);
 
 /* ---------------------------------------------------------------- timing (HIP events on the ctx stream) */
 #define MST_K_GRAD 0

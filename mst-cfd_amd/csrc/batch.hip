@@ -1,0 +1,1132 @@
+// batch.hip -- many independent cases advanced together (no reference counterpart: the reference
+// runs one mesh per process, work/Work.cpp:53-89).
+//
+// Layout: the cases' prepared meshes are concatenated into one set of SoA arrays (global cell and
+// face indices); per-case parameters (scheme, gamma, inlet state, CFL number) and the per-case time
+// step live in small device arrays.  A step is a fixed sequence of persistent kernels (grid = CU
+// count x resident blocks) that pull (case, tile) work items from a device counter: gradient +
+// limiter over cell items, fluxes over face items (Roe or AUSM chosen per item's case), gather +
+// stage update over cell items with per-item partial reductions, then one block per case reduces
+// its items' partials into the case's residual norms and next time step.  A whole RK step is
+// captured once into a hipGraph per state-buffer parity and replayed.  Tile sizes (per case-size
+// bucket) and the block size come from a measured search whose results are cached in a JSON file.
+//
+// Counters: each persistent kernel pulls from its own counter and, at its start, zeroes the other
+// counters (the kernels before it in the stream have finished), so a replayed graph needs no
+// memset nodes.  Results do not depend on the tiling: every cell and face is computed by the same
+// code as the single-case context, and the time step is a minimum (order-independent).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "mstcfd.h"
+#include "prep.hpp"
+
+using namespace mstk;
+
+namespace {
+
+#define HIPCHK(expr)                                                                                     \
+    do {                                                                                                 \
+        hipError_t _e = (expr);                                                                          \
+        if (_e != hipSuccess)                                                                            \
+            return mst::fail(MST_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));              \
+    } while (0)
+
+struct Item {
+    int32_t cs, begin, end, pad;
+};
+
+constexpr int NCNT = 4;
+enum { CNT_GRAD = 0, CNT_FLUX = 1, CNT_UPD = 2, CNT_CFL = 3 };
+constexpr int FIN_B = 256;
+
+template <typename T>
+struct BArgs {
+    GradArgs<T> g;
+    FluxArgs<T> f;
+    UpdArgs u;
+    const Params<T> *P;
+    const int32_t *scheme;
+    const double *dts, *gam, *cfl;
+    const Item *items;
+    int n_items, own;
+    int *cnt;
+    double *part_l2;  // [field][item]
+    double *part_dt;  // [item]
+};
+
+// next work item of this block (one atomic per block, broadcast through LDS)
+__device__ __forceinline__ int pull(int *cnt, int own) {
+    __shared__ int s_item;
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = atomicAdd(cnt + own, 1);
+    __syncthreads();
+    return s_item;
+}
+
+__device__ __forceinline__ void reset_others(int *cnt, int own) {
+    if (blockIdx.x == 0 && (int)threadIdx.x < NCNT && (int)threadIdx.x != own) cnt[threadIdx.x] = 0;
+}
+
+template <int DIM, int MAXF, int LIM, typename T, int NT>
+__global__ __launch_bounds__(NT) void kb_grad(const BArgs<T> a) {
+    reset_others(a.cnt, a.own);
+    for (;;) {
+        const int it = pull(a.cnt, a.own);
+        if (it >= a.n_items) break;
+        const Item I = a.items[it];
+        for (int64_t c = I.begin + (int)threadIdx.x; c < I.end; c += NT) grad_cell<DIM, MAXF, LIM, T>(a.g, c);
+    }
+}
+
+template <int DIM, bool SECOND, typename T, int NT>
+__global__ __launch_bounds__(NT) void kb_flux(const BArgs<T> a) {
+    constexpr int NV = DIM + 2;
+    reset_others(a.cnt, a.own);
+    for (;;) {
+        const int it = pull(a.cnt, a.own);
+        if (it >= a.n_items) break;
+        const Item I = a.items[it];
+        FluxArgs<T> fa = a.f;
+        fa.P = a.P[I.cs];
+        const bool roe = a.scheme[I.cs] == MST_SCHEME_ROE;
+        for (int64_t f = I.begin + (int)threadIdx.x; f < I.end; f += NT) {
+            const int info = fa.finfo[f];
+            const int kind = info & 15, mask = info >> 4;
+            double g[NV];
+            if (kind >= mst::KIND_NONE) {
+#pragma unroll
+                for (int k = 0; k < NV; k++) g[k] = 0;
+            } else if (roe) {
+                face_flux<DIM, MST_SCHEME_ROE, SECOND, T>(fa, f, kind, mask, g);
+            } else {
+                face_flux<DIM, MST_SCHEME_AUSM, SECOND, T>(fa, f, kind, mask, g);
+            }
+            stq<NV>(fa.G + f * NV, g);
+        }
+    }
+}
+
+template <int NV, int MAXF, typename T, int NT>
+__global__ __launch_bounds__(NT) void kb_update(const BArgs<T> a) {
+    constexpr bool F32 = std::is_same<T, float>::value;
+    reset_others(a.cnt, a.own);
+    const bool L2 = a.u.l2 != 0, DT = a.u.dtred != 0;
+    for (;;) {
+        const int it = pull(a.cnt, a.own);
+        if (it >= a.n_items) break;
+        const Item I = a.items[it];
+        UpdArgs ua = a.u;
+        ua.bad = a.u.bad + I.cs;  // per-case flag
+        ua.dt = a.dts + I.cs;
+        ua.gamma = a.gam[I.cs];
+        ua.cfl = a.cfl[I.cs];
+        double red[NV + 1];
+#pragma unroll
+        for (int k = 0; k < NV; k++) red[k] = 0;
+        red[NV] = INFINITY;
+        for (int64_t c = I.begin + (int)threadIdx.x; c < I.end; c += NT) {
+            double r[NV], cmp[NV];
+#pragma unroll
+            for (int k = 0; k < NV; k++) r[k] = cmp[k] = 0;
+#pragma unroll
+            for (int s = 0; s < MAXF; s++) {
+                const int32_t e = ua.ell[(int64_t)s * ua.nc + c];
+                if (e < 0) break;
+                double gf[NV];
+                ldq<NV>(ua.G + (int64_t)(e >> 1) * NV, gf);
+                accumulate<NV, F32>(r, cmp, gf, e & 1);
+            }
+            if constexpr (F32) {
+#pragma unroll
+                for (int k = 0; k < NV; k++) r[k] += cmp[k];
+            }
+            stage_cell<NV, F32>(ua, c, r, red, L2, DT);
+        }
+        if (L2) {
+            double v[NV];
+#pragma unroll
+            for (int k = 0; k < NV; k++) v[k] = red[k];
+            block_partial<NV, false, NT>(v, a.part_l2 + it, a.n_items);
+        }
+        if (DT) {
+            double v[1] = {red[NV]};
+            block_partial<1, true, NT>(v, a.part_dt + it, 1);
+        }
+    }
+}
+
+template <int DIM, typename T, int NT>
+__global__ __launch_bounds__(NT) void kb_cfl(const BArgs<T> a, const double *__restrict__ Q) {
+    constexpr int NV = DIM + 2;
+    reset_others(a.cnt, a.own);
+    for (;;) {
+        const int it = pull(a.cnt, a.own);
+        if (it >= a.n_items) break;
+        const Item I = a.items[it];
+        const double gam = a.gam[I.cs], cfl = a.cfl[I.cs];
+        double v[1] = {INFINITY};
+        for (int64_t c = I.begin + (int)threadIdx.x; c < I.end; c += NT) {
+            double q[NV];
+            ldq<NV>(Q + c * NV, q);
+            v[0] = fmin(v[0], local_dt<DIM>(q, a.u.sh, c, a.u.vol[c], gam, cfl));
+        }
+        block_partial<1, true, NT>(v, a.part_dt + it, 1);
+    }
+}
+
+// one block per case: its items' partials in item order -> L2 norms of R/V and the minimum time step
+struct BFin {
+    const int32_t *ci0, *ci1;  // cell-item range of each case (empty: inactive)
+    const double *part_l2, *part_dt;
+    const double *ncells;
+    double *l2, *dts;
+    int n_items, nv, want_l2, want_dt;
+};
+
+__global__ __launch_bounds__(FIN_B) void kb_finish(const BFin a) {
+    const int cs = blockIdx.x;
+    const int i0 = a.ci0[cs], i1 = a.ci1[cs];
+    if (i0 >= i1) return;
+    __shared__ double s[6][FIN_B];
+    double acc[6];
+    for (int k = 0; k < 5; k++) acc[k] = 0;
+    acc[5] = INFINITY;
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += FIN_B) {
+        if (a.want_l2)
+            for (int k = 0; k < a.nv; k++) acc[k] += a.part_l2[(int64_t)k * a.n_items + i];
+        if (a.want_dt) acc[5] = fmin(acc[5], a.part_dt[i]);
+    }
+    for (int k = 0; k < 6; k++) s[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int w = FIN_B / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int k = 0; k < 5; k++) s[k][threadIdx.x] += s[k][threadIdx.x + w];
+            s[5][threadIdx.x] = fmin(s[5][threadIdx.x], s[5][threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (a.want_l2)
+            for (int k = 0; k < a.nv; k++) a.l2[cs * a.nv + k] = sqrt(s[k][0] / a.ncells[cs]);
+        if (a.want_dt) a.dts[cs] = s[5][0];
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void kb_shadow(int64_t n, const double *__restrict__ Q, float *__restrict__ Q32) {
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) Q32[i] = (float)Q[i];
+}
+
+int bucket_of(int64_t nc) {
+    if (nc < 2048) return 0;
+    if (nc < 8192) return 1;
+    if (nc < 32768) return 2;
+    return 3;
+}
+
+}  // namespace
+
+struct mst_batch {
+    int n = 0, dim = 2, nv = 4, maxf = 3, slots = 3, order = 1, limiter = 0, rk = 1;
+    bool mixed = false;
+    std::vector<mst_config> cfgs;
+    std::vector<mst::Prepared> preps;
+    std::vector<int64_t> cstart, fstart;
+    int64_t NC = 0, NF = 0;
+    hipStream_t stream = nullptr;
+    int ncu = 0;
+    int32_t *fc0 = nullptr, *fc1 = nullptr, *finfo = nullptr, *ell = nullptr;
+    double *fn = nullptr, *fxf = nullptr, *feta = nullptr, *cvol = nullptr, *cxc = nullptr, *cshalf = nullptr;
+    double *Q[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *G = nullptr, *lg = nullptr;
+    float *Q32[4] = {nullptr, nullptr, nullptr, nullptr};
+    float *fn32 = nullptr, *feta32 = nullptr, *fd0 = nullptr, *fd1 = nullptr, *lg32 = nullptr;
+    Params<double> *P64 = nullptr;
+    Params<float> *P32 = nullptr;
+    int32_t *scheme = nullptr;
+    double *dts = nullptr, *gam = nullptr, *cfl = nullptr, *l2 = nullptr, *ncells = nullptr;
+    // work lists
+    Item *citems = nullptr, *fitems = nullptr;
+    int n_citems = 0, n_fitems = 0, cap_citems = 0, cap_fitems = 0;
+    int32_t *ci0 = nullptr, *ci1 = nullptr;
+    double *part = nullptr;
+    int *cnt = nullptr, *bad = nullptr;
+    int iold = 0, inew = 1;
+    bool dt_valid = false;
+    bool shadow_ok[4] = {false, false, false, false};
+    int block = 256;
+    int tiles[MST_BATCH_BUCKETS] = {1024, 2048, 4096, 8192};
+    std::vector<uint8_t> active;
+    bool use_graph = true;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    bool graphs_valid = false;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(T **p, size_t n) {
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc((void **)p, n * sizeof(T)));
+    return MST_OK;
+}
+template <typename T>
+int dupload(T **p, const std::vector<T> &v) {
+    int rc = dalloc(p, v.size());
+    if (rc) return rc;
+    if (!v.empty()) HIPCHK(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return MST_OK;
+}
+
+void drop_graphs(mst_batch *b) {
+    for (auto &g : b->gexec)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    b->graphs_valid = false;
+}
+
+void release(mst_batch *b) {
+    if (!b) return;
+    drop_graphs(b);
+    void *bufs[] = {b->fc0, b->fc1, b->finfo, b->ell, b->fn, b->fxf, b->feta, b->cvol, b->cxc, b->cshalf,
+                    b->Q[0], b->Q[1], b->Q[2], b->Q[3], b->G, b->lg, b->Q32[0], b->Q32[1], b->Q32[2], b->Q32[3],
+                    b->fn32, b->feta32, b->fd0, b->fd1, b->lg32, b->P64, b->P32, b->scheme, b->dts, b->gam,
+                    b->cfl, b->l2, b->ncells, b->citems, b->fitems, b->ci0, b->ci1, b->part, b->cnt, b->bad};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+// work items of the active cases for the current tiles; grows the device lists as needed
+int build_items(mst_batch *b) {
+    std::vector<Item> ci, fi;
+    std::vector<int32_t> c0(b->n, 0), c1(b->n, 0);
+    for (int i = 0; i < b->n; i++) {
+        const int64_t nc = b->cstart[i + 1] - b->cstart[i], nf = b->fstart[i + 1] - b->fstart[i];
+        c0[i] = c1[i] = (int32_t)ci.size();
+        if (!b->active[i]) continue;
+        const int64_t tc = b->tiles[bucket_of(nc)];
+        const int64_t tf = std::max<int64_t>(64, ((tc * nf / std::max<int64_t>(nc, 1)) + 63) / 64 * 64);
+        for (int64_t s = 0; s < nc; s += tc)
+            ci.push_back({i, (int32_t)(b->cstart[i] + s), (int32_t)(b->cstart[i] + std::min(nc, s + tc)), 0});
+        for (int64_t s = 0; s < nf; s += tf)
+            fi.push_back({i, (int32_t)(b->fstart[i] + s), (int32_t)(b->fstart[i] + std::min(nf, s + tf)), 0});
+        c1[i] = (int32_t)ci.size();
+    }
+    if ((int)ci.size() > b->cap_citems) {
+        if (b->citems) (void)hipFree(b->citems);
+        if (b->part) (void)hipFree(b->part);
+        b->citems = nullptr;
+        b->part = nullptr;
+        b->cap_citems = (int)ci.size();
+        int rc = dalloc(&b->citems, ci.size());
+        if (!rc) rc = dalloc(&b->part, ci.size() * (b->nv + 1));
+        if (rc) return rc;
+    }
+    if ((int)fi.size() > b->cap_fitems) {
+        if (b->fitems) (void)hipFree(b->fitems);
+        b->fitems = nullptr;
+        b->cap_fitems = (int)fi.size();
+        int rc = dalloc(&b->fitems, fi.size());
+        if (rc) return rc;
+    }
+    b->n_citems = (int)ci.size();
+    b->n_fitems = (int)fi.size();
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (!ci.empty()) HIPCHK(hipMemcpy(b->citems, ci.data(), sizeof(Item) * ci.size(), hipMemcpyHostToDevice));
+    if (!fi.empty()) HIPCHK(hipMemcpy(b->fitems, fi.data(), sizeof(Item) * fi.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->ci0, c0.data(), sizeof(int32_t) * b->n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->ci1, c1.data(), sizeof(int32_t) * b->n, hipMemcpyHostToDevice));
+    drop_graphs(b);
+    return MST_OK;
+}
+
+template <typename T>
+BArgs<T> base_args(const mst_batch *b, int isrc) {
+    BArgs<T> a;
+    memset(&a, 0, sizeof(a));
+    GradArgs<T> &g = a.g;
+    g.nc = b->NC;
+    g.pstart = 0;
+    g.pcount = 0;
+    g.pxf = nullptr;
+    g.ell = b->ell;
+    g.fc0 = b->fc0;
+    g.fc1 = b->fc1;
+    g.fxf = b->fxf;
+    g.cxc = b->cxc;
+    g.fd0 = b->fd0;
+    g.fd1 = b->fd1;
+    g.vol = b->cvol;
+    g.graw = nullptr;
+    g.phi = nullptr;
+    FluxArgs<T> &f = a.f;
+    f.nf = b->NF;
+    f.pstart = 0;
+    f.fc0 = b->fc0;
+    f.fc1 = b->fc1;
+    f.finfo = b->finfo;
+    f.fn = b->fn;
+    f.fxf = b->fxf;
+    f.pxf = nullptr;
+    f.cxc = b->cxc;
+    f.fd0 = b->fd0;
+    f.fd1 = b->fd1;
+    f.G = b->G;
+    if constexpr (std::is_same<T, float>::value) {
+        g.fn = b->fn32;
+        g.feta = b->feta32;
+        g.Q = b->Q32[isrc];
+        g.lg = b->lg32;
+        f.Q = b->Q32[isrc];
+        f.lg = b->lg32;
+        a.P = b->P32;
+    } else {
+        g.fn = b->fn;
+        g.feta = b->feta;
+        g.Q = b->Q[isrc];
+        g.lg = b->lg;
+        f.Q = b->Q[isrc];
+        f.lg = b->lg;
+        a.P = b->P64;
+    }
+    UpdArgs &u = a.u;
+    u.nc = b->NC;
+    u.ell = b->ell;
+    u.G = b->G;
+    u.vol = b->cvol;
+    u.sh = b->cshalf;
+    u.maxf = b->maxf;
+    u.bad = b->bad;
+    a.scheme = b->scheme;
+    a.dts = b->dts;
+    a.gam = b->gam;
+    a.cfl = b->cfl;
+    a.cnt = b->cnt;
+    a.part_l2 = b->part;
+    a.part_dt = b->part + (int64_t)b->nv * std::max(b->n_citems, 1);
+    return a;
+}
+
+template <typename K>
+int persistent_grid(const mst_batch *b, K kernel, int nt, int n_items, int *grid) {
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, nt, 0));
+    *grid = std::max(1, std::min(n_items, std::max(per_cu, 1) * b->ncu));
+    return MST_OK;
+}
+
+template <typename K, typename... A>
+int launch_persistent(mst_batch *b, K kernel, int nt, int n_items, const A &...args) {
+    if (n_items <= 0) return MST_OK;
+    int grid = 0;
+    int rc = persistent_grid(b, kernel, nt, n_items, &grid);
+    if (rc) return rc;
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(nt), 0, b->stream, args...);
+    HIPCHK(hipGetLastError());
+    return MST_OK;
+}
+
+template <int DIM, typename T, int NT>
+int run_grad(mst_batch *b, int isrc) {
+    BArgs<T> a = base_args<T>(b, isrc);
+    a.items = b->citems;
+    a.n_items = b->n_citems;
+    a.own = CNT_GRAD;
+#define MST_BG(MF)                                                                                    \
+    if (b->limiter == MST_LIM_NONE) return launch_persistent(b, kb_grad<DIM, MF, MST_LIM_NONE, T, NT>, NT, a.n_items, a); \
+    if (b->limiter == MST_LIM_MINMOD) return launch_persistent(b, kb_grad<DIM, MF, MST_LIM_MINMOD, T, NT>, NT, a.n_items, a); \
+    return launch_persistent(b, kb_grad<DIM, MF, MST_LIM_VANLEER, T, NT>, NT, a.n_items, a);
+    if (b->slots == 3) { MST_BG(3) }
+    if (b->slots == 4) { MST_BG(4) }
+    MST_BG(8)
+#undef MST_BG
+}
+
+template <int DIM, typename T, int NT>
+int run_flux(mst_batch *b, int isrc) {
+    BArgs<T> a = base_args<T>(b, isrc);
+    a.items = b->fitems;
+    a.n_items = b->n_fitems;
+    a.own = CNT_FLUX;
+    if (b->order >= 2) return launch_persistent(b, kb_flux<DIM, true, T, NT>, NT, a.n_items, a);
+    return launch_persistent(b, kb_flux<DIM, false, T, NT>, NT, a.n_items, a);
+}
+
+template <int DIM, typename T, int NT>
+int run_update(mst_batch *b, int is, int in, double ca, double cb, int stage0, int iout, bool l2, bool dt,
+               bool check) {
+    constexpr int NV = DIM + 2;
+    BArgs<T> a = base_args<T>(b, is);
+    a.items = b->citems;
+    a.n_items = b->n_citems;
+    a.own = CNT_UPD;
+    a.u.Qs = b->Q[is];
+    a.u.Qn = b->Q[in];
+    a.u.Qout = b->Q[iout];
+    a.u.Qout32 = b->mixed ? b->Q32[iout] : nullptr;
+    a.u.ca = ca;
+    a.u.cb = cb;
+    a.u.stage0 = stage0;
+    a.u.l2 = l2;
+    a.u.dtred = dt;
+    a.u.check = check;
+    if (b->slots == 3) return launch_persistent(b, kb_update<NV, 3, T, NT>, NT, a.n_items, a);
+    if (b->slots == 4) return launch_persistent(b, kb_update<NV, 4, T, NT>, NT, a.n_items, a);
+    return launch_persistent(b, kb_update<NV, 8, T, NT>, NT, a.n_items, a);
+}
+
+int run_finish(mst_batch *b, bool l2, bool dt) {
+    BFin f;
+    f.ci0 = b->ci0;
+    f.ci1 = b->ci1;
+    f.part_l2 = b->part;
+    f.part_dt = b->part + (int64_t)b->nv * std::max(b->n_citems, 1);
+    f.ncells = b->ncells;
+    f.l2 = b->l2;
+    f.dts = b->dts;
+    f.n_items = b->n_citems;
+    f.nv = b->nv;
+    f.want_l2 = l2;
+    f.want_dt = dt;
+    hipLaunchKernelGGL(kb_finish, dim3(b->n), dim3(FIN_B), 0, b->stream, f);
+    HIPCHK(hipGetLastError());
+    return MST_OK;
+}
+
+template <int DIM, typename T, int NT>
+int run_cfl_t(mst_batch *b) {
+    BArgs<T> a = base_args<T>(b, b->iold);
+    a.items = b->citems;
+    a.n_items = b->n_citems;
+    a.own = CNT_CFL;
+    int rc = launch_persistent(b, kb_cfl<DIM, T, NT>, NT, a.n_items, a, (const double *)b->Q[b->iold]);
+    if (rc) return rc;
+    return run_finish(b, false, true);
+}
+
+// one RK step from buffer iold to inew with the spare buffers between stages
+template <int DIM, typename T, int NT>
+int enqueue_step_t(mst_batch *b, int iold, int inew) {
+    static const double A2[2] = {0, 0.5}, B2[2] = {1, 0.5};
+    static const double A3[3] = {0, 0.75, 1.0 / 3.0}, B3[3] = {1, 0.25, 2.0 / 3.0};
+    const int ns = b->rk;
+    const double *A = ns == 3 ? A3 : A2, *B = ns == 3 ? B3 : B2;
+    int spare[2], w = 0;
+    for (int i = 0; i < 4; i++)
+        if (i != iold && i != inew) spare[w++] = i;
+    int src = iold;
+    for (int s = 0; s < ns; s++) {
+        const int dst = (s == ns - 1) ? inew : spare[s & 1];
+        int rc;
+        if (b->order >= 2) {
+            rc = run_grad<DIM, T, NT>(b, src);
+            if (rc) return rc;
+        }
+        rc = run_flux<DIM, T, NT>(b, src);
+        if (rc) return rc;
+        rc = run_update<DIM, T, NT>(b, src, iold, A[s], B[s], s == 0 ? 1 : 0, dst, s == 0, s == ns - 1, s == ns - 1);
+        if (rc) return rc;
+        src = dst;
+    }
+    return run_finish(b, true, true);
+}
+
+template <int DIM, typename T>
+int enqueue_step_nt(mst_batch *b, int iold, int inew) {
+    if (b->block == 64) return enqueue_step_t<DIM, T, 64>(b, iold, inew);
+    if (b->block == 128) return enqueue_step_t<DIM, T, 128>(b, iold, inew);
+    return enqueue_step_t<DIM, T, 256>(b, iold, inew);
+}
+
+int enqueue_step(mst_batch *b, int iold, int inew) {
+    if (b->mixed) return b->dim == 2 ? enqueue_step_nt<2, float>(b, iold, inew) : enqueue_step_nt<3, float>(b, iold, inew);
+    return b->dim == 2 ? enqueue_step_nt<2, double>(b, iold, inew) : enqueue_step_nt<3, double>(b, iold, inew);
+}
+
+int run_cfl(mst_batch *b) {
+    if (b->dim == 2) return b->mixed ? run_cfl_t<2, float, 256>(b) : run_cfl_t<2, double, 256>(b);
+    return b->mixed ? run_cfl_t<3, float, 256>(b) : run_cfl_t<3, double, 256>(b);
+}
+
+int ensure_shadow(mst_batch *b, int i) {
+    if (!b->mixed || b->shadow_ok[i]) return MST_OK;
+    const int64_t n = b->NC * b->nv;
+    hipLaunchKernelGGL(kb_shadow, dim3((uint32_t)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, b->stream, n,
+                       (const double *)b->Q[i], b->Q32[i]);
+    HIPCHK(hipGetLastError());
+    b->shadow_ok[i] = true;
+    return MST_OK;
+}
+
+int build_graphs(mst_batch *b) {
+    drop_graphs(b);
+    for (int par = 0; par < 2; par++) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_step(b, par, 1 - par);
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return mst::fail(MST_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&b->gexec[par], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return mst::fail(MST_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    b->graphs_valid = true;
+    return MST_OK;
+}
+
+int check_bad(mst_batch *b) {
+    std::vector<int> h(b->n, 0);
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(h.data(), b->bad, sizeof(int) * b->n, hipMemcpyDeviceToHost));
+    std::string list;
+    int count = 0;
+    for (int i = 0; i < b->n; i++)
+        if (h[i]) {
+            if (count < 32) list += (count ? ", " : "") + std::to_string(i);
+            count++;
+        }
+    if (count) {
+        HIPCHK(hipMemset(b->bad, 0, sizeof(int) * b->n));
+        return mst::fail(MST_E_STATE, "non-physical state (rho<=0, p<=0 or NaN, or no positive time step) produced by the update in " +
+                                          std::to_string(count) + " case(s): " + list + (count > 32 ? ", ..." : ""));
+    }
+    return MST_OK;
+}
+
+// the steps proper: iold/inew alternate; a graph per parity or direct launches
+int do_steps(mst_batch *b, int nsteps) {
+    if (b->n_citems == 0) return MST_OK;
+    int rc = ensure_shadow(b, b->iold);
+    if (rc) return rc;
+    if (!b->dt_valid) {
+        rc = run_cfl(b);
+        if (rc) return rc;
+        b->dt_valid = true;
+    }
+    if (b->use_graph && !b->graphs_valid) {
+        rc = build_graphs(b);
+        if (rc) return rc;
+    }
+    for (int it = 0; it < nsteps; it++) {
+        if (b->use_graph) {
+            HIPCHK(hipGraphLaunch(b->gexec[b->iold], b->stream));
+        } else {
+            rc = enqueue_step(b, b->iold, b->inew);
+            if (rc) return rc;
+        }
+        std::swap(b->iold, b->inew);
+    }
+    return MST_OK;
+}
+
+std::string signature(const mst_batch *b) {
+    int hist[MST_BATCH_BUCKETS] = {0, 0, 0, 0};
+    int64_t cells = 0;
+    for (int i = 0; i < b->n; i++) {
+        if (!b->active[i]) continue;
+        const int64_t nc = b->cstart[i + 1] - b->cstart[i];
+        hist[bucket_of(nc)]++;
+        cells += nc;
+    }
+    std::ostringstream s;
+    s << "dim" << b->dim << "-o" << b->order << "-l" << b->limiter << "-rk" << b->rk << "-p" << (b->mixed ? 1 : 0)
+      << "-h" << hist[0] << "." << hist[1] << "." << hist[2] << "." << hist[3] << "-c" << (cells >> 16);
+    return s.str();
+}
+
+// cache file: one JSON object, one entry per line  "key": {"block": B, "tiles": [t0, t1, t2, t3], "ms": x}
+bool cache_lookup(const std::string &path, const std::string &key, int *block, int *tiles, double *ms) {
+    std::ifstream in(path);
+    if (!in) return false;
+    std::string line, want = "\"" + key + "\"";
+    while (std::getline(in, line)) {
+        if (line.find(want) == std::string::npos) continue;
+        const char *p = strstr(line.c_str(), "{\"block\":");
+        if (!p) continue;
+        if (sscanf(p, "{\"block\": %d, \"tiles\": [%d, %d, %d, %d], \"ms\": %lf}", block, &tiles[0], &tiles[1],
+                   &tiles[2], &tiles[3], ms) == 6)
+            return true;
+    }
+    return false;
+}
+
+void cache_store(const std::string &path, const std::string &key, int block, const int *tiles, double ms) {
+    std::vector<std::string> lines;
+    {
+        std::ifstream in(path);
+        std::string line, want = "\"" + key + "\"";
+        while (in && std::getline(in, line))
+            if (line.find("{\"block\":") != std::string::npos && line.find(want) == std::string::npos) {
+                while (!line.empty() && (line.back() == ',' || line.back() == ' ')) line.pop_back();
+                lines.push_back(line);
+            }
+    }
+    char buf[256];
+    snprintf(buf, sizeof(buf), "  \"%s\": {\"block\": %d, \"tiles\": [%d, %d, %d, %d], \"ms\": %.6f}", key.c_str(),
+             block, tiles[0], tiles[1], tiles[2], tiles[3], ms);
+    lines.push_back(buf);
+    std::ofstream out(path, std::ios::trunc);
+    out << "{\n";
+    for (size_t i = 0; i < lines.size(); i++) out << lines[i] << (i + 1 < lines.size() ? ",\n" : "\n");
+    out << "}\n";
+}
+
+// time nsteps steps with the current tiles/block (graph replay), after 2 warm-up steps
+int time_steps(mst_batch *b, int nsteps, double *ms) {
+    int rc = build_items(b);
+    if (rc) return rc;
+    rc = do_steps(b, 2);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, b->stream));
+    rc = do_steps(b, nsteps);
+    if (rc) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return rc;
+    }
+    HIPCHK(hipEventRecord(e1, b->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t / nsteps;
+    return MST_OK;
+}
+
+int check_cfg(const mst_config *c) {
+    if (c->scheme != MST_SCHEME_ROE && c->scheme != MST_SCHEME_AUSM) return mst::fail(MST_E_ARG, "bad scheme");
+    if (c->order != 1 && c->order != 2) return mst::fail(MST_E_ARG, "order must be 1 or 2");
+    if (c->limiter < MST_LIM_NONE || c->limiter > MST_LIM_VANLEER) return mst::fail(MST_E_ARG, "bad limiter");
+    if (c->rk < 1 || c->rk > 3) return mst::fail(MST_E_ARG, "rk must be 1, 2 or 3");
+    if (c->precision != MST_PREC_FP64 && c->precision != MST_PREC_MIXED) return mst::fail(MST_E_ARG, "bad precision");
+    if (!(c->gamma > 1) || !(c->entropy_delta > 0) || c->eor < 0 || !(c->cfl > 0))
+        return mst::fail(MST_E_ARG, "bad gamma / entropy_delta / eor, or cfl not positive (batched cases "
+                                    "advance with their own CFL time step)");
+    if (!(c->q_inlet[0] > 0)) return mst::fail(MST_E_ARG, "inlet density must be positive");
+    return MST_OK;
+}
+
+}  // namespace
+
+#define B_GUARD(b) \
+    if (!(b)) return mst::fail(MST_E_ARG, "null batch");
+
+extern "C" {
+
+int mst_batch_create(int n_cases, const mst_mesh *const *meshes, const mst_config *cfgs, mst_batch **out) {
+    if (!out || n_cases <= 0 || !meshes || !cfgs) return mst::fail(MST_E_ARG, "bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < n_cases; i++) {
+        if (!meshes[i]) return mst::fail(MST_E_ARG, "null mesh");
+        int rc = check_cfg(&cfgs[i]);
+        if (rc) return rc;
+        const mst_config &a = cfgs[0], &c = cfgs[i];
+        if (c.order != a.order || c.limiter != a.limiter || c.rk != a.rk || c.precision != a.precision ||
+            meshes[i]->dim != meshes[0]->dim)
+            return mst::fail(MST_E_ARG, "batched cases must share dim, order, limiter, rk and precision");
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return mst::fail(MST_E_HIP, "no HIP device available: the batched path runs only on the GPU");
+    mst_batch *b = new (std::nothrow) mst_batch();
+    if (!b) return mst::fail(MST_E_ARG, "out of host memory");
+    b->n = n_cases;
+    b->cfgs.assign(cfgs, cfgs + n_cases);
+    b->preps.resize(n_cases);
+    b->dim = meshes[0]->dim;
+    b->nv = b->dim + 2;
+    b->order = cfgs[0].order;
+    b->limiter = cfgs[0].limiter;
+    b->rk = cfgs[0].rk;
+    b->mixed = cfgs[0].precision == MST_PREC_MIXED;
+    b->cstart.assign(n_cases + 1, 0);
+    b->fstart.assign(n_cases + 1, 0);
+    b->active.assign(n_cases, 1);
+    b->maxf = 0;
+    int rc = MST_OK;
+    try {
+        for (int i = 0; i < n_cases && !rc; i++) {
+            rc = mst::prepare(*meshes[i], b->preps[i]);
+            if (rc) break;
+            const mst::Prepared &P = b->preps[i];
+            if (P.bad_cells > 0) rc = mst::fail(MST_E_MESH, "case " + std::to_string(i) + ": degenerate cells");
+            else if (P.pcount > 0) rc = mst::fail(MST_E_UNSUPPORTED, "periodic faces are not supported in batches");
+            b->cstart[i + 1] = b->cstart[i] + P.nc;
+            b->fstart[i + 1] = b->fstart[i] + P.nf;
+            b->maxf = std::max(b->maxf, P.maxf);
+        }
+    } catch (const std::bad_alloc &) {
+        rc = mst::fail(MST_E_ARG, "out of host memory");
+    }
+    if (!rc && (b->cstart[n_cases] >= INT32_MAX / 2 || b->fstart[n_cases] >= INT32_MAX / 2))
+        rc = mst::fail(MST_E_UNSUPPORTED, "batch too large for 32-bit face indices");
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    b->NC = b->cstart[n_cases];
+    b->NF = b->fstart[n_cases];
+    b->slots = b->maxf <= 3 ? 3 : (b->maxf <= 4 ? 4 : 8);
+    const int dim = b->dim, nv = b->nv;
+    // concatenated arrays (global indices)
+    std::vector<int32_t> fc0(b->NF), fc1(b->NF), finfo(b->NF), ell((size_t)b->slots * b->NC, -1), scheme(n_cases);
+    std::vector<double> fn(b->NF * dim), fxf(b->NF * dim), feta(b->NF), cvol(b->NC), cxc(b->NC * dim),
+        cshalf(b->NC * dim), gam(n_cases), cflv(n_cases), ncells(n_cases);
+    std::vector<float> d0, d1;
+    if (b->mixed) {
+        d0.assign(b->NF * dim, 0.f);
+        d1.assign(b->NF * dim, 0.f);
+    }
+    std::vector<Params<double>> p64(n_cases);
+    std::vector<Params<float>> p32(n_cases);
+    for (int i = 0; i < n_cases; i++) {
+        const mst::Prepared &P = b->preps[i];
+        const mst_config &c = cfgs[i];
+        const int64_t cs = b->cstart[i], fs = b->fstart[i];
+        std::vector<int32_t> mask;
+        mst::lr_masks(P, c.compat, mask);
+        for (int64_t f = 0; f < P.nf; f++) {
+            fc0[fs + f] = (int32_t)(P.fc0[f] + cs);
+            fc1[fs + f] = P.fc1[f] >= 0 ? (int32_t)(P.fc1[f] + cs) : -1;
+            finfo[fs + f] = P.fkind[f] | (mask[f] << 4);
+            feta[fs + f] = P.feta[f];
+            for (int d = 0; d < dim; d++) {
+                fn[(fs + f) * dim + d] = P.fn[f * dim + d];
+                fxf[(fs + f) * dim + d] = P.fxf[f * dim + d];
+                if (b->mixed) {
+                    d0[(fs + f) * dim + d] = (float)(P.fxf[f * dim + d] - P.cxc[P.fc0[f] * dim + d]);
+                    if (P.fc1[f] >= 0) d1[(fs + f) * dim + d] = (float)(P.fxf[f * dim + d] - P.cxc[P.fc1[f] * dim + d]);
+                }
+            }
+        }
+        if ((c.compat & MST_COMPAT_GRAD_LASTINT) && P.quirk_face >= 0) feta[fs + P.quirk_face] = 1.0;
+        for (int64_t k = 0; k < P.nc; k++) {
+            cvol[cs + k] = P.cvol[k];
+            for (int d = 0; d < dim; d++) {
+                cxc[(cs + k) * dim + d] = P.cxc[k * dim + d];
+                cshalf[(cs + k) * dim + d] = P.cshalf[k * dim + d];
+            }
+            for (int s = 0; s < P.maxf; s++) {
+                const int32_t e = P.ell[(int64_t)s * P.nc + k];
+                if (e >= 0) ell[(size_t)s * b->NC + cs + k] = (int32_t)((((int64_t)(e >> 1) + fs) << 1) | (e & 1));
+            }
+        }
+        p64[i] = make_params<double>(c);
+        p32[i] = make_params<float>(c);
+        scheme[i] = c.scheme;
+        gam[i] = c.gamma;
+        cflv[i] = c.cfl;
+        ncells[i] = (double)P.nc;
+    }
+#define TRY(e)            \
+    do {                  \
+        int _r = (e);     \
+        if (_r) {         \
+            release(b);   \
+            return _r;    \
+        }                 \
+    } while (0)
+    if (hipStreamCreate(&b->stream) != hipSuccess) {
+        delete b;
+        return mst::fail(MST_E_HIP, "hipStreamCreate failed");
+    }
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        release(b);
+        return mst::fail(MST_E_HIP, "device query failed");
+    }
+    b->ncu = prop.multiProcessorCount;
+    TRY(dupload(&b->fc0, fc0));
+    TRY(dupload(&b->fc1, fc1));
+    TRY(dupload(&b->finfo, finfo));
+    TRY(dupload(&b->ell, ell));
+    TRY(dupload(&b->fn, fn));
+    TRY(dupload(&b->fxf, fxf));
+    TRY(dupload(&b->feta, feta));
+    TRY(dupload(&b->cvol, cvol));
+    TRY(dupload(&b->cxc, cxc));
+    TRY(dupload(&b->cshalf, cshalf));
+    TRY(dupload(&b->P64, p64));
+    TRY(dupload(&b->P32, p32));
+    TRY(dupload(&b->scheme, scheme));
+    TRY(dupload(&b->gam, gam));
+    TRY(dupload(&b->cfl, cflv));
+    TRY(dupload(&b->ncells, ncells));
+    TRY(dalloc(&b->dts, n_cases));
+    TRY(dalloc(&b->l2, (size_t)n_cases * nv));
+    TRY(dalloc(&b->ci0, n_cases));
+    TRY(dalloc(&b->ci1, n_cases));
+    for (int i = 0; i < 4; i++) TRY(dalloc(&b->Q[i], b->NC * nv));
+    TRY(dalloc(&b->G, b->NF * nv));
+    TRY(dalloc(&b->cnt, NCNT));
+    TRY(dalloc(&b->bad, n_cases));
+    if (b->order >= 2 && !b->mixed) TRY(dalloc(&b->lg, b->NC * nv * dim));
+    if (b->mixed) {
+        for (int i = 0; i < 4; i++) TRY(dalloc(&b->Q32[i], b->NC * nv));
+        std::vector<float> fn32(fn.begin(), fn.end()), e32(feta.begin(), feta.end());
+        TRY(dupload(&b->fn32, fn32));
+        TRY(dupload(&b->feta32, e32));
+        TRY(dupload(&b->fd0, d0));
+        TRY(dupload(&b->fd1, d1));
+        if (b->order >= 2) TRY(dalloc(&b->lg32, b->NC * nv * dim));
+    }
+    if (hipMemset(b->cnt, 0, sizeof(int) * NCNT) != hipSuccess || hipMemset(b->bad, 0, sizeof(int) * n_cases) != hipSuccess ||
+        hipMemset(b->l2, 0, sizeof(double) * n_cases * nv) != hipSuccess ||
+        hipMemset(b->dts, 0, sizeof(double) * n_cases) != hipSuccess) {
+        release(b);
+        return mst::fail(MST_E_HIP, "hipMemset failed");
+    }
+    TRY(build_items(b));
+#undef TRY
+    *out = b;
+    return MST_OK;
+}
+
+int mst_batch_destroy(mst_batch *b) {
+    if (!b) return MST_OK;
+    (void)hipStreamSynchronize(b->stream);
+    release(b);
+    return MST_OK;
+}
+
+int mst_batch_counts(const mst_batch *b, int64_t *out) {
+    B_GUARD(b);
+    if (!out) return mst::fail(MST_E_ARG, "null argument");
+    out[0] = b->n;
+    out[1] = b->NC;
+    out[2] = b->NF;
+    out[3] = b->n_citems;
+    out[4] = b->n_fitems;
+    out[5] = b->dim;
+    return MST_OK;
+}
+
+int mst_batch_case_counts(const mst_batch *b, int case_id, int64_t *n_cells, int64_t *n_faces) {
+    B_GUARD(b);
+    if (case_id < 0 || case_id >= b->n || !n_cells || !n_faces) return mst::fail(MST_E_ARG, "bad case");
+    *n_cells = b->cstart[case_id + 1] - b->cstart[case_id];
+    *n_faces = b->fstart[case_id + 1] - b->fstart[case_id];
+    return MST_OK;
+}
+
+int mst_batch_set_state(mst_batch *b, int case_id, const double *Q, int64_t n_cells) {
+    B_GUARD(b);
+    if (case_id < 0 || case_id >= b->n || !Q) return mst::fail(MST_E_ARG, "bad case");
+    const mst::Prepared &P = b->preps[case_id];
+    if (n_cells != P.nc) return mst::fail(MST_E_ARG, "state size does not match the case's mesh");
+    const int nv = b->nv;
+    std::vector<double> buf(P.nc * nv);
+    for (int64_t i = 0; i < P.nc; i++)
+        for (int k = 0; k < nv; k++) buf[i * nv + k] = Q[(int64_t)P.cell_old_of_new[i] * nv + k];
+    HIPCHK(hipStreamSynchronize(b->stream));
+    // an inactive case keeps its state in every buffer (the buffers rotate under it)
+    for (int i = 0; i < 4; i++)
+        if (i == b->iold || !b->active[case_id])
+            HIPCHK(hipMemcpy(b->Q[i] + b->cstart[case_id] * nv, buf.data(), sizeof(double) * buf.size(),
+                             hipMemcpyHostToDevice));
+    for (int i = 0; i < 4; i++) b->shadow_ok[i] = false;
+    b->dt_valid = false;
+    return MST_OK;
+}
+
+int mst_batch_get_state(mst_batch *b, int case_id, double *Q, int64_t n_cells) {
+    B_GUARD(b);
+    if (case_id < 0 || case_id >= b->n || !Q) return mst::fail(MST_E_ARG, "bad case");
+    const mst::Prepared &P = b->preps[case_id];
+    if (n_cells != P.nc) return mst::fail(MST_E_ARG, "state size does not match the case's mesh");
+    const int nv = b->nv;
+    std::vector<double> buf(P.nc * nv);
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(buf.data(), b->Q[b->iold] + b->cstart[case_id] * nv, sizeof(double) * buf.size(),
+                     hipMemcpyDeviceToHost));
+    for (int64_t c = 0; c < P.nc; c++)
+        for (int k = 0; k < nv; k++) Q[c * nv + k] = buf[(int64_t)P.cell_new_of_old[c] * nv + k];
+    return MST_OK;
+}
+
+int mst_batch_step(mst_batch *b, int nsteps) {
+    B_GUARD(b);
+    if (nsteps < 0) return mst::fail(MST_E_ARG, "nsteps must be >= 0");
+    int rc = do_steps(b, nsteps);
+    if (rc) return rc;
+    return check_bad(b);
+}
+
+int mst_batch_residual_norms(mst_batch *b, double *out) {
+    B_GUARD(b);
+    if (!out) return mst::fail(MST_E_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(out, b->l2, sizeof(double) * b->n * b->nv, hipMemcpyDeviceToHost));
+    return MST_OK;
+}
+
+int mst_batch_dt(mst_batch *b, double *out) {
+    B_GUARD(b);
+    if (!out) return mst::fail(MST_E_ARG, "null argument");
+    if (!b->dt_valid) {
+        int rc = ensure_shadow(b, b->iold);
+        if (!rc) rc = run_cfl(b);
+        if (rc) return rc;
+        b->dt_valid = true;
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(out, b->dts, sizeof(double) * b->n, hipMemcpyDeviceToHost));
+    return MST_OK;
+}
+
+int mst_batch_set_active(mst_batch *b, const uint8_t *active) {
+    B_GUARD(b);
+    if (!active) return mst::fail(MST_E_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const int nv = b->nv;
+    for (int i = 0; i < b->n; i++) {
+        const uint8_t a = active[i] ? 1 : 0;
+        if (a == b->active[i]) continue;
+        if (!a) {  // freeze: the current state goes to every buffer
+            const size_t off = b->cstart[i] * nv, bytes = sizeof(double) * (b->cstart[i + 1] - b->cstart[i]) * nv;
+            for (int k = 0; k < 4; k++)
+                if (k != b->iold) HIPCHK(hipMemcpy(b->Q[k] + off, b->Q[b->iold] + off, bytes, hipMemcpyDeviceToDevice));
+            for (int k = 0; k < 4; k++) b->shadow_ok[k] = false;
+        }
+        b->active[i] = a;
+    }
+    return build_items(b);
+}
+
+int mst_batch_set_tuning(mst_batch *b, int block, const int32_t *tiles) {
+    B_GUARD(b);
+    if (block != 64 && block != 128 && block != 256) return mst::fail(MST_E_ARG, "block must be 64, 128 or 256");
+    if (!tiles) return mst::fail(MST_E_ARG, "null argument");
+    for (int k = 0; k < MST_BATCH_BUCKETS; k++)
+        if (tiles[k] < 64 || tiles[k] > (1 << 24)) return mst::fail(MST_E_ARG, "tile out of range [64, 2^24]");
+    b->block = block;
+    for (int k = 0; k < MST_BATCH_BUCKETS; k++) b->tiles[k] = tiles[k];
+    return build_items(b);
+}
+
+int mst_batch_get_tuning(const mst_batch *b, int *block, int32_t *tiles) {
+    B_GUARD(b);
+    if (!block || !tiles) return mst::fail(MST_E_ARG, "null argument");
+    *block = b->block;
+    for (int k = 0; k < MST_BATCH_BUCKETS; k++) tiles[k] = b->tiles[k];
+    return MST_OK;
+}
+
+int mst_batch_set_graph(mst_batch *b, int on) {
+    B_GUARD(b);
+    b->use_graph = on != 0;
+    return MST_OK;
+}
+
+int mst_batch_autotune(mst_batch *b, const char *cache_path, int force, int steps, double *ms_per_step) {
+    B_GUARD(b);
+    if (steps < 1) return mst::fail(MST_E_ARG, "steps must be >= 1");
+    const std::string key = signature(b);
+    int tiles[MST_BATCH_BUCKETS];
+    int block = 0;
+    double ms = 0;
+    if (!force && cache_path && cache_lookup(cache_path, key, &block, tiles, &ms)) {
+        int rc = mst_batch_set_tuning(b, block, tiles);
+        if (rc) return rc;
+        if (ms_per_step) *ms_per_step = ms;
+        return MST_OK;
+    }
+    // the search advances the solution: keep the state and restore it afterwards
+    const size_t bytes = sizeof(double) * b->NC * b->nv;
+    double *save = nullptr;
+    HIPCHK(hipMalloc((void **)&save, bytes));
+    const int iold0 = b->iold, inew0 = b->inew;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(save, b->Q[b->iold], bytes, hipMemcpyDeviceToDevice));
+    int hist[MST_BATCH_BUCKETS] = {0, 0, 0, 0};
+    for (int i = 0; i < b->n; i++)
+        if (b->active[i]) hist[bucket_of(b->cstart[i + 1] - b->cstart[i])]++;
+    static const int cand_tiles[] = {256, 512, 1024, 2048, 4096, 8192, 16384};
+    static const int cand_blocks[] = {64, 128, 256};
+    int best_block = b->block, best_tiles[MST_BATCH_BUCKETS];
+    for (int k = 0; k < MST_BATCH_BUCKETS; k++) best_tiles[k] = b->tiles[k];
+    double best = INFINITY;
+    int rc = MST_OK;
+    auto trial = [&](int blk, const int *t) -> int {
+        // every trial starts from the saved state, so all trials time the same steps
+        HIPCHK(hipStreamSynchronize(b->stream));
+        b->iold = iold0;
+        b->inew = inew0;
+        HIPCHK(hipMemcpy(b->Q[b->iold], save, bytes, hipMemcpyDeviceToDevice));
+        for (int k = 0; k < 4; k++) b->shadow_ok[k] = false;
+        b->dt_valid = false;
+        b->block = blk;
+        for (int k = 0; k < MST_BATCH_BUCKETS; k++) b->tiles[k] = t[k];
+        double m = 0;
+        int r = time_steps(b, steps, &m);
+        if (r) return r;
+        if (m < best) {
+            best = m;
+            best_block = blk;
+            for (int k = 0; k < MST_BATCH_BUCKETS; k++) best_tiles[k] = t[k];
+        }
+        return MST_OK;
+    };
+    // coordinate search: block size, then each populated bucket's tile, then block size again
+    for (int pass = 0; pass < 2 && !rc; pass++) {
+        for (int blk : cand_blocks) {
+            if (pass == 1 && blk == best_block) continue;
+            rc = trial(blk, best_tiles);
+            if (rc) break;
+        }
+        if (pass == 1 || rc) break;
+        for (int k = 0; k < MST_BATCH_BUCKETS && !rc; k++) {
+            if (!hist[k]) continue;
+            for (int t : cand_tiles) {
+                if (t == best_tiles[k]) continue;
+                int tt[MST_BATCH_BUCKETS];
+                for (int j = 0; j < MST_BATCH_BUCKETS; j++) tt[j] = best_tiles[j];
+                tt[k] = t;
+                rc = trial(best_block, tt);
+                if (rc) break;
+            }
+        }
+    }
+    // restore the state and apply the best shape
+    if (hipStreamSynchronize(b->stream) == hipSuccess) {
+        b->iold = iold0;
+        b->inew = inew0;
+        (void)hipMemcpy(b->Q[b->iold], save, bytes, hipMemcpyDeviceToDevice);
+    }
+    (void)hipFree(save);
+    for (int k = 0; k < 4; k++) b->shadow_ok[k] = false;
+    b->dt_valid = false;
+    if (rc) return rc;
+    int rc2 = check_bad(b);
+    if (rc2) return rc2;
+    rc = mst_batch_set_tuning(b, best_block, best_tiles);
+    if (rc) return rc;
+    if (cache_path) cache_store(cache_path, key, best_block, best_tiles, best);
+    if (ms_per_step) *ms_per_step = best;
+    return MST_OK;
+}
+
+}  // extern "C"

@@ -138,6 +138,22 @@ int mst_mesh_gen_tri(int nx, int ny, double x0, double x1, double y0, double y1,
     return MST_OK;
 }
 
+int mst_mesh_gen_ramp(int nx, int ny, double x0, double x1, double height, double theta_deg, double jitter,
+                      uint64_t seed, const int32_t bc[4], mst_mesh **out) {
+    if (!out || !bc) return mst::fail(MST_E_ARG, "null argument");
+    *out = nullptr;
+    MST_GUARD({
+        mst_mesh *m = new mst_mesh();
+        int rc = mst::gen_ramp(nx, ny, x0, x1, height, theta_deg, jitter, seed, bc, *m);
+        if (rc) {
+            delete m;
+            return rc;
+        }
+        *out = m;
+    });
+    return MST_OK;
+}
+
 int mst_mesh_gen_tet(int nx, int ny, int nz, const double lo[3], const double hi[3], double jitter, uint64_t seed,
                      const int32_t bc[6], mst_mesh **out) {
     if (!out || !bc || !lo || !hi) return mst::fail(MST_E_ARG, "null argument");

@@ -166,11 +166,9 @@ struct GradArgs {
 // the outward normals of a closed cell sum to zero, and free of the cancellation an FP32 sum of
 // whole states would suffer.
 template <int DIM, int MAXF, int LIM, typename T>
-__global__ __launch_bounds__(BLOCK) void k_grad(const GradArgs<T> a) {
+__device__ __forceinline__ void grad_cell(const GradArgs<T> &a, const int64_t c) {
     constexpr int NV = DIM + 2;
     constexpr bool F32 = std::is_same<T, float>::value;
-    const int64_t c = block_id() * BLOCK + threadIdx.x;
-    if (c >= a.nc) return;
     T qc[NV], qmax[NV], qmin[NV], t[NV][DIM];
     ldq<NV>(a.Q + c * NV, qc);
 #pragma unroll
@@ -297,6 +295,12 @@ __global__ __launch_bounds__(BLOCK) void k_grad(const GradArgs<T> a) {
     }
 }
 
+template <int DIM, int MAXF, int LIM, typename T>
+__global__ __launch_bounds__(BLOCK) void k_grad(const GradArgs<T> a) {
+    const int64_t c = block_id() * BLOCK + threadIdx.x;
+    if (c < a.nc) grad_cell<DIM, MAXF, LIM, T>(a, c);
+}
+
 // ------------------------------------------------------------------------------------ gather + update
 struct UpdArgs {
     int64_t nc;
@@ -311,6 +315,7 @@ struct UpdArgs {
     int stage0;
     int maxf;       // ELL slots (cell strategy)
     int l2, dtred;  // cell strategy: reduce |R/V|^2 / the CFL time step in this launch
+    int check;      // flag rho <= 0 / p <= 0 / NaN in the output (the state that ends a step)
     double gamma, cfl;
     double *__restrict__ Qout;
     float *__restrict__ Qout32;     // FP32 shadow of Qout (mixed form), else null
@@ -321,19 +326,23 @@ struct UpdArgs {
 };
 
 // mode bits: UPD_STAGE writes the stage update (else only R), UPD_L2 reduces |R/V|^2,
-// UPD_DT reduces the CFL time step of the updated state (for the next step)
-enum { UPD_RESIDUAL = 0, UPD_STAGE = 1, UPD_L2 = 2, UPD_DT = 4 };
+// UPD_DT reduces the CFL time step of the updated state (for the next step); UPD_CHECK (a runtime
+// flag, not a kernel variant) checks the output for a non-physical state.  Intermediate RK stages
+// are not checked: an SSP stage may leave the admissible set and the step's final combination
+// return to it (the reference checks nothing at all).
+enum { UPD_RESIDUAL = 0, UPD_STAGE = 1, UPD_L2 = 2, UPD_DT = 4, UPD_CHECK = 8 };
 
 // Block-level reduction of W values (fixed tree order): sums, or a min when MIN is set; thread 0..W-1
 // of the block write the results.  The per-block partials are combined by k_finish in block order,
 // so every reduction is bit-reproducible (no float atomics, no cross-block fences).
-template <int W, bool MIN>
+template <int W, bool MIN, int NT = BLOCK>
 __device__ __forceinline__ void block_partial(double (&v)[W], double *__restrict__ out, int64_t stride) {
-    __shared__ double sh[W][BLOCK];
+    __shared__ double sh[W][NT];
+    __syncthreads();  // a previous call's readers of sh[.][0] are done (persistent loops reuse it)
 #pragma unroll
     for (int k = 0; k < W; k++) sh[k][threadIdx.x] = v[k];
     __syncthreads();
-    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    for (int w = NT / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
 #pragma unroll
             for (int k = 0; k < W; k++) {
@@ -380,12 +389,50 @@ __device__ __forceinline__ void accumulate(double (&r)[NV], double (&cmp)[NV], c
 }
 
 // Stage update of cell c from its residual r (RhoSolver::solve, RhoSolver.cpp:57-66, in SSP-RK stage
-// form), the non-physical-state flag, and the per-block partials of |R/V|^2 and the CFL time step.
+// form) and the non-physical-state flag; red[] receives this cell's |R/V|^2 (L2) and CFL time step (DT)
+template <int NV, bool F32>
+__device__ __forceinline__ void stage_cell(const UpdArgs &a, int64_t c, const double (&r)[NV], double (&red)[NV + 1],
+                                           const bool L2, const bool DT) {
+    constexpr int DIM = NV - 2;
+    const double V = a.vol[c];
+    const double dtv = a.dt[0] / V;
+    double qs[NV], out[NV];
+    ldq<NV>(a.Qs + c * NV, qs);
+    if (a.stage0) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) out[k] = qs[k] - dtv * r[k];
+    } else {
+        double qn[NV];
+        ldq<NV>(a.Qn + c * NV, qn);
+#pragma unroll
+        for (int k = 0; k < NV; k++) out[k] = a.ca * qn[k] + a.cb * (qs[k] - dtv * r[k]);
+    }
+    stq<NV>(a.Qout + c * NV, out);
+    if constexpr (F32) stq<NV>(a.Qout32 + c * NV, out);
+    if (a.check) {
+        double ke = 0;
+#pragma unroll
+        for (int k = 1; k < NV - 1; k++) ke += out[k] * out[k];
+        const double p = (out[NV - 1] - 0.5 * ke / out[0]) * (a.gamma - 1);
+        if (!(out[0] > 0) || !(p > 0)) a.bad[0] = 1;
+    }
+    if (!(a.dt[0] > 0)) a.bad[0] = 1;  // a NaN or non-physical state gave no positive CFL time step
+    if (L2) {
+        const double iV = frcp(V);
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            const double x = r[k] * iV;
+            red[k] += x * x;
+        }
+    }
+    if (DT) red[NV] = fmin(red[NV], local_dt<DIM>(out, a.sh, c, V, a.gamma, a.cfl));
+}
+
+// stage_cell for the lanes of one block, then the block's partials of |R/V|^2 and the CFL time step.
 // Every lane of the block must call it (block-level reductions).
 template <int NV, bool STAGE, bool F32>
 __device__ __forceinline__ void stage_tail(const UpdArgs &a, int64_t c, bool valid, const double (&r)[NV],
                                            int64_t blk, const bool L2, const bool DT) {
-    constexpr int DIM = NV - 2;
     if constexpr (!STAGE) {
         if (valid) stq<NV>(a.R + c * NV, r);
         return;
@@ -394,37 +441,7 @@ __device__ __forceinline__ void stage_tail(const UpdArgs &a, int64_t c, bool val
 #pragma unroll
         for (int k = 0; k < NV; k++) red[k] = 0;
         red[NV] = INFINITY;
-        if (valid) {
-            const double V = a.vol[c];
-            const double dtv = a.dt[0] / V;
-            double qs[NV], out[NV];
-            ldq<NV>(a.Qs + c * NV, qs);
-            if (a.stage0) {
-#pragma unroll
-                for (int k = 0; k < NV; k++) out[k] = qs[k] - dtv * r[k];
-            } else {
-                double qn[NV];
-                ldq<NV>(a.Qn + c * NV, qn);
-#pragma unroll
-                for (int k = 0; k < NV; k++) out[k] = a.ca * qn[k] + a.cb * (qs[k] - dtv * r[k]);
-            }
-            stq<NV>(a.Qout + c * NV, out);
-            if constexpr (F32) stq<NV>(a.Qout32 + c * NV, out);
-            double ke = 0;
-#pragma unroll
-            for (int k = 1; k < NV - 1; k++) ke += out[k] * out[k];
-            const double p = (out[NV - 1] - 0.5 * ke / out[0]) * (a.gamma - 1);
-            if (!(out[0] > 0) || !(p > 0)) a.bad[0] = 1;
-            if (L2) {
-                const double iV = frcp(V);
-#pragma unroll
-                for (int k = 0; k < NV; k++) {
-                    const double x = r[k] * iV;
-                    red[k] = x * x;
-                }
-            }
-            if (DT) red[NV] = local_dt<DIM>(out, a.sh, c, V, a.gamma, a.cfl);
-        }
+        if (valid) stage_cell<NV, F32>(a, c, r, red, L2, DT);
         if (L2) {  // uniform over the launch
             double v[NV];
 #pragma unroll
@@ -569,7 +586,7 @@ __device__ __forceinline__ double fin_op(int kind, double a, double b) {
     return kind == 0 ? a + b : (kind == 1 ? fmin(a, b) : (a < b ? b : a));
 }
 
-__global__ __launch_bounds__(FIN) void k_finish(const FinArgs a) {
+static __global__ __launch_bounds__(FIN) void k_finish(const FinArgs a) {
     constexpr int MAXW = 12;
     __shared__ double s[MAXW][FIN / 64];
     double acc[MAXW];
@@ -597,13 +614,13 @@ __global__ __launch_bounds__(FIN) void k_finish(const FinArgs a) {
 }
 
 // FP32 shadow of a state (mixed form), after the host writes the FP64 state
-__global__ __launch_bounds__(BLOCK) void k_shadow(int64_t n, const double *__restrict__ Q, float *__restrict__ Q32) {
+static __global__ __launch_bounds__(BLOCK) void k_shadow(int64_t n, const double *__restrict__ Q, float *__restrict__ Q32) {
     const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i < n) Q32[i] = (float)Q[i];
 }
 
 // plain streaming copy: the reference point that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes
-__global__ __launch_bounds__(BLOCK) void k_copy(int64_t n2, const double2 *__restrict__ src,
+static __global__ __launch_bounds__(BLOCK) void k_copy(int64_t n2, const double2 *__restrict__ src,
                                                 double2 *__restrict__ dst) {
     const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i < n2) dst[i] = src[i];

@@ -51,6 +51,10 @@ int gen_tri(int nx, int ny, double x0, double x1, double y0, double y1, double j
             const int32_t bc[4], mst_mesh &m);
 int gen_tet(int nx, int ny, int nz, const double lo[3], const double hi[3], double jitter, uint64_t seed,
             const int32_t bc[6], mst_mesh &m);
+int gen_ramp(int nx, int ny, double x0, double x1, double height, double theta_deg, double jitter, uint64_t seed,
+             const int32_t bc[4], mst_mesh &m);
+// cell centres (mean of face centres, mesh/Cell.cpp:6-14) in file order
+int cell_centres(const mst_mesh &m, std::vector<double> &ccen);
 
 // std::mt19937_64 (its output sequence is fixed by the C++ standard) with a portable 53-bit
 // uniform in [0,1), so generated meshes are identical on every host

@@ -101,6 +101,33 @@ int gen_tri(int nx, int ny, double x0, double x1, double y0, double y1, double j
     return validate(m);
 }
 
+// Compression ramp: the jittered triangle lattice of gen_tri on the unit square, mapped so that the
+// bottom side is a wall with a corner at x = 0 (flat for x < 0, rising at theta for x > 0) and the top
+// side is straight at y = height.  The column through the corner is pinned to it; the map keeps
+// every cell's orientation, so faces still point into their c0 cell.
+int gen_ramp(int nx, int ny, double x0, double x1, double height, double theta_deg, double jitter, uint64_t seed,
+             const int32_t bc[4], mst_mesh &m) {
+    if (!(x0 < 0) || !(x1 > 0) || !(theta_deg >= 0) || !(theta_deg < 45))
+        return fail(MST_E_ARG, "gen_ramp: need x0 < 0 < x1 and 0 <= theta < 45 degrees");
+    const double tn = std::tan(theta_deg * M_PI / 180.0);
+    if (!(height > x1 * tn)) return fail(MST_E_ARG, "gen_ramp: the ramp reaches the top of the domain");
+    const int ic = (int)std::lround(nx * (-x0) / (x1 - x0));
+    if (ic < 1 || ic >= nx) return fail(MST_E_ARG, "gen_ramp: too few columns on one side of the corner");
+    int rc = gen_tri(nx, ny, 0.0, 1.0, 0.0, 1.0, jitter, seed, bc, m);
+    if (rc) return rc;
+    const double xc = (double)ic / nx;
+    m.xyz[(int64_t)ic * 2 + 0] = xc;  // node (ic, 0) is the corner
+    m.xyz[(int64_t)ic * 2 + 1] = 0.0;
+    for (int64_t n = 0; n < m.n_nodes; n++) {
+        const double xi = m.xyz[n * 2], eta = m.xyz[n * 2 + 1];
+        const double x = xi <= xc ? x0 * (1.0 - xi / xc) : x1 * (xi - xc) / (1.0 - xc);
+        const double yb = x > 0 ? x * tn : 0.0;
+        m.xyz[n * 2 + 0] = x;
+        m.xyz[n * 2 + 1] = yb + (height - yb) * eta;
+    }
+    return validate(m);
+}
+
 // ---------------------------------------------------------------------------------------------- 3D
 namespace {
 struct FaceRec {

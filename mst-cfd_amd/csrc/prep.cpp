@@ -271,6 +271,18 @@ int32_t kind_of(int32_t t) {
 
 }  // namespace
 
+int cell_centres(const mst_mesh &m, std::vector<double> &ccen) {
+    int rc = validate(m);
+    if (rc) return rc;
+    Prepared P;
+    std::vector<int64_t> cptr;
+    std::vector<int32_t> cface;
+    P.dim = m.dim;
+    geometry(m, P, cptr, cface);
+    ccen.swap(P.ccen);
+    return MST_OK;
+}
+
 int prepare(const mst_mesh &m, Prepared &P) {
     int rc = validate(m);
     if (rc) return rc;

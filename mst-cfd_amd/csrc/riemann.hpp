@@ -113,6 +113,7 @@ __device__ __forceinline__ void ausm_face(const T *A, const T *B, int mask, cons
         aB = fsqrt(P.gamma * pB * iB);
     }
     const bool unscaled = (P.compat & MST_COMPAT_AUSM_P_UNSCALED) != 0;
+    const bool onesided = (P.compat & MST_COMPAT_AUSM_ONESIDED) != 0;
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
         const bool left_is_A = (mask >> d) & 1;
@@ -123,7 +124,10 @@ __device__ __forceinline__ void ausm_face(const T *A, const T *B, int mask, cons
         const T mL = L[1 + d] * iL * iaF, mR = R[1 + d] * iR * iaF;
         T Mp, Mm, Pp, Pm;
         const T qL = mL * mL - one, qR = mR * mR - one;
-        if (mL <= one) {
+        // SolverAusm.cpp:116,122 test M <= 1 (compat); AUSM+ tests |M| <= 1
+        const bool subL = onesided ? mL <= one : fabs(mL) <= one;
+        const bool subR = onesided ? mR <= one : fabs(mR) <= one;
+        if (subL) {
             Mp = q4 * (mL + one) * (mL + one) + e8 * qL * qL;
             const T poly = q4 * (mL + one) * (mL + one) * (two - mL);
             const T corr = c316 * mL * qL * qL;
@@ -132,7 +136,7 @@ __device__ __forceinline__ void ausm_face(const T *A, const T *B, int mask, cons
             Mp = h * (mL + fabs(mL));
             Pp = fdiv(pL * h * (mL + fabs(mL)), mL);
         }
-        if (mR <= one) {
+        if (subR) {
             Mm = -q4 * (mR - one) * (mR - one) - e8 * qR * qR;
             const T poly = q4 * (mR - one) * (mR - one) * (two + mR);
             const T corr = c316 * mR * qR * qR;

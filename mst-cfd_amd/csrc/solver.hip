@@ -227,6 +227,7 @@ UpdArgs upd_args(const mst_ctx *x, int mode, int is, int in, double ca, double c
     a.maxf = x->maxf;
     a.l2 = (mode & UPD_L2) != 0;
     a.dtred = (mode & UPD_DT) != 0;
+    a.check = (mode & UPD_CHECK) != 0;
     a.gamma = x->cfg.gamma;
     a.cfl = x->cfg.cfl;
     a.Qout = iout >= 0 ? x->Q[iout] : nullptr;
@@ -242,7 +243,7 @@ template <int NV, bool GT>
 int run_update_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
     const UpdArgs a = upd_args(x, mode, is, in, ca, cb, stage0, iout);
 #define MST_UPD(MF)                                                                                             \
-    switch (mode) {                                                                                             \
+    switch (mode & ~UPD_CHECK) {                                                                                             \
         case UPD_RESIDUAL: return launch(x, MST_K_UPDATE, k_update<NV, MF, UPD_RESIDUAL, GT>, x->nc, a);        \
         case UPD_STAGE: return launch(x, MST_K_UPDATE, k_update<NV, MF, UPD_STAGE, GT>, x->nc, a);              \
         case UPD_STAGE | UPD_L2: return launch(x, MST_K_UPDATE, k_update<NV, MF, UPD_STAGE | UPD_L2, GT>, x->nc, a); \
@@ -451,7 +452,7 @@ int check_bad(mst_ctx *x) {
     HIPCHK(hipMemcpy(&h, x->bad, sizeof(int), hipMemcpyDeviceToHost));
     if (h) {
         HIPCHK(hipMemset(x->bad, 0, sizeof(int)));
-        return mst::fail(MST_E_STATE, "non-physical state (rho<=0, p<=0 or NaN) produced by the update");
+        return mst::fail(MST_E_STATE, "non-physical state (rho<=0, p<=0 or NaN, or no positive time step) produced by the update");
     }
     return MST_OK;
 }
@@ -607,14 +608,14 @@ int mst_face_flux(mst_ctx *ctx) {
 
 int mst_residual_update(mst_ctx *ctx) {
     CTX_GUARD(ctx);
-    int rc = run_update(ctx, UPD_STAGE, ctx->iold, ctx->iold, 0.0, 1.0, 1, ctx->inew);
+    int rc = run_update(ctx, UPD_STAGE | UPD_CHECK, ctx->iold, ctx->iold, 0.0, 1.0, 1, ctx->inew);
     if (rc) return rc;
     return check_bad(ctx);
 }
 
 int mst_solve(mst_ctx *ctx) {
     CTX_GUARD(ctx);
-    int rc = run_stage(ctx, UPD_STAGE, ctx->iold, ctx->iold, 0.0, 1.0, 1, ctx->inew);
+    int rc = run_stage(ctx, UPD_STAGE | UPD_CHECK, ctx->iold, ctx->iold, 0.0, 1.0, 1, ctx->inew);
     if (rc) return rc;
     return check_bad(ctx);
 }
@@ -662,7 +663,8 @@ int mst_step(mst_ctx *ctx, int nsteps) {
         int src = ctx->iold;
         for (int s = 0; s < ns; s++) {
             const int dst = (s == ns - 1) ? ctx->inew : spare[s & 1];
-            const int mode = UPD_STAGE | (s == 0 ? UPD_L2 : 0) | ((cfl && s == ns - 1) ? UPD_DT : 0);
+            const int mode = UPD_STAGE | (s == 0 ? UPD_L2 : 0) | ((cfl && s == ns - 1) ? UPD_DT : 0) |
+                             (s == ns - 1 ? UPD_CHECK : 0);
             rc = run_stage(ctx, mode, src, ctx->iold, A[s], B[s], s == 0 ? 1 : 0, dst);
             if (rc) return rc;
             src = dst;

@@ -20,10 +20,13 @@ ROE, AUSM = 0, 1
 LIM_NONE, LIM_MINMOD, LIM_VANLEER = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 COMPAT_BRANCH2_LR, COMPAT_GRAD_LASTINT, COMPAT_AUSM_LOCAL_A, COMPAT_AUSM_P_UNSCALED = 1, 2, 4, 8
-COMPAT_REFERENCE = COMPAT_GRAD_LASTINT | COMPAT_AUSM_LOCAL_A | COMPAT_AUSM_P_UNSCALED
+COMPAT_AUSM_ONESIDED = 16
+COMPAT_REFERENCE = COMPAT_GRAD_LASTINT | COMPAT_AUSM_LOCAL_A | COMPAT_AUSM_P_UNSCALED | COMPAT_AUSM_ONESIDED
 K_GRAD, K_FLUX, K_UPDATE, K_DT, K_CELL = 0, 1, 2, 3, 4
 KERNEL_NAMES = {K_GRAD: "k_grad", K_FLUX: "k_flux", K_UPDATE: "k_update", K_DT: "k_cfl", K_CELL: "k_cell"}
 STRATEGY_FACE, STRATEGY_CELL = 0, 1
+INIT_REFERENCE, INIT_UNIFORM, INIT_RIEMANN, INIT_VORTEX, INIT_PULSE, INIT_FREESTREAM = 0, 1, 2, 3, 4, 5
+BATCH_BUCKETS = 4
 
 # public exported symbols of include/mstcfd.h (checked by the CPU tests)
 EXPORTS = [
@@ -35,7 +38,10 @@ EXPORTS = [
     "mst_update_new_to_old", "mst_step", "mst_compute_dt", "mst_residual_norms", "mst_monitor_maxrel",
     "mst_get_residual", "mst_get_face_flux", "mst_get_gradient", "mst_riemann_flux", "mst_timing_enable",
     "mst_timing_get", "mst_timing_reset", "mst_sync", "mst_algorithmic_bytes", "mst_copy_probe",
-    "mst_set_strategy", "mst_get_strategy",
+    "mst_set_strategy", "mst_get_strategy", "mst_mesh_gen_ramp", "mst_mesh_init_state",
+    "mst_batch_create", "mst_batch_destroy", "mst_batch_counts", "mst_batch_case_counts", "mst_batch_set_state",
+    "mst_batch_get_state", "mst_batch_step", "mst_batch_residual_norms", "mst_batch_dt", "mst_batch_set_active",
+    "mst_batch_set_tuning", "mst_batch_get_tuning", "mst_batch_set_graph", "mst_batch_autotune",
 ]
 
 
@@ -138,6 +144,23 @@ def lib():
         "mst_sync": (C.c_int, [vp]),
         "mst_algorithmic_bytes": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double)]),
         "mst_copy_probe": (C.c_int, [C.c_int64, C.POINTER(C.c_double)]),
+        "mst_mesh_gen_ramp": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_uint64, I32P, pp]),
+        "mst_mesh_init_state": (C.c_int, [vp, C.c_int, F64P, C.c_int, F64P, C.c_int64]),
+        "mst_batch_create": (C.c_int, [C.c_int, pp, C.POINTER(Config), pp]),
+        "mst_batch_destroy": (C.c_int, [vp]),
+        "mst_batch_counts": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "mst_batch_case_counts": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "mst_batch_set_state": (C.c_int, [vp, C.c_int, F64P, C.c_int64]),
+        "mst_batch_get_state": (C.c_int, [vp, C.c_int, F64P, C.c_int64]),
+        "mst_batch_step": (C.c_int, [vp, C.c_int]),
+        "mst_batch_residual_norms": (C.c_int, [vp, F64P]),
+        "mst_batch_dt": (C.c_int, [vp, F64P]),
+        "mst_batch_set_active": (C.c_int, [vp, np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")]),
+        "mst_batch_set_tuning": (C.c_int, [vp, C.c_int, I32P]),
+        "mst_batch_get_tuning": (C.c_int, [vp, C.POINTER(C.c_int), I32P]),
+        "mst_batch_set_graph": (C.c_int, [vp, C.c_int]),
+        "mst_batch_autotune": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -188,6 +211,21 @@ class Mesh:
     def gen_tet(cls, nx, ny, nz, lo=(0, 0, 0), hi=(1, 1, 1), jitter=0.0, seed=12345, bc=(BC_WALL,) * 6):
         return cls._new(lib().mst_mesh_gen_tet, int(nx), int(ny), int(nz), np.asarray(lo, np.float64),
                         np.asarray(hi, np.float64), float(jitter), int(seed), np.asarray(bc, np.int32))
+
+    @classmethod
+    def gen_ramp(cls, nx, ny, theta_deg, x0=-0.5, x1=1.5, height=1.2, jitter=0.0, seed=2024,
+                 bc=(BC_INLET, BC_OUTLET, BC_WALL, BC_OUTLET)):
+        """Compression ramp: wall corner at x = 0, bc = (left, right, bottom, top)."""
+        return cls._new(lib().mst_mesh_gen_ramp, int(nx), int(ny), float(x0), float(x1), float(height),
+                        float(theta_deg), float(jitter), int(seed), np.asarray(bc, np.int32))
+
+    def init_state(self, kind, params):
+        """Initial field on the cell centres, file order (mst_mesh_init_state)."""
+        p = np.ascontiguousarray(params, np.float64).reshape(-1)
+        Q = np.zeros((self.n_cells, self.nv))
+        _check(lib().mst_mesh_init_state(self.h, int(kind), p if p.size else np.zeros(1), int(p.size),
+                                         Q.reshape(-1), self.n_cells), "mst_mesh_init_state")
+        return Q
 
     @classmethod
     def read_msh(cls, path):
@@ -401,3 +439,79 @@ def riemann_flux(cfg, QL, QR, dirs, dim=2):
     _check(lib().mst_riemann_flux(C.byref(cfg), int(dim), QL.shape[0], QL.reshape(-1), QR.reshape(-1), d,
                                   F.reshape(-1)), "mst_riemann_flux")
     return F
+
+
+class Batch:
+    """Many independent cases advanced together (persistent kernels, graph-replayed RK steps)."""
+
+    def __init__(self, meshes, cfgs):
+        self.meshes = list(meshes)
+        n = len(self.meshes)
+        arr = (C.c_void_p * n)(*[m.h for m in self.meshes])
+        cf = (Config * n)(*cfgs)
+        h = C.c_void_p()
+        _check(lib().mst_batch_create(n, C.cast(arr, C.POINTER(C.c_void_p)), cf, C.byref(h)), "mst_batch_create")
+        self.h = h
+        c = (C.c_int64 * 6)()
+        _check(lib().mst_batch_counts(self.h, c), "mst_batch_counts")
+        self.n_cases, self.n_cells, self.n_faces, self.n_cell_items, self.n_face_items, self.dim = [int(v) for v in c]
+        self.nv = self.dim + 2
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mst_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def counts(self):
+        c = (C.c_int64 * 6)()
+        _check(lib().mst_batch_counts(self.h, c), "mst_batch_counts")
+        return [int(v) for v in c]
+
+    def set_state(self, i, Q):
+        Q = np.ascontiguousarray(Q, np.float64)
+        _check(lib().mst_batch_set_state(self.h, int(i), Q.reshape(-1), Q.shape[0]), "mst_batch_set_state")
+
+    def get_state(self, i):
+        m = self.meshes[i]
+        Q = np.zeros((m.n_cells, self.nv))
+        _check(lib().mst_batch_get_state(self.h, int(i), Q.reshape(-1), m.n_cells), "mst_batch_get_state")
+        return Q
+
+    def step(self, n=1):
+        _check(lib().mst_batch_step(self.h, int(n)), "mst_batch_step")
+
+    def residual_norms(self):
+        out = np.zeros((self.n_cases, self.nv))
+        _check(lib().mst_batch_residual_norms(self.h, out.reshape(-1)), "mst_batch_residual_norms")
+        return out
+
+    def dt(self):
+        out = np.zeros(self.n_cases)
+        _check(lib().mst_batch_dt(self.h, out), "mst_batch_dt")
+        return out
+
+    def set_active(self, active):
+        a = np.ascontiguousarray(np.asarray(active, bool).astype(np.uint8))
+        _check(lib().mst_batch_set_active(self.h, a), "mst_batch_set_active")
+
+    def set_tuning(self, block, tiles):
+        _check(lib().mst_batch_set_tuning(self.h, int(block), np.asarray(tiles, np.int32)), "mst_batch_set_tuning")
+
+    def get_tuning(self):
+        b = C.c_int()
+        t = np.zeros(BATCH_BUCKETS, np.int32)
+        _check(lib().mst_batch_get_tuning(self.h, C.byref(b), t), "mst_batch_get_tuning")
+        return b.value, t.tolist()
+
+    def set_graph(self, on):
+        _check(lib().mst_batch_set_graph(self.h, 1 if on else 0), "mst_batch_set_graph")
+
+    def autotune(self, cache_path=None, force=False, steps=5):
+        ms = C.c_double()
+        path = str(cache_path).encode() if cache_path else None
+        _check(lib().mst_batch_autotune(self.h, path, 1 if force else 0, int(steps), C.byref(ms)),
+               "mst_batch_autotune")
+        return ms.value

@@ -240,23 +240,27 @@ void orc_ausm_flux(const double *QL, const double *QR, int d, int dim, const orc
         aR = aF;
     }
     double Mp, Mm, Pp, Pm;
-    /* the reference writes abs(machL <= 1), i.e. the one-sided test machL <= 1 (:116,122,129,135) */
-    if (mL <= 1)
+    /* the reference writes abs(machL <= 1), i.e. the one-sided test machL <= 1 (:116,122,129,135);
+     * without the compat bit the AUSM+ test |M| <= 1 */
+    const int onesided = (cfg->compat & ORC_COMPAT_AUSM_ONESIDED) != 0;
+    const int subL = onesided ? mL <= 1 : fabs(mL) <= 1;
+    const int subR = onesided ? mR <= 1 : fabs(mR) <= 1;
+    if (subL)
         Mp = 0.25 * (mL + 1) * (mL + 1) + 0.125 * (mL * mL - 1) * (mL * mL - 1); /* :117 */
     else
         Mp = 0.5 * (mL + fabs(mL)); /* :120 */
-    if (mR <= 1)
+    if (subR)
         Mm = -0.25 * (mR - 1) * (mR - 1) - 0.125 * (mR * mR - 1) * (mR * mR - 1); /* :123 */
     else
         Mm = 0.5 * (mR - fabs(mR)); /* :126 */
-    if (mL <= 1) {
+    if (subL) {
         if (cfg->compat & ORC_COMPAT_AUSM_P_UNSCALED) /* :130 literal precedence */
             Pp = pL * 0.25 * (mL + 1) * (mL + 1) * (2 - mL) + 0.1875 * mL * (mL * mL - 1) * (mL * mL - 1);
         else
             Pp = pL * (0.25 * (mL + 1) * (mL + 1) * (2 - mL) + 0.1875 * mL * (mL * mL - 1) * (mL * mL - 1));
     } else
         Pp = pL * 0.5 * (mL + fabs(mL)) / mL; /* :133 */
-    if (mR <= 1) {
+    if (subR) {
         if (cfg->compat & ORC_COMPAT_AUSM_P_UNSCALED) /* :136 */
             Pm = pR * 0.25 * (mR - 1) * (mR - 1) * (2 + mR) - 0.1875 * mR * (mR * mR - 1) * (mR * mR - 1);
         else

@@ -36,6 +36,8 @@ enum { ORC_LIM_NONE = 0, ORC_LIM_MINMOD = 1, ORC_LIM_VANLEER = 2 };
 #define ORC_COMPAT_GRAD_LASTINT    2  /* RhoSolver.cpp:437-440 boundary loop starting at N_int-1 */
 #define ORC_COMPAT_AUSM_LOCAL_A    4  /* SolverAusm.cpp:112-113: flux scaled by local a_L, a_R */
 #define ORC_COMPAT_AUSM_P_UNSCALED 8  /* SolverAusm.cpp:130,140: 3/16 term of P+- not multiplied by p */
+#define ORC_COMPAT_AUSM_ONESIDED   16 /* SolverAusm.cpp:116,122,129,135: subsonic branch taken for M <= 1
+                                         (the code's abs(M <= 1)) instead of |M| <= 1 */
 
 /* Same field layout as mst_config in include/mstcfd.h. */
 typedef struct {

@@ -19,7 +19,8 @@ INTERIOR, WALL, OUTLET, SYMMETRY, INLET, PERIODIC = 2, 3, 5, 7, 10, 12
 ROE, AUSM = 0, 1
 LIM_NONE, LIM_MINMOD, LIM_VANLEER = 0, 1, 2
 COMPAT_BRANCH2_LR, COMPAT_GRAD_LASTINT, COMPAT_AUSM_LOCAL_A, COMPAT_AUSM_P_UNSCALED = 1, 2, 4, 8
-COMPAT_REFERENCE = COMPAT_GRAD_LASTINT | COMPAT_AUSM_LOCAL_A | COMPAT_AUSM_P_UNSCALED
+COMPAT_AUSM_ONESIDED = 16
+COMPAT_REFERENCE = COMPAT_GRAD_LASTINT | COMPAT_AUSM_LOCAL_A | COMPAT_AUSM_P_UNSCALED | COMPAT_AUSM_ONESIDED
 
 
 class Config(C.Structure):

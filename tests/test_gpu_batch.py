@@ -1,0 +1,177 @@
+"""Batched cases (persistent kernels over (case, tile) work items, graph-replayed RK steps).
+
+Every cell and face of a case is computed by the same device code as the single-case context and
+the per-case time step is a minimum, so a case advanced inside a batch must equal, bit for bit, the
+same case advanced alone -- whatever the tiles, block size, graph replay or the other cases."""
+import json
+
+import numpy as np
+import pytest
+
+import mstcfd as M
+
+pytestmark = pytest.mark.gpu
+
+
+def riemann_cases(n, seed=777, dim=2, cells=(200, 3000)):
+    """Sod-like shock tubes on [0,1] x [0,0.01] strips (2 rows of quads), uneven sizes, Roe/AUSM+ mixed."""
+    rng = np.random.default_rng(seed)
+    meshes, cfgs, Q0 = [], [], []
+    for i in range(n):
+        nc = int(rng.uniform(*cells))
+        nx = max(4, nc // 4)
+        m = M.Mesh.gen_tri(nx, 2, 0, 1, 0, 0.01, 0.1, 1000 + i)
+        roe = rng.uniform() < 0.5
+        # AUSM+ loses positivity on these strips above a pressure ratio of ~25 (see bench.batch_cases)
+        pr = 10 ** rng.uniform(np.log10(2), 2 if roe else np.log10(20))
+        dr = rng.uniform(1, 10)
+        cfgs.append(dict(scheme=M.ROE if roe else M.AUSM, order=2, limiter=M.LIM_MINMOD, rk=2, cfl=0.5,
+                         compat=M.COMPAT_REFERENCE if roe else M.COMPAT_GRAD_LASTINT))
+        meshes.append(m)
+        Q0.append(m.init_state(M.INIT_RIEMANN, [1.4, 0, 0.5, 0.125 * dr, 0, 0, 0.1 * pr, 0.125, 0, 0, 0.1]))
+    return meshes, cfgs, Q0
+
+
+def make_batch(meshes, cfgs, Q0, **extra):
+    b = M.Batch(meshes, [M.make_config(**dict(c, **extra)) for c in cfgs])
+    for i, q in enumerate(Q0):
+        b.set_state(i, q)
+    return b
+
+
+def single(m, cfg, q, steps, **extra):
+    s = M.Solver(m, M.make_config(**dict(cfg, **extra)))
+    s.set_state(q)
+    s.step(steps)
+    return s.get_state(), s.residual_norms()
+
+
+@pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_MIXED])
+def test_batch_equals_single_case_contexts(precision):
+    meshes, cfgs, Q0 = riemann_cases(10)
+    b = make_batch(meshes, cfgs, Q0, precision=precision)
+    n0 = b.counts()
+    print(f"batch: {n0[0]} cases, {n0[1]} cells, {n0[3]} cell items, {n0[4]} face items", flush=True)
+    dt0 = b.dt()
+    b.step(25)
+    res = b.residual_norms()
+    for i in range(len(meshes)):
+        s = M.Solver(meshes[i], M.make_config(precision=precision, **cfgs[i]))
+        s.set_state(Q0[i])
+        assert s.compute_dt(0.5) == dt0[i]
+        s.step(25)
+        assert np.array_equal(b.get_state(i), s.get_state()), f"case {i}"
+        assert np.allclose(res[i], s.residual_norms(), rtol=1e-13, atol=0)
+
+
+def test_batch_tiling_block_graph_independence():
+    meshes, cfgs, Q0 = riemann_cases(9, seed=5, cells=(100, 6000))
+    out = []
+    for block, tiles, graph in [(256, [1024, 2048, 4096, 8192], True), (64, [64, 128, 256, 512], True),
+                                (128, [256, 256, 512, 16384], False)]:
+        b = make_batch(meshes, cfgs, Q0)
+        b.set_tuning(block, tiles)
+        b.set_graph(graph)
+        assert b.get_tuning() == (block, tiles)
+        b.step(20)
+        out.append([b.get_state(i) for i in range(len(meshes))])
+        b.close()
+    for k in (1, 2):
+        for i in range(len(meshes)):
+            assert np.array_equal(out[k][i], out[0][i])
+
+
+def test_batch_freeze_and_reactivate():
+    meshes, cfgs, Q0 = riemann_cases(6, seed=9)
+    b = make_batch(meshes, cfgs, Q0)
+    b.step(5)
+    frozen = [b.get_state(i) for i in range(6)]
+    active = np.array([1, 0, 1, 0, 1, 1], bool)
+    b.set_active(active)
+    b.step(7)  # an odd count: the buffers rotate under the frozen cases
+    for i in range(6):
+        if not active[i]:
+            assert np.array_equal(b.get_state(i), frozen[i])
+    b.set_active(np.ones(6, bool))
+    b.step(3)
+    for i in range(6):
+        steps = 5 + (10 if active[i] else 3)
+        q, _ = single(meshes[i], cfgs[i], Q0[i], steps)
+        assert np.array_equal(b.get_state(i), q), f"case {i}"
+
+
+def test_batch_autotune_cache_and_state(tmp_path):
+    meshes, cfgs, Q0 = riemann_cases(8, seed=11, cells=(500, 40000))
+    b = make_batch(meshes, cfgs, Q0)
+    before = [b.get_state(i) for i in range(8)]
+    path = tmp_path / "tune.json"
+    ms = b.autotune(path, steps=3)
+    block, tiles = b.get_tuning()
+    print(f"autotune: block {block}, tiles {tiles}, {ms:.3f} ms/step", flush=True)
+    assert ms > 0 and block in (64, 128, 256)
+    for i in range(8):  # the search advanced and then restored the state
+        assert np.array_equal(b.get_state(i), before[i])
+    entries = json.loads(path.read_text())
+    assert len(entries) == 1
+    (e,) = entries.values()
+    assert e["block"] == block and e["tiles"] == tiles
+    # a second batch of the same composition takes the cached shape without measuring
+    b2 = make_batch(meshes, cfgs, Q0)
+    assert b2.autotune(path, steps=3) == pytest.approx(e["ms"])
+    assert b2.get_tuning() == (block, tiles)
+    b.step(10)
+    b2.step(10)
+    for i in range(8):
+        assert np.array_equal(b.get_state(i), b2.get_state(i))
+
+
+def test_batch_3d_cases():
+    bc = (M.BC_INLET, M.BC_OUTLET) + (M.BC_SYMMETRY,) * 4
+    meshes = [M.Mesh.gen_tet(n, 4, 4, jitter=0.1, seed=n, bc=bc) for n in (4, 9, 6)]
+    qin = [1.0, 0.8, 0.0, 0.0, 1 / 0.4 / 1.4 + 0.32]
+    cfgs = [dict(scheme=s, order=2, limiter=M.LIM_VANLEER, rk=3, cfl=0.4, q_inlet=qin, dim=3)
+            for s in (M.AUSM, M.ROE, M.AUSM)]
+    Q0 = [m.init_state(M.INIT_PULSE, [1.4, 0.1, 0.15, 0.5, 0.5, 0.5, 0.8]) for m in meshes]
+    b = make_batch(meshes, cfgs, Q0)
+    b.step(12)
+    for i in range(3):
+        q, _ = single(meshes[i], cfgs[i], Q0[i], 12)
+        assert np.array_equal(b.get_state(i), q)
+
+
+def test_batch_rejects_mixed_settings():
+    m = M.Mesh.gen_tri(8, 2, 0, 1, 0, 0.01)
+    with pytest.raises(M.MstError):
+        M.Batch([m, m], [M.make_config(order=1, cfl=0.5), M.make_config(order=2, cfl=0.5)])
+    with pytest.raises(M.MstError):
+        M.Batch([m], [M.make_config(order=1, cfl=0.0)])
+    p = M.Mesh.gen_tri(8, 8, 0, 1, 0, 1, bc=(M.BC_PERIODIC,) * 4)
+    with pytest.raises(M.MstError):
+        M.Batch([p], [M.make_config(cfl=0.5)])
+
+
+def test_batch_multi_item_case_with_stage_dip_matches_oracle():
+    """A 47k-cell strong shock tube (split over several work items; pressure ratio 93, Roe): the first
+    stage of its third step leaves the admissible set near the contact and the SSP combination
+    returns to it.  No error is reported, and the case equals the single context bit for bit and the
+    oracle to rounding."""
+    from oracle import pyoracle as O
+    m = M.Mesh.gen_tri(11747, 2, 0.0, 1.0, 0.0, 0.01, 0.1, 777075)
+    cfg = dict(scheme=M.ROE, order=2, limiter=M.LIM_MINMOD, rk=2, cfl=0.5)
+    q = m.init_state(M.INIT_RIEMANN, [1.4, 0, 0.5, 1.2020672546473616, 0, 0, 0.1 * 93.05071251044939,
+                                      0.125, 0, 0, 0.1])
+    small, scfg, sq = riemann_cases(3, seed=21)
+    b = make_batch([m] + small, [cfg] + scfg, [q] + sq)
+    b.set_tuning(256, [1024, 2048, 4096, 8192])
+    assert b.counts()[3] >= 6 + 3
+    b.step(6)
+    qs, _ = single(m, cfg, q, 6)
+    assert np.array_equal(b.get_state(0), qs)
+    om = O.OracleMesh(m.raw())
+    co = O.make_config(**cfg)
+    Qr = q.copy()
+    for _ in range(6):
+        Qr, _ = om.rk_step(co, Qr, om.cfl_dt(co, Qr))
+    err = np.abs(qs - Qr).max() / np.abs(Qr).max()
+    print(f"47k-cell tube: batch == single, vs oracle {err:.2e}", flush=True)
+    assert err <= 1e-12

@@ -159,19 +159,29 @@ def test_ausm_consistency_and_quirk():
 
 def test_ausm_supersonic_and_one_sided_branch():
     cfg = O.make_config(scheme=O.AUSM, compat=0)
+    one = O.make_config(scheme=O.AUSM, compat=O.COMPAT_AUSM_ONESIDED)
+    # supersonic to the right (M > 1 on both sides): the flux is the left state's Euler flux
     QL = cons([1.0], [[3.0, 0.2]], [0.5])[0]
     QR = cons([0.8], [[2.9, 0.1]], [0.45])[0]
-    F = O.ausm_flux(QL, QR, 0, cfg)
-    assert np.abs(F - phys_flux(QL, 0, 2)).max() <= 1e-13 * np.abs(F).max()
-    # subsonic pairs are mirror symmetric; the one-sided "M <= 1" test breaks it for M_L < -1
+    for c in (cfg, one):
+        F = O.ausm_flux(QL, QR, 0, c)
+        assert np.abs(F - phys_flux(QL, 0, 2)).max() <= 1e-13 * np.abs(F).max()
+    # subsonic pairs are mirror symmetric
     QL, QR = cons([1.0], [[0.3, 0.1]], [1.0])[0], cons([0.9], [[-0.2, 0.0]], [0.8])[0]
     F = O.ausm_flux(QL, QR, 0, cfg)
     Fm = O.ausm_flux(mirror(QR, 0), mirror(QL, 0), 0, cfg)
     assert np.abs(Fm + mirror(F, 0)).max() <= 1e-13 * np.abs(F).max()
+    # supersonic to the left (M < -1 on both sides): AUSM+ (|M| <= 1 test) gives the right state's
+    # Euler flux and stays mirror symmetric; the reference's one-sided "M <= 1" test does neither
     QL, QR = cons([1.0], [[-3.0, 0.0]], [0.5])[0], cons([1.0], [[-2.4, 0.0]], [0.5])[0]
     F = O.ausm_flux(QL, QR, 0, cfg)
+    assert np.abs(F - phys_flux(QR, 0, 2)).max() <= 1e-13 * np.abs(F).max()
     Fm = O.ausm_flux(mirror(QR, 0), mirror(QL, 0), 0, cfg)
-    assert np.abs(Fm + mirror(F, 0)).max() > 1e-3 * np.abs(F).max()
+    assert np.abs(Fm + mirror(F, 0)).max() <= 1e-13 * np.abs(F).max()
+    F1 = O.ausm_flux(QL, QR, 0, one)
+    Fm1 = O.ausm_flux(mirror(QR, 0), mirror(QL, 0), 0, one)
+    assert np.abs(Fm1 + mirror(F1, 0)).max() > 1e-3 * np.abs(F1).max()
+    assert np.abs(F1 - phys_flux(QR, 0, 2)).max() > 1e-3 * np.abs(F1).max()
 
 
 # ------------------------------------------------------------------------ gradient / limiter
