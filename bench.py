@@ -42,6 +42,25 @@ def build_workload(M, name):
     raise SystemExit(f"unknown workload {name}")
 
 
+# HBM bytes per launch of a workload's dominant kernel from the committed rocprofv3 counter passes
+# (FETCH_SIZE and WRITE_SIZE scaled by a plain copy kernel of known size); None where not profiled
+TRAFFIC_PROFILES = {("vortex2d", "face", "k_grad"): ("profiles/hbm_traffic_vortex2d.md", "k_grad<2, 3, 2, double>")}
+
+
+def committed_traffic(workload, strategy, kernel):
+    entry = TRAFFIC_PROFILES.get((workload, strategy, kernel))
+    if not entry:
+        return None, None
+    path, name = entry
+    try:
+        for line in open(os.path.join(ROOT, path)):
+            if line.startswith(f"traffic_bytes_per_launch {name}:"):
+                return float(line.split(":")[1]), path
+    except OSError:
+        pass
+    return None, None
+
+
 def measure(M, solver, steps, warmup):
     """Run warmup steps, then time exactly `steps` steps (stream-synchronised on both sides) with the
     per-kernel HIP events recorded on the solver's stream."""
@@ -207,13 +226,15 @@ def main():
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     d = kern[dom]
     achieved = d["alg_bytes"] / (d["avg_us"] * 1e-6) / 1e9
+    traffic, traffic_src = committed_traffic(args.workload, args.strategy, dom)
     out = dict(
         metric="face-flux updates/s (FP64 residual path); %HBM peak of the dominant kernel",
         value=value, unit="face-updates/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
         ms_per_step=1e3 * wall / args.steps, higher_is_better=True, scaling="weak", vs_baseline=None,
         dtype="f64", data="synthetic", config=desc,
         roofline=dict(bound="hbm", kernel=dom, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=achieved / HBM_PEAK_GBS, traffic=None),
+                      frac=achieved / HBM_PEAK_GBS, traffic=traffic, traffic_source=traffic_src,
+                      alg_bytes=d["alg_bytes"]),
         kernels=kern,
     )
     if copy:

@@ -23,6 +23,15 @@
 #include "prep.hpp"
 #include "riemann.hpp"
 
+// Minimum resident waves per SIMD requested from the compiler for the gradient and flux kernels (a
+// register budget: 8 waves = 64 VGPRs, 6 = 80, 5 = 96); 1 leaves the choice to the compiler.
+#ifndef MST_GRAD_WAVES
+#define MST_GRAD_WAVES 1
+#endif
+#ifndef MST_FLUX_WAVES
+#define MST_FLUX_WAVES 1
+#endif
+
 namespace mstk {
 
 // ------------------------------------------------------------------------------------ face flux
@@ -123,7 +132,7 @@ __device__ __forceinline__ void face_flux(const FluxArgs<T> &a, int64_t f, int k
 
 // face strategy: one lane per face writes G_f (RhoSolver::updateFaceFlux[2ndOrder])
 template <int DIM, int SCHEME, bool SECOND, typename T>
-__global__ __launch_bounds__(BLOCK) void k_flux(const FluxArgs<T> a) {
+__global__ __launch_bounds__(BLOCK, MST_FLUX_WAVES) void k_flux(const FluxArgs<T> a) {
     constexpr int NV = DIM + 2;
     const int64_t f = block_id() * BLOCK + threadIdx.x;
     if (f >= a.nf) return;
@@ -296,7 +305,7 @@ __device__ __forceinline__ void grad_cell(const GradArgs<T> &a, const int64_t c)
 }
 
 template <int DIM, int MAXF, int LIM, typename T>
-__global__ __launch_bounds__(BLOCK) void k_grad(const GradArgs<T> a) {
+__global__ __launch_bounds__(BLOCK, MST_GRAD_WAVES) void k_grad(const GradArgs<T> a) {
     const int64_t c = block_id() * BLOCK + threadIdx.x;
     if (c < a.nc) grad_cell<DIM, MAXF, LIM, T>(a, c);
 }
@@ -568,18 +577,22 @@ __global__ __launch_bounds__(BLOCK) void k_maxrel(int64_t nc, const double *__re
     if (threadIdx.x < NV) part[threadIdx.x * (int64_t)gridDim.x + blk] = s[threadIdx.x][0];
 }
 
-// One-block finish of the per-block partials.  Slot k is a sum (kind 0), a min (kind 1) or a max
-// (kind 2); every thread walks the blocks with a fixed stride, then a fixed butterfly inside each
-// wave and a fixed pass over the waves, so the result is bit-reproducible.
+// Finish of the per-block partials.  Slot k is a sum (kind 0), a min (kind 1) or a max (kind 2).
+// Block b reduces the partials [b*chunk, (b+1)*chunk): every thread walks its share with a fixed
+// stride, then a fixed butterfly inside each wave and a fixed pass over the waves, so the result is
+// bit-reproducible.  With mid set (first level of a two-level finish) block b writes its raw result
+// to mid[k*gridDim.x + b]; otherwise (one block) it writes dst[k] (post: sqrt(v / count)).
 constexpr int FIN = 1024;
+constexpr int FIN_CHUNK = 1024;  // partials per block of the first level
 struct FinArgs {
     const double *src[12];
     int kind[12];
     double *dst[12];
     int post[12];  // 1: sqrt(v / count)
     int n;
-    int64_t nblk;
+    int64_t nblk, chunk;
     double count;
+    double *mid;
 };
 
 __device__ __forceinline__ double fin_op(int kind, double a, double b) {
@@ -592,7 +605,9 @@ static __global__ __launch_bounds__(FIN) void k_finish(const FinArgs a) {
     double acc[MAXW];
 #pragma unroll
     for (int k = 0; k < MAXW; k++) acc[k] = (k < a.n && a.kind[k] == 1) ? INFINITY : 0.0;
-    for (int64_t i = threadIdx.x; i < a.nblk; i += FIN) {
+    const int64_t i0 = (int64_t)blockIdx.x * a.chunk;
+    const int64_t i1 = i0 + a.chunk < a.nblk ? i0 + a.chunk : a.nblk;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += FIN) {
 #pragma unroll
         for (int k = 0; k < MAXW; k++)
             if (k < a.n) acc[k] = fin_op(a.kind[k], acc[k], a.src[k][i]);
@@ -609,7 +624,10 @@ static __global__ __launch_bounds__(FIN) void k_finish(const FinArgs a) {
         const int k = threadIdx.x;
         double v = s[k][0];
         for (int w = 1; w < FIN / 64; w++) v = fin_op(a.kind[k], v, s[k][w]);
-        a.dst[k][0] = a.post[k] ? sqrt(v / a.count) : v;
+        if (a.mid)
+            a.mid[(int64_t)k * gridDim.x + blockIdx.x] = v;
+        else
+            a.dst[k][0] = a.post[k] ? sqrt(v / a.count) : v;
     }
 }
 

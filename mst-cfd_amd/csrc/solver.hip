@@ -67,6 +67,7 @@ struct mst_ctx {
     float *fn32 = nullptr, *feta32 = nullptr, *fd0 = nullptr, *fd1 = nullptr, *lg32 = nullptr;
     double *scal = nullptr;  // [0] dt, [1..5] l2 norms, [6..10] maxrel
     double *part = nullptr;  // per-block partials
+    double *fin_mid = nullptr;  // first-level results of the two-level finish
     bool dt_valid = false;   // scal[0] holds the CFL time step of the current Old state
     int *bad = nullptr;
     bool timing = false;
@@ -305,7 +306,28 @@ int run_stage(mst_ctx *x, int mode, int is, int in, double ca, double cb, int st
     return rc;
 }
 
-// one finish launch: L2 norms (nl2 fields) and/or the minimum time step, from the per-block partials
+// finish of f's partials: one block when they are few, else a first level of FIN_CHUNK-partial
+// blocks into fin_mid and one block over those (fixed partition: bit-reproducible)
+int launch_finish(mst_ctx *x, FinArgs f) {
+    f.chunk = f.nblk;
+    f.mid = nullptr;
+    if (f.nblk > 2 * FIN_CHUNK) {
+        const int64_t g = (f.nblk + FIN_CHUNK - 1) / FIN_CHUNK;
+        FinArgs a = f;
+        a.chunk = FIN_CHUNK;
+        a.mid = x->fin_mid;
+        hipLaunchKernelGGL(k_finish, dim3((uint32_t)g), dim3(FIN), 0, x->stream, a);
+        HIPCHK(hipGetLastError());
+        for (int k = 0; k < f.n; k++) f.src[k] = x->fin_mid + k * g;
+        f.nblk = g;
+        f.chunk = g;
+    }
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(FIN), 0, x->stream, f);
+    HIPCHK(hipGetLastError());
+    return MST_OK;
+}
+
+// one finish: L2 norms (nl2 fields) and/or the minimum time step, from the per-block partials
 int run_finish(mst_ctx *x, int nl2, int want_dt) {
     FinArgs f;
     memset(&f, 0, sizeof(f));
@@ -326,9 +348,7 @@ int run_finish(mst_ctx *x, int nl2, int want_dt) {
         f.post[f.n] = 0;
         f.n++;
     }
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(FIN), 0, x->stream, f);
-    HIPCHK(hipGetLastError());
-    return MST_OK;
+    return launch_finish(x, f);
 }
 
 int run_cfl(mst_ctx *x, const double *Qsrc, double cfl) {
@@ -413,7 +433,7 @@ void release(mst_ctx *x) {
     if (!x) return;
     void *bufs[] = {x->fc0, x->fc1, x->finfo, x->ell, x->fn, x->fxf, x->pxf, x->feta, x->cvol, x->cxc,
                     x->cshalf, x->Q[0], x->Q[1], x->Q[2], x->Q[3], x->G, x->lg, x->graw, x->phi, x->R,
-                    x->scal, x->part, x->bad, x->Q32[0], x->Q32[1], x->Q32[2], x->Q32[3], x->fn32, x->feta32,
+                    x->scal, x->part, x->fin_mid, x->bad, x->Q32[0], x->Q32[1], x->Q32[2], x->Q32[3], x->fn32, x->feta32,
                     x->fd0, x->fd1, x->lg32};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -537,6 +557,7 @@ int mst_ctx_create(const mst_mesh *m, const mst_config *cfg, mst_ctx **out) {
     TRY(dalloc(&x->lg, x->nc * x->nv * x->dim));
     TRY(dalloc(&x->scal, 16));
     TRY(dalloc(&x->part, std::max(x->nblk_c, x->nblk_f) * (x->nv + 1) + 16));
+    TRY(dalloc(&x->fin_mid, 12 * (x->nblk_c / FIN_CHUNK + 2)));
     TRY(dalloc(&x->bad, 1));
     if (hipMemset(x->bad, 0, sizeof(int)) != hipSuccess || hipMemset(x->scal, 0, 16 * sizeof(double)) != hipSuccess ||
         hipMemset(x->lg, 0, sizeof(double) * x->nc * x->nv * x->dim) != hipSuccess) {
@@ -715,8 +736,7 @@ int mst_monitor_maxrel(mst_ctx *ctx, double *out) {
         f.dst[k] = ctx->scal + 6 + k;
     }
     f.n = ctx->nv;
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(FIN), 0, ctx->stream, f);
-    HIPCHK(hipGetLastError());
+    rc = launch_finish(ctx, f);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(out, ctx->scal + 6, sizeof(double) * ctx->nv, hipMemcpyDeviceToHost));

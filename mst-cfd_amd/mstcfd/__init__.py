@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libmstcfd.so")
+# MSTCFD_LIB selects another build of the library (e.g. one with different launch bounds)
+LIB_PATH = os.environ.get("MSTCFD_LIB") or os.path.join(os.path.dirname(_HERE), "lib", "libmstcfd.so")
 
 I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 F64P = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
