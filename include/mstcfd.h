@@ -244,6 +244,37 @@ int mst_batch_set_graph(mst_batch *b, int on);
  * is applied without measuring unless force.  The state is restored afterwards. */
 int mst_batch_autotune(mst_batch *b, const char *cache_path, int force, int steps, double *ms_per_step);
 
+/* ---------------------------------------------------------------- learned warm start (NEW, no reference
+ * counterpart).  A fully connected network n_in -> width (tanh) x hidden_layers -> n_out (linear),
+ * FP32, trained and evaluated by the library's own kernels.  Parameters: per layer W [out x in]
+ * row-major, then b [out].  Features are normalised (f - in_shift) * in_scale, outputs de-normalised
+ * y * out_scale + out_shift; the loss is the mean squared error of the de-normalised outputs. */
+typedef struct mst_mlp mst_mlp;
+int mst_mlp_create(int n_in, int width, int hidden_layers, int n_out, uint64_t seed, mst_mlp **out);
+int mst_mlp_destroy(mst_mlp *m);
+int mst_mlp_num_params(const mst_mlp *m, int64_t *n);
+int mst_mlp_get_params(mst_mlp *m, float *p, int64_t n);
+int mst_mlp_set_params(mst_mlp *m, const float *p, int64_t n);
+int mst_mlp_set_scaling(mst_mlp *m, const float *in_shift, const float *in_scale, const float *out_shift,
+                        const float *out_scale);
+/* training set, host arrays X [n x n_in], T [n x n_out] (de-normalised targets), copied to the device */
+int mst_mlp_set_data(mst_mlp *m, const float *X, const float *T, int64_t n);
+/* zero the Adam moments and the step counter */
+int mst_mlp_reset_optimizer(mst_mlp *m);
+/* steps of Adam (beta 0.9 / 0.999, eps 1e-8) on batches of `batch` samples drawn by a counter-based
+ * hash of (step, row) (the whole set in order when batch >= n); use_graph: replay one captured step.
+ * loss: mean squared error of the last batch */
+int mst_mlp_train(mst_mlp *m, int steps, int batch, double lr, int use_graph, double *loss);
+/* de-normalised outputs Y [n x n_out] of X [n x n_in] (host arrays) */
+int mst_mlp_forward(mst_mlp *m, const float *X, int64_t n, float *Y);
+/* gradient of the mean squared error over (X, T) w.r.t. the parameters, and the loss (host arrays) */
+int mst_mlp_gradient(mst_mlp *m, const float *X, const float *T, int64_t n, float *grad, double *loss);
+/* Old state := decoded network prediction at the cell centres.  Features: (x_c (dim), cond[0..n_cond)),
+ * n_in = dim + n_cond, n_out = nv.  Outputs y decode as rho = exp(y_0), u_d = cond[0] * y_{1+d},
+ * p = exp(y_{nv-1}) / gamma (freestream rho = 1, p = 1/gamma, |u| = cond[0] = Mach), so density and
+ * pressure are positive; a NaN prediction returns MST_E_STATE (the caller keeps the freestream start). */
+int mst_ctx_warm_start(mst_ctx *ctx, mst_mlp *m, const double *cond, int n_cond);
+
 /* ---------------------------------------------------------------- timing (HIP events on the ctx stream) */
 #define MST_K_GRAD 0
 #define MST_K_FLUX 1

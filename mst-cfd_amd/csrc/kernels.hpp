@@ -631,6 +631,54 @@ static __global__ __launch_bounds__(FIN) void k_finish(const FinArgs a) {
     }
 }
 
+// ------------------------------------------------------------------------------------ warm start
+// features (cell centre, conditions) for the network and the decoding of its outputs
+//   y = (log(rho/rho_inf), u_1/|u_inf|, .., u_dim/|u_inf|, log(p/p_inf)),  rho_inf = 1, p_inf = 1/gamma,
+//   |u_inf| = the first condition (the freestream Mach number: a_inf = 1)
+// which gives a positive density and pressure for any finite output.
+struct WarmArgs {
+    int64_t nc;
+    const double *cxc;
+    int n_cond;
+    float cond[8];
+    double mach, gamma;
+    float *feat;
+    const float *Y;
+    double *Q;
+    int *bad;
+};
+
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_warm_features(const WarmArgs w) {
+    const int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= w.nc) return;
+    const int n_in = DIM + w.n_cond;
+    for (int d = 0; d < DIM; d++) w.feat[c * n_in + d] = (float)w.cxc[c * DIM + d];
+    for (int k = 0; k < w.n_cond; k++) w.feat[c * n_in + DIM + k] = w.cond[k];
+}
+
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_warm_decode(const WarmArgs w) {
+    constexpr int NV = DIM + 2;
+    const int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= w.nc) return;
+    const float *y = w.Y + c * NV;
+    const double rho = exp((double)y[0]);
+    const double p = exp((double)y[NV - 1]) / w.gamma;
+    double ke = 0, q[NV];
+    q[0] = rho;
+    for (int d = 0; d < DIM; d++) {
+        const double u = w.mach * (double)y[1 + d];
+        q[1 + d] = rho * u;
+        ke += u * u;
+    }
+    q[NV - 1] = p / (w.gamma - 1) + 0.5 * rho * ke;
+    bool ok = rho > 0 && p > 0 && rho < INFINITY && p < INFINITY;
+    for (int k = 0; k < NV; k++) ok = ok && q[k] == q[k];
+    if (!ok) w.bad[0] = 1;
+    stq<NV>(w.Q + c * NV, q);
+}
+
 // FP32 shadow of a state (mixed form), after the host writes the FP64 state
 static __global__ __launch_bounds__(BLOCK) void k_shadow(int64_t n, const double *__restrict__ Q, float *__restrict__ Q32) {
     const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "mlp.hpp"
 #include "mstcfd.h"
 #include "prep.hpp"
 
@@ -741,6 +742,45 @@ int mst_monitor_maxrel(mst_ctx *ctx, double *out) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(out, ctx->scal + 6, sizeof(double) * ctx->nv, hipMemcpyDeviceToHost));
     return MST_OK;
+}
+
+int mst_ctx_warm_start(mst_ctx *ctx, mst_mlp *mlp, const double *cond, int n_cond) {
+    CTX_GUARD(ctx);
+    if (!mlp || n_cond < 1 || n_cond > 8 || !cond) return mst::fail(MST_E_ARG, "bad arguments");
+    int n_in = 0, n_out = 0;
+    mst::mlp_dims(mlp, &n_in, &n_out);
+    if (n_in != ctx->dim + n_cond || n_out != ctx->nv)
+        return mst::fail(MST_E_ARG, "network sizes do not match (dim + conditions) -> nv");
+    if (!(cond[0] > 0)) return mst::fail(MST_E_ARG, "the first condition (freestream Mach number) must be positive");
+    WarmArgs w;
+    w.nc = ctx->nc;
+    w.cxc = ctx->cxc;
+    w.n_cond = n_cond;
+    for (int k = 0; k < 8; k++) w.cond[k] = k < n_cond ? (float)cond[k] : 0.f;
+    w.mach = cond[0];
+    w.gamma = ctx->cfg.gamma;
+    w.Q = ctx->Q[ctx->iold];
+    w.bad = ctx->bad;
+    float *feat = nullptr, *out = nullptr;
+    HIPCHK(hipMalloc((void **)&feat, sizeof(float) * ctx->nc * n_in));
+    if (hipMalloc((void **)&out, sizeof(float) * ctx->nc * n_out) != hipSuccess) {
+        (void)hipFree(feat);
+        return mst::fail(MST_E_HIP, "hipMalloc failed");
+    }
+    w.feat = feat;
+    w.Y = out;
+    int rc = ctx->dim == 2 ? launch(ctx, -1, k_warm_features<2>, ctx->nc, w) : launch(ctx, -1, k_warm_features<3>, ctx->nc, w);
+    if (!rc) rc = mst::mlp_forward_device(mlp, feat, ctx->nc, out, ctx->stream);
+    if (!rc)
+        rc = ctx->dim == 2 ? launch(ctx, -1, k_warm_decode<2>, ctx->nc, w) : launch(ctx, -1, k_warm_decode<3>, ctx->nc, w);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(feat);
+    (void)hipFree(out);
+    if (rc) return rc;
+    if (e != hipSuccess) return mst::fail(MST_E_HIP, hipGetErrorString(e));
+    for (int i = 0; i < 4; i++) ctx->shadow_ok[i] = false;
+    ctx->dt_valid = false;
+    return check_bad(ctx);
 }
 
 int mst_get_residual(mst_ctx *ctx, double *R, int64_t n_cells) {

@@ -15,6 +15,7 @@ LIB_PATH = os.environ.get("MSTCFD_LIB") or os.path.join(os.path.dirname(_HERE), 
 
 I32P = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 F64P = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+F32P = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 
 BC_INTERIOR, BC_WALL, BC_OUTLET, BC_SYMMETRY, BC_INLET, BC_PERIODIC = 2, 3, 5, 7, 10, 12
 ROE, AUSM = 0, 1
@@ -43,6 +44,9 @@ EXPORTS = [
     "mst_batch_create", "mst_batch_destroy", "mst_batch_counts", "mst_batch_case_counts", "mst_batch_set_state",
     "mst_batch_get_state", "mst_batch_step", "mst_batch_residual_norms", "mst_batch_dt", "mst_batch_set_active",
     "mst_batch_set_tuning", "mst_batch_get_tuning", "mst_batch_set_graph", "mst_batch_autotune",
+    "mst_mlp_create", "mst_mlp_destroy", "mst_mlp_num_params", "mst_mlp_get_params", "mst_mlp_set_params",
+    "mst_mlp_set_scaling", "mst_mlp_set_data", "mst_mlp_reset_optimizer", "mst_mlp_train", "mst_mlp_forward",
+    "mst_mlp_gradient", "mst_ctx_warm_start",
 ]
 
 
@@ -162,6 +166,18 @@ def lib():
         "mst_batch_get_tuning": (C.c_int, [vp, C.POINTER(C.c_int), I32P]),
         "mst_batch_set_graph": (C.c_int, [vp, C.c_int]),
         "mst_batch_autotune": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+        "mst_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, pp]),
+        "mst_mlp_destroy": (C.c_int, [vp]),
+        "mst_mlp_num_params": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "mst_mlp_get_params": (C.c_int, [vp, F32P, C.c_int64]),
+        "mst_mlp_set_params": (C.c_int, [vp, F32P, C.c_int64]),
+        "mst_mlp_set_scaling": (C.c_int, [vp, F32P, F32P, F32P, F32P]),
+        "mst_mlp_set_data": (C.c_int, [vp, F32P, F32P, C.c_int64]),
+        "mst_mlp_reset_optimizer": (C.c_int, [vp]),
+        "mst_mlp_train": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_double)]),
+        "mst_mlp_forward": (C.c_int, [vp, F32P, C.c_int64, F32P]),
+        "mst_mlp_gradient": (C.c_int, [vp, F32P, F32P, C.c_int64, F32P, C.POINTER(C.c_double)]),
+        "mst_ctx_warm_start": (C.c_int, [vp, vp, F64P, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -330,6 +346,11 @@ class Solver:
     def set_config(self, cfg):
         _check(lib().mst_ctx_set_config(self.h, C.byref(cfg)), "mst_ctx_set_config")
         self.cfg = cfg
+
+    def warm_start(self, mlp, cond):
+        """Old state := the network's decoded prediction at the cell centres (mst_ctx_warm_start)."""
+        c = np.ascontiguousarray(cond, np.float64).reshape(-1)
+        _check(lib().mst_ctx_warm_start(self.h, mlp.h, c, c.size), "mst_ctx_warm_start")
 
     @property
     def strategy(self):
@@ -516,3 +537,73 @@ class Batch:
         _check(lib().mst_batch_autotune(self.h, path, 1 if force else 0, int(steps), C.byref(ms)),
                "mst_batch_autotune")
         return ms.value
+
+
+class MLP:
+    """Fully connected tanh network trained and evaluated by the library's kernels (FP32)."""
+
+    def __init__(self, n_in=4, width=128, hidden=4, n_out=4, seed=2024):
+        h = C.c_void_p()
+        _check(lib().mst_mlp_create(int(n_in), int(width), int(hidden), int(n_out), int(seed), C.byref(h)),
+               "mst_mlp_create")
+        self.h = h
+        self.n_in, self.width, self.hidden, self.n_out = n_in, width, hidden, n_out
+        n = C.c_int64()
+        _check(lib().mst_mlp_num_params(self.h, C.byref(n)), "mst_mlp_num_params")
+        self.n_params = n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mst_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def layers(self):
+        """[(W, b)] shapes in parameter order."""
+        dims = [self.n_in] + [self.width] * self.hidden + [self.n_out]
+        return [(dims[i + 1], dims[i]) for i in range(len(dims) - 1)]
+
+    def get_params(self):
+        p = np.zeros(self.n_params, np.float32)
+        _check(lib().mst_mlp_get_params(self.h, p, p.size), "mst_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().mst_mlp_set_params(self.h, p, p.size), "mst_mlp_set_params")
+
+    def set_scaling(self, in_shift, in_scale, out_shift, out_scale):
+        f = lambda a: np.ascontiguousarray(a, np.float32)
+        _check(lib().mst_mlp_set_scaling(self.h, f(in_shift), f(in_scale), f(out_shift), f(out_scale)),
+               "mst_mlp_set_scaling")
+
+    def set_data(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        _check(lib().mst_mlp_set_data(self.h, X.reshape(-1), T.reshape(-1), X.shape[0]), "mst_mlp_set_data")
+
+    def reset_optimizer(self):
+        _check(lib().mst_mlp_reset_optimizer(self.h), "mst_mlp_reset_optimizer")
+
+    def train(self, steps, batch=8192, lr=1e-3, graph=True):
+        loss = C.c_double()
+        _check(lib().mst_mlp_train(self.h, int(steps), int(batch), float(lr), 1 if graph else 0, C.byref(loss)),
+               "mst_mlp_train")
+        return loss.value
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((X.shape[0], self.n_out), np.float32)
+        _check(lib().mst_mlp_forward(self.h, X.reshape(-1), X.shape[0], Y.reshape(-1)), "mst_mlp_forward")
+        return Y
+
+    def gradient(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_double()
+        _check(lib().mst_mlp_gradient(self.h, X.reshape(-1), T.reshape(-1), X.shape[0], g, C.byref(loss)),
+               "mst_mlp_gradient")
+        return g, loss.value
