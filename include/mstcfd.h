@@ -124,6 +124,8 @@ int mst_mesh_gen_tet(int nx, int ny, int nz, const double lo[3], const double hi
 #define MST_INIT_PULSE 4      /* params: gamma, amp, sigma, centre(dim), mach: rho = 1+b, p = (1+b)/gamma, u = mach e_x */
 #define MST_INIT_FREESTREAM 5 /* params: gamma, mach, angle (deg): rho = 1, p = 1/gamma, |u| = mach */
 int mst_mesh_init_state(const mst_mesh *m, int kind, const double *params, int n_params, double *Q, int64_t n_cells);
+/* cell centres (mean of face centres, mesh/Cell.cpp:6-14), n_cells*dim, file order */
+int mst_mesh_cell_centres(const mst_mesh *m, double *ccen, int64_t n_cells);
 
 /* ---------------------------------------------------------------- host preprocessing (no device needed)
  * The one-time part of the reference's mesh set-up (mesh/MshBlock.cpp:272-334, mesh/Face.cpp:8-69,
@@ -187,6 +189,11 @@ int mst_step(mst_ctx *ctx, int nsteps);
 #define MST_STRATEGY_CELL 1
 int mst_set_strategy(mst_ctx *ctx, int strategy);
 int mst_get_strategy(const mst_ctx *ctx, int *strategy);
+/* NEW: steady driver.  Steps of mst_step until max_f L2(R/V)_f / ref_f <= tol (checked every
+ * check_every steps) or max_steps.  ref[nv]: reference norms; when ref[0] <= 0 on entry the first
+ * step's norms are taken and written back.  steps_done / drop: steps taken and the final ratio. */
+int mst_run_to_steady(mst_ctx *ctx, double tol, int max_steps, int check_every, double *ref, int *steps_done,
+                      double *drop);
 /* NEW: dt = cfl * min_c V_c / sum_d (|u_d| + a) Shalf_c,d on the Old state */
 int mst_compute_dt(mst_ctx *ctx, double cfl, double *dt);
 /* NEW: L2 norm over cells of R(Q_old)/V per field (nv values) from the first stage of the last step */
@@ -234,6 +241,11 @@ int mst_batch_residual_norms(mst_batch *b, double *out);
 int mst_batch_dt(mst_batch *b, double *out);
 /* active[n_cases]: inactive cases keep their state and are left out of the work list */
 int mst_batch_set_active(mst_batch *b, const uint8_t *active);
+/* steady driver for every active case: a case that reaches max_f L2(R/V)_f / ref_f <= tol is frozen;
+ * ref [n_cases*nv] in/out (a case with ref <= 0 takes its first step's norms), steps [n_cases] and
+ * drop [n_cases] out.  The activity flags are restored at the end. */
+int mst_batch_run_to_steady(mst_batch *b, double tol, int max_steps, int check_every, double *ref, int32_t *steps,
+                            double *drop);
 /* launch shape: block size (64, 128, 256) and cell tile per case-size bucket (faces: scaled by nf/nc) */
 int mst_batch_set_tuning(mst_batch *b, int block, const int32_t *tiles);
 int mst_batch_get_tuning(const mst_batch *b, int *block, int32_t *tiles);

@@ -251,6 +251,9 @@ struct mst_batch {
     double *G = nullptr, *lg = nullptr;
     float *Q32[4] = {nullptr, nullptr, nullptr, nullptr};
     float *fn32 = nullptr, *feta32 = nullptr, *fd0 = nullptr, *fd1 = nullptr, *lg32 = nullptr;
+    int32_t *inc_code = nullptr;  // gradient incidence layout (slot-major, stride NC)
+    double *inc_w = nullptr, *inc_n = nullptr, *inc_dx = nullptr;
+    float *inc_w32 = nullptr, *inc_n32 = nullptr, *inc_dx32 = nullptr;
     Params<double> *P64 = nullptr;
     Params<float> *P32 = nullptr;
     int32_t *scheme = nullptr;
@@ -303,7 +306,8 @@ void release(mst_batch *b) {
     void *bufs[] = {b->fc0, b->fc1, b->finfo, b->ell, b->fn, b->fxf, b->feta, b->cvol, b->cxc, b->cshalf,
                     b->Q[0], b->Q[1], b->Q[2], b->Q[3], b->G, b->lg, b->Q32[0], b->Q32[1], b->Q32[2], b->Q32[3],
                     b->fn32, b->feta32, b->fd0, b->fd1, b->lg32, b->P64, b->P32, b->scheme, b->dts, b->gam,
-                    b->cfl, b->l2, b->ncells, b->citems, b->fitems, b->ci0, b->ci1, b->part, b->cnt, b->bad};
+                    b->cfl, b->l2, b->ncells, b->citems, b->fitems, b->ci0, b->ci1, b->part, b->cnt, b->bad,
+                    b->inc_code, b->inc_w, b->inc_n, b->inc_dx, b->inc_w32, b->inc_n32, b->inc_dx32};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -360,16 +364,7 @@ BArgs<T> base_args(const mst_batch *b, int isrc) {
     memset(&a, 0, sizeof(a));
     GradArgs<T> &g = a.g;
     g.nc = b->NC;
-    g.pstart = 0;
-    g.pcount = 0;
-    g.pxf = nullptr;
-    g.ell = b->ell;
-    g.fc0 = b->fc0;
-    g.fc1 = b->fc1;
-    g.fxf = b->fxf;
-    g.cxc = b->cxc;
-    g.fd0 = b->fd0;
-    g.fd1 = b->fd1;
+    g.inc_code = b->inc_code;
     g.vol = b->cvol;
     g.graw = nullptr;
     g.phi = nullptr;
@@ -387,16 +382,18 @@ BArgs<T> base_args(const mst_batch *b, int isrc) {
     f.fd1 = b->fd1;
     f.G = b->G;
     if constexpr (std::is_same<T, float>::value) {
-        g.fn = b->fn32;
-        g.feta = b->feta32;
+        g.inc_w = b->inc_w32;
+        g.inc_n = b->inc_n32;
+        g.inc_dx = b->inc_dx32;
         g.Q = b->Q32[isrc];
         g.lg = b->lg32;
         f.Q = b->Q32[isrc];
         f.lg = b->lg32;
         a.P = b->P32;
     } else {
-        g.fn = b->fn;
-        g.feta = b->feta;
+        g.inc_w = b->inc_w;
+        g.inc_n = b->inc_n;
+        g.inc_dx = b->inc_dx;
         g.Q = b->Q[isrc];
         g.lg = b->lg;
         f.Q = b->Q[isrc];
@@ -799,6 +796,10 @@ int mst_batch_create(int n_cases, const mst_mesh *const *meshes, const mst_confi
         d0.assign(b->NF * dim, 0.f);
         d1.assign(b->NF * dim, 0.f);
     }
+    std::vector<int32_t> inc_code((size_t)b->slots * b->NC, -2);
+    std::vector<double> inc_w((size_t)b->slots * b->NC, 0.0), inc_n((size_t)b->slots * b->NC * dim, 0.0),
+        inc_dx((size_t)b->slots * b->NC * dim, 0.0);
+    std::vector<float> inc_w32(b->mixed ? (size_t)b->slots * b->NC : 0, 0.f);
     std::vector<Params<double>> p64(n_cases);
     std::vector<Params<float>> p32(n_cases);
     for (int i = 0; i < n_cases; i++) {
@@ -833,6 +834,22 @@ int mst_batch_create(int n_cases, const mst_mesh *const *meshes, const mst_confi
                 if (e >= 0) ell[(size_t)s * b->NC + cs + k] = (int32_t)((((int64_t)(e >> 1) + fs) << 1) | (e & 1));
             }
         }
+        for (int sl = 0; sl < b->slots; sl++)
+            for (int64_t k = 0; k < P.nc; k++) {
+                const size_t gi = (size_t)sl * b->NC + cs + k;
+                int32_t code;
+                double nn[3], dd[3];
+                const int64_t f = mst::incidence(P, sl, k, code, nn, dd);
+                inc_code[gi] = code >= 0 ? (int32_t)((((int64_t)(code >> 1) + cs) << 1) | (code & 1)) : code;
+                if (f < 0) continue;
+                const double e = feta[fs + f];
+                inc_w[gi] = e;
+                for (int d = 0; d < dim; d++) {
+                    inc_n[gi * dim + d] = nn[d];
+                    inc_dx[gi * dim + d] = dd[d];
+                }
+                if (b->mixed) inc_w32[gi] = (code >= 0 && (code & 1)) ? (float)e : 1.0f - (float)e;
+            }
         p64[i] = make_params<double>(c);
         p32[i] = make_params<float>(c);
         scheme[i] = c.scheme;
@@ -869,6 +886,16 @@ int mst_batch_create(int n_cases, const mst_mesh *const *meshes, const mst_confi
     TRY(dupload(&b->cvol, cvol));
     TRY(dupload(&b->cxc, cxc));
     TRY(dupload(&b->cshalf, cshalf));
+    TRY(dupload(&b->inc_code, inc_code));
+    TRY(dupload(&b->inc_w, inc_w));
+    TRY(dupload(&b->inc_n, inc_n));
+    TRY(dupload(&b->inc_dx, inc_dx));
+    if (b->mixed) {
+        std::vector<float> n32(inc_n.begin(), inc_n.end()), d32(inc_dx.begin(), inc_dx.end());
+        TRY(dupload(&b->inc_w32, inc_w32));
+        TRY(dupload(&b->inc_n32, n32));
+        TRY(dupload(&b->inc_dx32, d32));
+    }
     TRY(dupload(&b->P64, p64));
     TRY(dupload(&b->P32, p32));
     TRY(dupload(&b->scheme, scheme));
@@ -1014,6 +1041,68 @@ int mst_batch_set_active(mst_batch *b, const uint8_t *active) {
         b->active[i] = a;
     }
     return build_items(b);
+}
+
+int mst_batch_run_to_steady(mst_batch *b, double tol, int max_steps, int check_every, double *ref, int32_t *steps,
+                            double *drop) {
+    B_GUARD(b);
+    if (!ref || !steps || !drop || !(tol > 0) || max_steps < 1 || check_every < 1)
+        return mst::fail(MST_E_ARG, "bad arguments");
+    const int nv = b->nv, n = b->n;
+    const std::vector<uint8_t> active0 = b->active;
+    std::vector<uint8_t> act = active0;
+    std::vector<double> r((size_t)n * nv);
+    for (int i = 0; i < n; i++) {
+        steps[i] = 0;
+        drop[i] = INFINITY;
+    }
+    int done = 0;
+    int rc = MST_OK;
+    // cases without a reference take the residual of their first step
+    bool need_ref = false;
+    for (int i = 0; i < n; i++) need_ref |= act[i] && !(ref[(size_t)i * nv] > 0);
+    if (need_ref) {
+        rc = mst_batch_step(b, 1);
+        if (!rc) rc = mst_batch_residual_norms(b, r.data());
+        if (rc) return rc;
+        for (int i = 0; i < n; i++) {
+            if (!act[i]) continue;
+            steps[i] = 1;
+            if (!(ref[(size_t)i * nv] > 0))
+                for (int k = 0; k < nv; k++) ref[(size_t)i * nv + k] = r[(size_t)i * nv + k];
+        }
+        done = 1;
+    }
+    while (done < max_steps) {
+        int nact = 0;
+        for (int i = 0; i < n; i++) nact += act[i];
+        if (!nact) break;
+        const int k = std::min(check_every, max_steps - done);
+        rc = mst_batch_step(b, k);
+        if (rc) break;
+        done += k;
+        rc = mst_batch_residual_norms(b, r.data());
+        if (rc) break;
+        bool changed = false;
+        for (int i = 0; i < n; i++) {
+            if (!act[i]) continue;
+            steps[i] += k;
+            double q = 0;
+            for (int c = 0; c < nv; c++)
+                q = std::max(q, ref[(size_t)i * nv + c] > 0 ? r[(size_t)i * nv + c] / ref[(size_t)i * nv + c] : 0.0);
+            drop[i] = q;
+            if (q <= tol) {  // converged: freeze
+                act[i] = 0;
+                changed = true;
+            }
+        }
+        if (changed) {
+            rc = mst_batch_set_active(b, act.data());
+            if (rc) break;
+        }
+    }
+    int rc2 = mst_batch_set_active(b, active0.data());
+    return rc ? rc : rc2;
 }
 
 int mst_batch_set_tuning(mst_batch *b, int block, const int32_t *tiles) {

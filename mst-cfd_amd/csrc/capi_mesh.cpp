@@ -154,6 +154,18 @@ int mst_mesh_gen_ramp(int nx, int ny, double x0, double x1, double height, doubl
     return MST_OK;
 }
 
+int mst_mesh_cell_centres(const mst_mesh *m, double *ccen, int64_t n_cells) {
+    if (!m || !ccen) return mst::fail(MST_E_ARG, "null argument");
+    if (n_cells != m->n_cells) return mst::fail(MST_E_ARG, "size does not match the mesh");
+    MST_GUARD({
+        std::vector<double> c;
+        int rc = mst::cell_centres(*m, c);
+        if (rc) return rc;
+        std::copy(c.begin(), c.end(), ccen);
+    });
+    return MST_OK;
+}
+
 int mst_mesh_gen_tet(int nx, int ny, int nz, const double lo[3], const double hi[3], double jitter, uint64_t seed,
                      const int32_t bc[6], mst_mesh **out) {
     if (!out || !bc || !lo || !hi) return mst::fail(MST_E_ARG, "null argument");

@@ -151,17 +151,15 @@ __global__ __launch_bounds__(BLOCK, MST_FLUX_WAVES) void k_flux(const FluxArgs<T
 // ------------------------------------------------------------------------------------ gradient + limiter
 template <typename T>
 struct GradArgs {
-    int64_t nc, pstart, pcount;
-    const double *__restrict__ pxf;  // FP64 form: partner face centres of the periodic pairs
-    const int32_t *__restrict__ ell;
-    const int32_t *__restrict__ fc0;
-    const int32_t *__restrict__ fc1;
-    const T *__restrict__ fn;
-    const T *__restrict__ feta;
-    const double *__restrict__ fxf;  // FP64 form: face centres and cell centres
-    const double *__restrict__ cxc;
-    const float *__restrict__ fd0;   // FP32 form: offsets x_f - x_c0, x_f' - x_c1
-    const float *__restrict__ fd1;
+    int64_t nc;
+    // incidence layout (slot-major, index s*nc + c): (neighbour << 1) | is_c1, -1 boundary face, -2
+    // empty slot; FP64: eta of the face, FP32: the difference-form weight; the cell's outward normal;
+    // offset x_f - x_c (partner face centre on the c1 side of a periodic pair).  Every load but the
+    // neighbour states is coalesced.
+    const int32_t *__restrict__ inc_code;
+    const T *__restrict__ inc_w;
+    const T *__restrict__ inc_n;
+    const T *__restrict__ inc_dx;
     const T *__restrict__ Q;
     const double *__restrict__ vol;
     T *__restrict__ lg;
@@ -170,10 +168,10 @@ struct GradArgs {
 };
 
 // RhoSolver::updateGradFlux (RhoSolver.cpp:430-454) fused with the (new) per-cell limiter.
-// FP64: grad = (1/V) sum_f Q_f n_out as the reference; FP32: the same sum written with differences,
-// (1/V) sum_f w_f (Q_nb - Q_c) n_out (w = 1-eta on the c0 side, eta on the c1 side), equal because
-// the outward normals of a closed cell sum to zero, and free of the cancellation an FP32 sum of
-// whole states would suffer.
+// FP64: grad = (1/V) sum_f Q_f n_out as the reference, Q_f = eta Q_c0 + (1-eta) Q_c1; FP32: the same
+// sum written with differences, (1/V) sum_f w_f (Q_nb - Q_c) n_out (w = 1-eta on the c0 side, eta on
+// the c1 side), equal because the outward normals of a closed cell sum to zero, and free of the
+// cancellation an FP32 sum of whole states would suffer.
 template <int DIM, int MAXF, int LIM, typename T>
 __device__ __forceinline__ void grad_cell(const GradArgs<T> &a, const int64_t c) {
     constexpr int NV = DIM + 2;
@@ -187,35 +185,28 @@ __device__ __forceinline__ void grad_cell(const GradArgs<T> &a, const int64_t c)
         for (int d = 0; d < DIM; d++) t[k][d] = 0;
     }
     T dxs[MAXF][DIM];
-    double xf[MAXF][DIM];
     int nfc = 0;
 #pragma unroll
     for (int s = 0; s < MAXF; s++) {
-        const int32_t e = a.ell[(int64_t)s * a.nc + c];
-        if (e < 0) break;
-        const int64_t f = e >> 1;
-        const bool is1 = e & 1;
-        const int64_t nb = is1 ? a.fc0[f] : a.fc1[f];
+        const int64_t i = (int64_t)s * a.nc + c;
+        const int32_t code = a.inc_code[i];
+        if (code == -2) break;
+        const int64_t nb = code >= 0 ? (int64_t)(code >> 1) : -1;
+        const bool is1 = code >= 0 && (code & 1);
+        const T eta = a.inc_w[i];
         T n[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
-            const T v = a.fn[f * DIM + d];
-            n[d] = is1 ? -v : v;
-            if constexpr (F32)
-                dxs[s][d] = (T)(is1 ? a.fd1[f * DIM + d] : a.fd0[f * DIM + d]);
-            else  // the c1 side of a periodic pair sees the partner face's centre
-                xf[s][d] = (is1 && f >= a.pstart && f < a.pstart + a.pcount) ? a.pxf[(f - a.pstart) * DIM + d]
-                                                                             : a.fxf[f * DIM + d];
+            n[d] = a.inc_n[i * DIM + d];
+            dxs[s][d] = a.inc_dx[i * DIM + d];
         }
         T qf[NV];
         if (nb >= 0) {
             T qn[NV];
             ldq<NV>(a.Q + nb * NV, qn);
-            const T eta = a.feta[f];
             if constexpr (F32) {
-                const T w = is1 ? eta : T(1) - eta;
 #pragma unroll
-                for (int k = 0; k < NV; k++) qf[k] = w * (qn[k] - qc[k]);
+                for (int k = 0; k < NV; k++) qf[k] = eta * (qn[k] - qc[k]);
             } else {
 #pragma unroll
                 for (int k = 0; k < NV; k++) qf[k] = is1 ? eta * qn[k] + (1 - eta) * qc[k] : eta * qc[k] + (1 - eta) * qn[k];
@@ -236,14 +227,6 @@ __device__ __forceinline__ void grad_cell(const GradArgs<T> &a, const int64_t c)
         nfc = s + 1;
     }
     const T iV = (T)frcp(a.vol[c]);
-    if constexpr (!F32) {
-#pragma unroll
-        for (int s = 0; s < MAXF; s++) {
-            if (s >= nfc) break;
-#pragma unroll
-            for (int d = 0; d < DIM; d++) dxs[s][d] = (T)(xf[s][d] - a.cxc[c * DIM + d]);
-        }
-    }
     T ph[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) {

@@ -447,6 +447,25 @@ int prepare(const mst_mesh &m, Prepared &P) {
     return MST_OK;
 }
 
+int64_t incidence(const Prepared &P, int s, int64_t c, int32_t &code, double *n, double *dx) {
+    const int dim = P.dim;
+    code = -2;
+    if (s >= P.maxf) return -1;
+    const int32_t e = P.ell[(int64_t)s * P.nc + c];
+    if (e < 0) return -1;
+    const int64_t f = e >> 1;
+    const bool is1 = e & 1;
+    const int32_t nb = is1 ? P.fc0[f] : P.fc1[f];
+    code = nb >= 0 ? (int32_t)(((int64_t)nb << 1) | (is1 ? 1 : 0)) : -1;
+    const bool per = is1 && P.fkind[f] == KIND_PERIODIC;
+    for (int d = 0; d < dim; d++) {
+        n[d] = is1 ? -P.fn[f * dim + d] : P.fn[f * dim + d];
+        const double xf = per ? P.pxf[(f - P.pstart) * dim + d] : P.fxf[f * dim + d];
+        dx[d] = xf - P.cxc[c * dim + d];
+    }
+    return f;
+}
+
 void lr_masks(const Prepared &P, int compat, std::vector<int32_t> &mask) {
     const int dim = P.dim;
     mask.assign(P.nf, 0);

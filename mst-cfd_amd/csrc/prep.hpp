@@ -56,6 +56,11 @@ int prepare(const mst_mesh &m, Prepared &P);
 // per new face: bit d set when c0 (or the interior-side state) is the Riemann left state along d
 void lr_masks(const Prepared &P, int compat, std::vector<int32_t> &mask);
 
+// Incidence entry of slot s of cell c (the gradient kernel's layout): code = (neighbour << 1) | is_c1,
+// -1 for a boundary face, -2 for an empty slot; n = the cell's outward normal; dx = x_f - x_c (the
+// partner face's centre on the c1 side of a periodic pair).  Returns the face (or -1).
+int64_t incidence(const Prepared &P, int s, int64_t c, int32_t &code, double *n, double *dx);
+
 }  // namespace mst
 
 // the opaque handle of the C ABI

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -66,6 +67,12 @@ struct mst_ctx {
     float *Q32[4] = {nullptr, nullptr, nullptr, nullptr};
     bool shadow_ok[4] = {false, false, false, false};
     float *fn32 = nullptr, *feta32 = nullptr, *fd0 = nullptr, *fd1 = nullptr, *lg32 = nullptr;
+    // incidence layout of the gradient kernel (slot-major over the padded ELL slots)
+    int slots = 3;
+    int64_t n_inc = 0;  // non-empty incidences
+    int32_t *inc_code = nullptr;
+    double *inc_w = nullptr, *inc_n = nullptr, *inc_dx = nullptr;
+    float *inc_w32 = nullptr, *inc_n32 = nullptr, *inc_dx32 = nullptr;
     double *scal = nullptr;  // [0] dt, [1..5] l2 norms, [6..10] maxrel
     double *part = nullptr;  // per-block partials
     double *fin_mid = nullptr;  // first-level results of the two-level finish
@@ -121,27 +128,20 @@ template <int DIM, typename T>
 int run_grad_t(mst_ctx *x, int isrc, bool debug) {
     GradArgs<T> a;
     a.nc = x->nc;
-    a.ell = x->ell;
-    a.fc0 = x->fc0;
-    a.fc1 = x->fc1;
+    a.inc_code = x->inc_code;
     if constexpr (std::is_same<T, float>::value) {
-        a.fn = x->fn32;
-        a.feta = x->feta32;
+        a.inc_w = x->inc_w32;
+        a.inc_n = x->inc_n32;
+        a.inc_dx = x->inc_dx32;
         a.Q = x->Q32[isrc];
         a.lg = x->lg32;
     } else {
-        a.fn = x->fn;
-        a.feta = x->feta;
+        a.inc_w = x->inc_w;
+        a.inc_n = x->inc_n;
+        a.inc_dx = x->inc_dx;
         a.Q = x->Q[isrc];
         a.lg = x->lg;
     }
-    a.fxf = x->fxf;
-    a.cxc = x->cxc;
-    a.pstart = x->prep.P.pstart;
-    a.pcount = x->prep.P.pcount;
-    a.pxf = x->pxf;
-    a.fd0 = x->fd0;
-    a.fd1 = x->fd1;
     a.vol = x->cvol;
     a.graw = debug ? x->graw : nullptr;
     a.phi = debug ? x->phi : nullptr;
@@ -150,8 +150,8 @@ int run_grad_t(mst_ctx *x, int isrc, bool debug) {
     if (lim == MST_LIM_NONE) return launch(x, MST_K_GRAD, k_grad<DIM, MF, MST_LIM_NONE, T>, x->nc, a);     \
     if (lim == MST_LIM_MINMOD) return launch(x, MST_K_GRAD, k_grad<DIM, MF, MST_LIM_MINMOD, T>, x->nc, a); \
     return launch(x, MST_K_GRAD, k_grad<DIM, MF, MST_LIM_VANLEER, T>, x->nc, a);
-    if (x->maxf <= 3) { MST_GRAD(3) }
-    if (x->maxf <= 4) { MST_GRAD(4) }
+    if (x->slots == 3) { MST_GRAD(3) }
+    if (x->slots == 4) { MST_GRAD(4) }
     MST_GRAD(8)
 #undef MST_GRAD
 }
@@ -408,6 +408,64 @@ int alloc_mixed(mst_ctx *x) {
     return rc;
 }
 
+// incidence arrays from the padded ELL table: code, signed normal, offset (FP64, and FP32 copies
+// in the mixed form); the weights follow the configuration (upload_incidence_weights)
+int build_incidence(mst_ctx *x) {
+    const mst::Prepared &P = x->prep.P;
+    const int dim = x->dim;
+    const int64_t n = (int64_t)x->slots * x->nc;
+    std::vector<int32_t> code(n, -2);
+    std::vector<double> nrm(n * dim, 0.0), dx(n * dim, 0.0);
+    for (int s = 0; s < x->slots; s++)
+        for (int64_t c = 0; c < x->nc; c++) {
+            const int64_t i = (int64_t)s * x->nc + c;
+            if (mst::incidence(P, s, c, code[i], &nrm[i * dim], &dx[i * dim]) >= 0) x->n_inc++;
+        }
+    int rc = dupload(&x->inc_code, code);
+    if (!rc) rc = dupload(&x->inc_n, nrm);
+    if (!rc) rc = dupload(&x->inc_dx, dx);
+    if (!rc) rc = dalloc(&x->inc_w, n);
+    return rc;
+}
+
+// per incidence: FP64 eta of the face (with the last-interior-face quirk), FP32 the difference-form
+// weight (1 - eta on the c0 side, eta on the c1 side)
+int upload_incidence_weights(mst_ctx *x, const std::vector<double> &eta) {
+    const mst::Prepared &P = x->prep.P;
+    const int64_t n = (int64_t)x->slots * x->nc;
+    std::vector<double> w(n, 0.0);
+    std::vector<float> w32(x->mixed ? n : 0, 0.f);
+    for (int s = 0; s < P.maxf; s++)
+        for (int64_t c = 0; c < x->nc; c++) {
+            const int32_t e = P.ell[(int64_t)s * x->nc + c];
+            if (e < 0) continue;
+            const int64_t i = (int64_t)s * x->nc + c;
+            w[i] = eta[e >> 1];
+            if (x->mixed) {
+                const float ef = (float)eta[e >> 1];
+                w32[i] = (e & 1) ? ef : 1.0f - ef;
+            }
+        }
+    HIPCHK(hipMemcpy(x->inc_w, w.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    if (x->mixed) {
+        if (!x->inc_w32) {
+            const int dim = x->dim;
+            int rc = dalloc(&x->inc_w32, n);
+            if (rc) return rc;
+            std::vector<double> h(n * dim);
+            std::vector<float> f(n * dim);
+            for (auto pr : {std::make_pair(x->inc_n, &x->inc_n32), std::make_pair(x->inc_dx, &x->inc_dx32)}) {
+                HIPCHK(hipMemcpy(h.data(), pr.first, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+                for (size_t k = 0; k < h.size(); k++) f[k] = (float)h[k];
+                rc = dupload(pr.second, f);
+                if (rc) return rc;
+            }
+        }
+        HIPCHK(hipMemcpy(x->inc_w32, w32.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    }
+    return MST_OK;
+}
+
 int upload_config(mst_ctx *x) {
     std::vector<int32_t> mask;
     mst::lr_masks(x->prep.P, x->cfg.compat, mask);
@@ -427,7 +485,7 @@ int upload_config(mst_ctx *x) {
         HIPCHK(hipMemcpy(x->feta32, e32.data(), sizeof(float) * x->nf, hipMemcpyHostToDevice));
         for (int i = 0; i < 4; i++) x->shadow_ok[i] = false;
     }
-    return MST_OK;
+    return upload_incidence_weights(x, eta);
 }
 
 void release(mst_ctx *x) {
@@ -435,7 +493,8 @@ void release(mst_ctx *x) {
     void *bufs[] = {x->fc0, x->fc1, x->finfo, x->ell, x->fn, x->fxf, x->pxf, x->feta, x->cvol, x->cxc,
                     x->cshalf, x->Q[0], x->Q[1], x->Q[2], x->Q[3], x->G, x->lg, x->graw, x->phi, x->R,
                     x->scal, x->part, x->fin_mid, x->bad, x->Q32[0], x->Q32[1], x->Q32[2], x->Q32[3], x->fn32, x->feta32,
-                    x->fd0, x->fd1, x->lg32};
+                    x->fd0, x->fd1, x->lg32, x->inc_code, x->inc_w, x->inc_n, x->inc_dx, x->inc_w32, x->inc_n32,
+                    x->inc_dx32};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (int k = 0; k < MST_K_COUNT; k++)
@@ -542,10 +601,12 @@ int mst_ctx_create(const mst_mesh *m, const mst_config *cfg, mst_ctx **out) {
         // the gather loops are instantiated for 3, 4 or 8 slots: pad the slot-major table with empty
         // rows up to that count so an unrolled loop over a full cell never reads past it
         const int64_t slots = x->maxf <= 3 ? 3 : (x->maxf <= 4 ? 4 : 8);
+        x->slots = (int)slots;
         std::vector<int32_t> ell(P.ell);
         ell.resize(slots * x->nc, -1);
         TRY(dupload(&x->ell, ell));
     }
+
     TRY(dupload(&x->fn, P.fn));
     TRY(dupload(&x->fxf, P.fxf));
     TRY(dupload(&x->pxf, P.pxf));
@@ -565,6 +626,7 @@ int mst_ctx_create(const mst_mesh *m, const mst_config *cfg, mst_ctx **out) {
         release(x);
         return mst::fail(MST_E_HIP, "hipMemset failed");
     }
+    TRY(build_incidence(x));
     TRY(upload_config(x));
 #undef TRY
     *out = x;
@@ -697,6 +759,38 @@ int mst_step(mst_ctx *ctx, int nsteps) {
         std::swap(ctx->iold, ctx->inew);
     }
     return check_bad(ctx);
+}
+
+int mst_run_to_steady(mst_ctx *ctx, double tol, int max_steps, int check_every, double *ref, int *steps_done,
+                      double *drop) {
+    CTX_GUARD(ctx);
+    if (!ref || !(tol > 0) || max_steps < 1 || check_every < 1) return mst::fail(MST_E_ARG, "bad arguments");
+    if (!(ctx->cfg.cfl > 0)) return mst::fail(MST_E_ARG, "steady runs advance with the CFL time step (cfl > 0)");
+    const int nv = ctx->nv;
+    double r[5];
+    int steps = 0;
+    if (!(ref[0] > 0)) {  // reference: the residual of the first step
+        int rc = mst_step(ctx, 1);
+        if (rc) return rc;
+        rc = mst_residual_norms(ctx, ref);
+        if (rc) return rc;
+        steps = 1;
+    }
+    double ratio = INFINITY;
+    while (steps < max_steps) {
+        const int k = std::min(check_every, max_steps - steps);
+        int rc = mst_step(ctx, k);
+        if (rc) return rc;
+        steps += k;
+        rc = mst_residual_norms(ctx, r);
+        if (rc) return rc;
+        ratio = 0;
+        for (int i = 0; i < nv; i++) ratio = std::max(ratio, ref[i] > 0 ? r[i] / ref[i] : 0.0);
+        if (ratio <= tol) break;
+    }
+    if (steps_done) *steps_done = steps;
+    if (drop) *drop = ratio;
+    return MST_OK;
 }
 
 int mst_compute_dt(mst_ctx *ctx, double cfl, double *dt) {
@@ -972,10 +1066,11 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
         case MST_K_FLUX:
             *bytes = flux_in + nf * 8 * NV;
             break;
-        case MST_K_GRAD:  // faces: c0, c1, normal, eta, offsets; cells: ell, state, volume, limited gradient out
-            *bytes = nf * (8 + r * D + r) + ell + nc * (r * NV + 8 + r * NV * D);
-            *bytes += mixed ? nf * 8 * D : nf * 8 * D + np * 8 * D + nc * 8 * D;
+        case MST_K_GRAD: {  // incidences: code, weight, normal, offset; cells: state, volume, limited gradient out
+            const double ninc = (double)ctx->n_inc;
+            *bytes = 4.0 * ctx->slots * nc + ninc * (r + 2 * r * D) + nc * (r * NV + 8 + r * NV * D);
             break;
+        }
         case MST_K_UPDATE:
             *bytes = upd + nf * 8 * NV;
             break;

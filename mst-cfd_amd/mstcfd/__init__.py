@@ -46,7 +46,8 @@ EXPORTS = [
     "mst_batch_set_tuning", "mst_batch_get_tuning", "mst_batch_set_graph", "mst_batch_autotune",
     "mst_mlp_create", "mst_mlp_destroy", "mst_mlp_num_params", "mst_mlp_get_params", "mst_mlp_set_params",
     "mst_mlp_set_scaling", "mst_mlp_set_data", "mst_mlp_reset_optimizer", "mst_mlp_train", "mst_mlp_forward",
-    "mst_mlp_gradient", "mst_ctx_warm_start",
+    "mst_mlp_gradient", "mst_ctx_warm_start", "mst_run_to_steady", "mst_batch_run_to_steady",
+    "mst_mesh_cell_centres",
 ]
 
 
@@ -178,6 +179,10 @@ def lib():
         "mst_mlp_forward": (C.c_int, [vp, F32P, C.c_int64, F32P]),
         "mst_mlp_gradient": (C.c_int, [vp, F32P, F32P, C.c_int64, F32P, C.POINTER(C.c_double)]),
         "mst_ctx_warm_start": (C.c_int, [vp, vp, F64P, C.c_int]),
+        "mst_run_to_steady": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, F64P, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_double)]),
+        "mst_batch_run_to_steady": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, F64P, I32P, F64P]),
+        "mst_mesh_cell_centres": (C.c_int, [vp, F64P, C.c_int64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -235,6 +240,11 @@ class Mesh:
         """Compression ramp: wall corner at x = 0, bc = (left, right, bottom, top)."""
         return cls._new(lib().mst_mesh_gen_ramp, int(nx), int(ny), float(x0), float(x1), float(height),
                         float(theta_deg), float(jitter), int(seed), np.asarray(bc, np.int32))
+
+    def cell_centres(self):
+        c = np.zeros((self.n_cells, self.dim))
+        _check(lib().mst_mesh_cell_centres(self.h, c.reshape(-1), self.n_cells), "mst_mesh_cell_centres")
+        return c
 
     def init_state(self, kind, params):
         """Initial field on the cell centres, file order (mst_mesh_init_state)."""
@@ -346,6 +356,15 @@ class Solver:
     def set_config(self, cfg):
         _check(lib().mst_ctx_set_config(self.h, C.byref(cfg)), "mst_ctx_set_config")
         self.cfg = cfg
+
+    def run_to_steady(self, tol=1e-8, max_steps=60000, check_every=200, ref=None):
+        """Steps until every field's L2(R/V) / ref <= tol; ref None: the first step's norms.
+        Returns (steps, final ratio, ref)."""
+        r = np.zeros(self.nv) if ref is None else np.ascontiguousarray(ref, np.float64).copy()
+        n, d = C.c_int(), C.c_double()
+        _check(lib().mst_run_to_steady(self.h, float(tol), int(max_steps), int(check_every), r, C.byref(n),
+                                       C.byref(d)), "mst_run_to_steady")
+        return n.value, d.value, r
 
     def warm_start(self, mlp, cond):
         """Old state := the network's decoded prediction at the cell centres (mst_ctx_warm_start)."""
@@ -518,6 +537,15 @@ class Batch:
     def set_active(self, active):
         a = np.ascontiguousarray(np.asarray(active, bool).astype(np.uint8))
         _check(lib().mst_batch_set_active(self.h, a), "mst_batch_set_active")
+
+    def run_to_steady(self, tol=1e-8, max_steps=60000, check_every=500, ref=None):
+        """Per case: steps taken, final ratio, reference norms (converged cases are frozen)."""
+        r = np.zeros((self.n_cases, self.nv)) if ref is None else np.ascontiguousarray(ref, np.float64).copy()
+        steps = np.zeros(self.n_cases, np.int32)
+        drop = np.zeros(self.n_cases)
+        _check(lib().mst_batch_run_to_steady(self.h, float(tol), int(max_steps), int(check_every), r.reshape(-1),
+                                             steps, drop), "mst_batch_run_to_steady")
+        return steps, drop, r
 
     def set_tuning(self, block, tiles):
         _check(lib().mst_batch_set_tuning(self.h, int(block), np.asarray(tiles, np.int32)), "mst_batch_set_tuning")
