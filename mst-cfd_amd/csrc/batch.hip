@@ -248,9 +248,10 @@ struct mst_batch {
     int32_t *fc0 = nullptr, *fc1 = nullptr, *finfo = nullptr, *ell = nullptr;
     double *fn = nullptr, *fxf = nullptr, *feta = nullptr, *cvol = nullptr, *cxc = nullptr, *cshalf = nullptr;
     double *Q[4] = {nullptr, nullptr, nullptr, nullptr};
-    double *G = nullptr, *lg = nullptr;
+    double *G = nullptr, *rec = nullptr;  // face fluxes; reconstructed face states (order 2)
+    int32_t *frec0 = nullptr, *frec1 = nullptr;  // incidence records of each face's c0 / c1 side
     float *Q32[4] = {nullptr, nullptr, nullptr, nullptr};
-    float *fn32 = nullptr, *feta32 = nullptr, *fd0 = nullptr, *fd1 = nullptr, *lg32 = nullptr;
+    float *rec32 = nullptr;
     int32_t *inc_code = nullptr;  // gradient incidence layout (slot-major, stride NC)
     double *inc_w = nullptr, *inc_n = nullptr, *inc_dx = nullptr;
     float *inc_w32 = nullptr, *inc_n32 = nullptr, *inc_dx32 = nullptr;
@@ -304,8 +305,8 @@ void release(mst_batch *b) {
     if (!b) return;
     drop_graphs(b);
     void *bufs[] = {b->fc0, b->fc1, b->finfo, b->ell, b->fn, b->fxf, b->feta, b->cvol, b->cxc, b->cshalf,
-                    b->Q[0], b->Q[1], b->Q[2], b->Q[3], b->G, b->lg, b->Q32[0], b->Q32[1], b->Q32[2], b->Q32[3],
-                    b->fn32, b->feta32, b->fd0, b->fd1, b->lg32, b->P64, b->P32, b->scheme, b->dts, b->gam,
+                    b->Q[0], b->Q[1], b->Q[2], b->Q[3], b->G, b->rec, b->Q32[0], b->Q32[1], b->Q32[2], b->Q32[3],
+                    b->frec0, b->frec1, b->rec32, b->P64, b->P32, b->scheme, b->dts, b->gam,
                     b->cfl, b->l2, b->ncells, b->citems, b->fitems, b->ci0, b->ci1, b->part, b->cnt, b->bad,
                     b->inc_code, b->inc_w, b->inc_n, b->inc_dx, b->inc_w32, b->inc_n32, b->inc_dx32};
     for (void *p : bufs)
@@ -370,34 +371,30 @@ BArgs<T> base_args(const mst_batch *b, int isrc) {
     g.phi = nullptr;
     FluxArgs<T> &f = a.f;
     f.nf = b->NF;
-    f.pstart = 0;
     f.fc0 = b->fc0;
     f.fc1 = b->fc1;
     f.finfo = b->finfo;
     f.fn = b->fn;
-    f.fxf = b->fxf;
-    f.pxf = nullptr;
-    f.cxc = b->cxc;
-    f.fd0 = b->fd0;
-    f.fd1 = b->fd1;
+    f.frec0 = b->frec0;
+    f.frec1 = b->frec1;
     f.G = b->G;
     if constexpr (std::is_same<T, float>::value) {
         g.inc_w = b->inc_w32;
         g.inc_n = b->inc_n32;
         g.inc_dx = b->inc_dx32;
         g.Q = b->Q32[isrc];
-        g.lg = b->lg32;
+        g.rec = b->rec32;
         f.Q = b->Q32[isrc];
-        f.lg = b->lg32;
+        f.rec = b->rec32;
         a.P = b->P32;
     } else {
         g.inc_w = b->inc_w;
         g.inc_n = b->inc_n;
         g.inc_dx = b->inc_dx;
         g.Q = b->Q[isrc];
-        g.lg = b->lg;
+        g.rec = b->rec;
         f.Q = b->Q[isrc];
-        f.lg = b->lg;
+        f.rec = b->rec;
         a.P = b->P64;
     }
     UpdArgs &u = a.u;
@@ -786,17 +783,16 @@ int mst_batch_create(int n_cases, const mst_mesh *const *meshes, const mst_confi
     b->NC = b->cstart[n_cases];
     b->NF = b->fstart[n_cases];
     b->slots = b->maxf <= 3 ? 3 : (b->maxf <= 4 ? 4 : 8);
+    if ((int64_t)b->slots * b->NC >= (int64_t)INT32_MAX) {  // incidence records are indexed in 32 bits
+        delete b;
+        return mst::fail(MST_E_UNSUPPORTED, "batch too large: cells x face slots must stay below 2^31");
+    }
     const int dim = b->dim, nv = b->nv;
     // concatenated arrays (global indices)
     std::vector<int32_t> fc0(b->NF), fc1(b->NF), finfo(b->NF), ell((size_t)b->slots * b->NC, -1), scheme(n_cases);
     std::vector<double> fn(b->NF * dim), fxf(b->NF * dim), feta(b->NF), cvol(b->NC), cxc(b->NC * dim),
         cshalf(b->NC * dim), gam(n_cases), cflv(n_cases), ncells(n_cases);
-    std::vector<float> d0, d1;
-    if (b->mixed) {
-        d0.assign(b->NF * dim, 0.f);
-        d1.assign(b->NF * dim, 0.f);
-    }
-    std::vector<int32_t> inc_code((size_t)b->slots * b->NC, -2);
+    std::vector<int32_t> inc_code((size_t)b->slots * b->NC, -2), frec0(b->NF, -1), frec1(b->NF, -1);
     std::vector<double> inc_w((size_t)b->slots * b->NC, 0.0), inc_n((size_t)b->slots * b->NC * dim, 0.0),
         inc_dx((size_t)b->slots * b->NC * dim, 0.0);
     std::vector<float> inc_w32(b->mixed ? (size_t)b->slots * b->NC : 0, 0.f);
@@ -816,10 +812,6 @@ int mst_batch_create(int n_cases, const mst_mesh *const *meshes, const mst_confi
             for (int d = 0; d < dim; d++) {
                 fn[(fs + f) * dim + d] = P.fn[f * dim + d];
                 fxf[(fs + f) * dim + d] = P.fxf[f * dim + d];
-                if (b->mixed) {
-                    d0[(fs + f) * dim + d] = (float)(P.fxf[f * dim + d] - P.cxc[P.fc0[f] * dim + d]);
-                    if (P.fc1[f] >= 0) d1[(fs + f) * dim + d] = (float)(P.fxf[f * dim + d] - P.cxc[P.fc1[f] * dim + d]);
-                }
             }
         }
         if ((c.compat & MST_COMPAT_GRAD_LASTINT) && P.quirk_face >= 0) feta[fs + P.quirk_face] = 1.0;
@@ -842,6 +834,8 @@ int mst_batch_create(int n_cases, const mst_mesh *const *meshes, const mst_confi
                 const int64_t f = mst::incidence(P, sl, k, code, nn, dd);
                 inc_code[gi] = code >= 0 ? (int32_t)((((int64_t)(code >> 1) + cs) << 1) | (code & 1)) : code;
                 if (f < 0) continue;
+                const int32_t ent = P.ell[(int64_t)sl * P.nc + k];
+                ((ent & 1) ? frec1 : frec0)[fs + f] = (int32_t)gi;
                 const double e = feta[fs + f];
                 inc_w[gi] = e;
                 for (int d = 0; d < dim; d++) {
@@ -910,15 +904,12 @@ int mst_batch_create(int n_cases, const mst_mesh *const *meshes, const mst_confi
     TRY(dalloc(&b->G, b->NF * nv));
     TRY(dalloc(&b->cnt, NCNT));
     TRY(dalloc(&b->bad, n_cases));
-    if (b->order >= 2 && !b->mixed) TRY(dalloc(&b->lg, b->NC * nv * dim));
+    TRY(dupload(&b->frec0, frec0));
+    TRY(dupload(&b->frec1, frec1));
+    if (b->order >= 2 && !b->mixed) TRY(dalloc(&b->rec, (size_t)b->slots * b->NC * nv));
     if (b->mixed) {
         for (int i = 0; i < 4; i++) TRY(dalloc(&b->Q32[i], b->NC * nv));
-        std::vector<float> fn32(fn.begin(), fn.end()), e32(feta.begin(), feta.end());
-        TRY(dupload(&b->fn32, fn32));
-        TRY(dupload(&b->feta32, e32));
-        TRY(dupload(&b->fd0, d0));
-        TRY(dupload(&b->fd1, d1));
-        if (b->order >= 2) TRY(dalloc(&b->lg32, b->NC * nv * dim));
+        if (b->order >= 2) TRY(dalloc(&b->rec32, (size_t)b->slots * b->NC * nv));
     }
     if (hipMemset(b->cnt, 0, sizeof(int) * NCNT) != hipSuccess || hipMemset(b->bad, 0, sizeof(int) * n_cases) != hipSuccess ||
         hipMemset(b->l2, 0, sizeof(double) * n_cases * nv) != hipSuccess ||

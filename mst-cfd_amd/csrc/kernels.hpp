@@ -37,81 +37,66 @@ namespace mstk {
 // ------------------------------------------------------------------------------------ face flux
 template <typename T>
 struct FluxArgs {
-    int64_t nf, pstart;
+    int64_t nf;
     const int32_t *__restrict__ fc0;
     const int32_t *__restrict__ fc1;
     const int32_t *__restrict__ finfo;  // kind | (L/R mask << 4)
     const double *__restrict__ fn;      // c0 outward normal
-    // FP64: face centre, partner face centre (periodic) and cell centres, as the reference
-    const double *__restrict__ fxf;
-    const double *__restrict__ pxf;
-    const double *__restrict__ cxc;
-    // FP32: offsets x_f - x_c0 and x_f' - x_c1, formed in FP64 on the host
-    const float *__restrict__ fd0;
-    const float *__restrict__ fd1;
-    const T *__restrict__ Q;
-    const T *__restrict__ lg;
+    const T *__restrict__ Q;            // cell states
+    // order 2: the gradient kernel's reconstructed face states (NV values per incidence record) and
+    // the records of each face's c0 and c1 sides (-1: none)
+    const T *__restrict__ rec;
+    const int32_t *__restrict__ frec0;
+    const int32_t *__restrict__ frec1;
     double *__restrict__ G;
     Params<T> P;
 };
 
-// Riemann left (A: c0 side) and right (B: c1 side or ghost) states of face f: 2nd-order reconstruction
-// with the limited gradients, ghost states of the boundary faces (RhoSolver::updateFaceFlux,
-// RhoSolver.cpp:90-233, and updateFaceFlux2ndOrder, :234-369).  Every input is read from memory by
-// face index, so the lanes of the two cells of a face (cell strategy) compute bit-identical states.
+// Riemann left (A: c0 side) and right (B: c1 side or ghost) states of face f (RhoSolver::updateFaceFlux,
+// RhoSolver.cpp:90-233, and updateFaceFlux2ndOrder, :234-369): order 1 the cell states, order 2 the
+// reconstructed states the gradient kernel stored for the face's two incidences, and the ghost states
+// of the boundary faces, which combine the raw and the reconstructed interior state as the reference
+// does.  Every input is read from memory by face index, so the lanes of the two cells of a face (cell
+// strategy) compute bit-identical states.
 template <int DIM, bool SECOND, typename T>
 __device__ __forceinline__ void face_states(const FluxArgs<T> &a, int64_t f, int kind, const T *n, T (&A)[DIM + 2],
                                             T (&B)[DIM + 2]) {
     constexpr int NV = DIM + 2;
-    constexpr bool OFFS = std::is_same<T, float>::value;
-    const int64_t c0 = a.fc0[f];
-    ldq<NV>(a.Q + c0 * NV, A);
-    T dx0[DIM];
-    if constexpr (SECOND) {
-#pragma unroll
-        for (int d = 0; d < DIM; d++)
-            dx0[d] = OFFS ? (T)a.fd0[f * DIM + d] : (T)(a.fxf[f * DIM + d] - a.cxc[c0 * DIM + d]);
-    }
     if (kind == mst::KIND_INTERIOR || kind == mst::KIND_PERIODIC) {
-        const int64_t c1 = a.fc1[f];
-        ldq<NV>(a.Q + c1 * NV, B);
         if constexpr (SECOND) {
-            T dx1[DIM];
-#pragma unroll
-            for (int d = 0; d < DIM; d++) {
-                if constexpr (OFFS) {
-                    dx1[d] = (T)a.fd1[f * DIM + d];
-                } else {
-                    const double xf = kind == mst::KIND_PERIODIC ? a.pxf[(f - a.pstart) * DIM + d] : a.fxf[f * DIM + d];
-                    dx1[d] = (T)(xf - a.cxc[c1 * DIM + d]);
-                }
-            }
-            recon<DIM>(A, a.lg + c0 * NV * DIM, dx0);
-            recon<DIM>(B, a.lg + c1 * NV * DIM, dx1);
+            ldq<NV>(a.rec + (int64_t)a.frec0[f] * NV, A);
+            ldq<NV>(a.rec + (int64_t)a.frec1[f] * NV, B);
+        } else {
+            ldq<NV>(a.Q + (int64_t)a.fc0[f] * NV, A);
+            ldq<NV>(a.Q + (int64_t)a.fc1[f] * NV, B);
         }
+        return;
+    }
+    ldq<NV>(a.Q + (int64_t)a.fc0[f] * NV, A);
+    T Rc[NV];
+    if constexpr (SECOND) {
+        ldq<NV>(a.rec + (int64_t)a.frec0[f] * NV, Rc);
     } else {
-        T Rc[NV];
 #pragma unroll
         for (int k = 0; k < NV; k++) Rc[k] = A[k];
-        if constexpr (SECOND) recon<DIM>(Rc, a.lg + c0 * NV * DIM, dx0);
-        if (kind == mst::KIND_WALL) {  // RhoSolver.cpp:142-179 / 285-319
+    }
+    if (kind == mst::KIND_WALL) {  // RhoSolver.cpp:142-179 / 285-319
 #pragma unroll
-            for (int k = 0; k < NV; k++) A[k] = B[k] = Rc[k];
-            reflect<DIM>(B, A, n);
-        } else if (kind == mst::KIND_SYMMETRY) {  // :184-211 / 322-350
+        for (int k = 0; k < NV; k++) A[k] = B[k] = Rc[k];
+        reflect<DIM>(B, A, n);
+    } else if (kind == mst::KIND_SYMMETRY) {  // :184-211 / 322-350
 #pragma unroll
-            for (int k = 0; k < NV; k++) B[k] = Rc[k];
-            reflect<DIM>(B, A, n);
-        } else if (kind == mst::KIND_INLET) {  // :121-137 / 265-281
+        for (int k = 0; k < NV; k++) B[k] = Rc[k];
+        reflect<DIM>(B, A, n);
+    } else if (kind == mst::KIND_INLET) {  // :121-137 / 265-281
 #pragma unroll
-            for (int k = 0; k < NV; k++) {
-                A[k] = Rc[k];
-                B[k] = a.P.qin[k];
-            }
-        } else {  // outlet :216-226 / 353-363
-#pragma unroll
-            for (int k = 0; k < NV; k++) B[k] = Rc[k];
+        for (int k = 0; k < NV; k++) {
+            A[k] = Rc[k];
+            B[k] = a.P.qin[k];
         }
+    } else {  // outlet :216-226 / 353-363
+#pragma unroll
+        for (int k = 0; k < NV; k++) B[k] = Rc[k];
     }
 }
 
@@ -162,7 +147,7 @@ struct GradArgs {
     const T *__restrict__ inc_dx;
     const T *__restrict__ Q;
     const double *__restrict__ vol;
-    T *__restrict__ lg;
+    T *__restrict__ rec;  // out: reconstructed face state of every incidence, Q_c + phi grad Q . dx
     double *__restrict__ graw;
     double *__restrict__ phi;
 };
@@ -268,13 +253,22 @@ __device__ __forceinline__ void grad_cell(const GradArgs<T> &a, const int64_t c)
                 ph[k] = fmin(T(1), fdiv(T(2) * bn[k], bd[k] + bn[k]));  // (r+|r|)/(1+|r|) at r = bn/bd >= 0
         }
     }
-    T out[NV * DIM];
+    T lgr[NV * DIM];  // limited gradient
 #pragma unroll
     for (int k = 0; k < NV; k++)
 #pragma unroll
-        for (int d = 0; d < DIM; d++) out[k * DIM + d] = ph[k] * t[k][d];
+        for (int d = 0; d < DIM; d++) lgr[k * DIM + d] = ph[k] * t[k][d];
+    // reconstructed states at the cell's faces (RhoSolver.cpp:250,254 with the limiter), one record
+    // per incidence, coalesced over the cells of a slot
 #pragma unroll
-    for (int i = 0; i < NV * DIM; i++) a.lg[c * NV * DIM + i] = out[i];
+    for (int s = 0; s < MAXF; s++) {
+        if (s >= nfc) break;
+        T q[NV];
+#pragma unroll
+        for (int k = 0; k < NV; k++) q[k] = qc[k];
+        recon<DIM>(q, lgr, dxs[s]);
+        stq<NV>(a.rec + ((int64_t)s * a.nc + c) * NV, q);
+    }
     if (a.graw) {
 #pragma unroll
         for (int k = 0; k < NV; k++)

@@ -62,11 +62,12 @@ struct mst_ctx {
     double *cvol = nullptr, *cxc = nullptr, *cshalf = nullptr;
     double *Q[4] = {nullptr, nullptr, nullptr, nullptr};
     int iold = 0, inew = 1;
-    double *G = nullptr, *lg = nullptr, *graw = nullptr, *phi = nullptr, *R = nullptr;
+    double *G = nullptr, *rec = nullptr, *graw = nullptr, *phi = nullptr, *R = nullptr;
+    int32_t *frec0 = nullptr, *frec1 = nullptr;  // incidence records of each face's c0 / c1 side
     // mixed form: FP32 shadows of the states, FP32 geometry, offsets, face fluxes, limited gradients
     float *Q32[4] = {nullptr, nullptr, nullptr, nullptr};
     bool shadow_ok[4] = {false, false, false, false};
-    float *fn32 = nullptr, *feta32 = nullptr, *fd0 = nullptr, *fd1 = nullptr, *lg32 = nullptr;
+    float *rec32 = nullptr;  // FP32 reconstructed face states (order 2)
     // incidence layout of the gradient kernel (slot-major over the padded ELL slots)
     int slots = 3;
     int64_t n_inc = 0;  // non-empty incidences
@@ -134,13 +135,13 @@ int run_grad_t(mst_ctx *x, int isrc, bool debug) {
         a.inc_n = x->inc_n32;
         a.inc_dx = x->inc_dx32;
         a.Q = x->Q32[isrc];
-        a.lg = x->lg32;
+        a.rec = x->rec32;
     } else {
         a.inc_w = x->inc_w;
         a.inc_n = x->inc_n;
         a.inc_dx = x->inc_dx;
         a.Q = x->Q[isrc];
-        a.lg = x->lg;
+        a.rec = x->rec;
     }
     a.vol = x->cvol;
     a.graw = debug ? x->graw : nullptr;
@@ -167,24 +168,20 @@ template <typename T>
 FluxArgs<T> flux_args(const mst_ctx *x, int isrc) {
     FluxArgs<T> a;
     a.nf = x->nf;
-    a.pstart = x->prep.P.pstart;
     a.fc0 = x->fc0;
     a.fc1 = x->fc1;
     a.finfo = x->finfo;
-    a.fxf = x->fxf;
-    a.pxf = x->pxf;
-    a.cxc = x->cxc;
-    a.fd0 = x->fd0;
-    a.fd1 = x->fd1;
     a.fn = x->fn;
+    a.frec0 = x->frec0;
+    a.frec1 = x->frec1;
     a.G = x->G;
     if constexpr (std::is_same<T, float>::value) {
         a.Q = x->Q32[isrc];
-        a.lg = x->lg32;
+        a.rec = x->rec32;
         a.P = x->p32;
     } else {
         a.Q = x->Q[isrc];
-        a.lg = x->lg;
+        a.rec = x->rec;
         a.P = x->p64;
     }
     return a;
@@ -378,34 +375,13 @@ int dupload(T **p, const std::vector<T> &v) {
 
 // device arrays of the mixed form (allocated the first time a mixed configuration is set)
 int alloc_mixed(mst_ctx *x) {
-    if (x->lg32) return MST_OK;
-    const mst::Prepared &P = x->prep.P;
-    const int dim = x->dim;
+    if (x->rec32) return MST_OK;
     for (int i = 0; i < 4; i++) {
         int rc = dalloc(&x->Q32[i], x->nc * x->nv);
         if (rc) return rc;
         x->shadow_ok[i] = false;
     }
-    std::vector<float> fn32(P.fn.begin(), P.fn.end());
-    std::vector<float> d0(x->nf * dim, 0.f), d1(x->nf * dim, 0.f);
-    for (int64_t i = 0; i < x->nf; i++) {
-        const int64_t a = P.fc0[i], b = P.fc1[i];
-        for (int d = 0; d < dim; d++) {
-            d0[i * dim + d] = (float)(P.fxf[i * dim + d] - P.cxc[a * dim + d]);
-            if (b >= 0) {
-                const double xf = P.fkind[i] == mst::KIND_PERIODIC ? P.pxf[(i - P.pstart) * dim + d] : P.fxf[i * dim + d];
-                d1[i * dim + d] = (float)(xf - P.cxc[b * dim + d]);
-            }
-        }
-    }
-    int rc = dupload(&x->fn32, fn32);
-    if (!rc) rc = dupload(&x->fd0, d0);
-    if (!rc) rc = dupload(&x->fd1, d1);
-    if (!rc) rc = dalloc(&x->feta32, x->nf);
-    if (!rc) rc = dalloc(&x->lg32, x->nc * x->nv * dim);
-    if (!rc && hipMemset(x->lg32, 0, sizeof(float) * x->nc * x->nv * dim) != hipSuccess)
-        rc = mst::fail(MST_E_HIP, "hipMemset failed");
-    return rc;
+    return dalloc(&x->rec32, (int64_t)x->slots * x->nc * x->nv);
 }
 
 // incidence arrays from the padded ELL table: code, signed normal, offset (FP64, and FP32 copies
@@ -421,10 +397,20 @@ int build_incidence(mst_ctx *x) {
             const int64_t i = (int64_t)s * x->nc + c;
             if (mst::incidence(P, s, c, code[i], &nrm[i * dim], &dx[i * dim]) >= 0) x->n_inc++;
         }
+    // records of each face's sides: the incidence (s, c) whose table entry is the face
+    std::vector<int32_t> r0(x->nf, -1), r1(x->nf, -1);
+    for (int s = 0; s < P.maxf; s++)
+        for (int64_t c = 0; c < x->nc; c++) {
+            const int32_t e = P.ell[(int64_t)s * x->nc + c];
+            if (e < 0) continue;
+            ((e & 1) ? r1 : r0)[e >> 1] = (int32_t)((int64_t)s * x->nc + c);
+        }
     int rc = dupload(&x->inc_code, code);
     if (!rc) rc = dupload(&x->inc_n, nrm);
     if (!rc) rc = dupload(&x->inc_dx, dx);
     if (!rc) rc = dalloc(&x->inc_w, n);
+    if (!rc) rc = dupload(&x->frec0, r0);
+    if (!rc) rc = dupload(&x->frec1, r1);
     return rc;
 }
 
@@ -481,8 +467,6 @@ int upload_config(mst_ctx *x) {
     if (x->mixed) {
         int rc = alloc_mixed(x);
         if (rc) return rc;
-        std::vector<float> e32(eta.begin(), eta.end());
-        HIPCHK(hipMemcpy(x->feta32, e32.data(), sizeof(float) * x->nf, hipMemcpyHostToDevice));
         for (int i = 0; i < 4; i++) x->shadow_ok[i] = false;
     }
     return upload_incidence_weights(x, eta);
@@ -491,9 +475,9 @@ int upload_config(mst_ctx *x) {
 void release(mst_ctx *x) {
     if (!x) return;
     void *bufs[] = {x->fc0, x->fc1, x->finfo, x->ell, x->fn, x->fxf, x->pxf, x->feta, x->cvol, x->cxc,
-                    x->cshalf, x->Q[0], x->Q[1], x->Q[2], x->Q[3], x->G, x->lg, x->graw, x->phi, x->R,
-                    x->scal, x->part, x->fin_mid, x->bad, x->Q32[0], x->Q32[1], x->Q32[2], x->Q32[3], x->fn32, x->feta32,
-                    x->fd0, x->fd1, x->lg32, x->inc_code, x->inc_w, x->inc_n, x->inc_dx, x->inc_w32, x->inc_n32,
+                    x->cshalf, x->Q[0], x->Q[1], x->Q[2], x->Q[3], x->G, x->rec, x->graw, x->phi, x->R,
+                    x->scal, x->part, x->fin_mid, x->bad, x->Q32[0], x->Q32[1], x->Q32[2], x->Q32[3], x->rec32,
+                    x->frec0, x->frec1, x->inc_code, x->inc_w, x->inc_n, x->inc_dx, x->inc_w32, x->inc_n32,
                     x->inc_dx32};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -602,6 +586,10 @@ int mst_ctx_create(const mst_mesh *m, const mst_config *cfg, mst_ctx **out) {
         // rows up to that count so an unrolled loop over a full cell never reads past it
         const int64_t slots = x->maxf <= 3 ? 3 : (x->maxf <= 4 ? 4 : 8);
         x->slots = (int)slots;
+        if (slots * x->nc >= (int64_t)INT32_MAX) {  // incidence records are indexed in 32 bits
+            release(x);
+            return mst::fail(MST_E_UNSUPPORTED, "mesh too large: cells x face slots must stay below 2^31");
+        }
         std::vector<int32_t> ell(P.ell);
         ell.resize(slots * x->nc, -1);
         TRY(dupload(&x->ell, ell));
@@ -616,13 +604,13 @@ int mst_ctx_create(const mst_mesh *m, const mst_config *cfg, mst_ctx **out) {
     TRY(dupload(&x->cshalf, P.cshalf));
     for (int i = 0; i < 4; i++) TRY(dalloc(&x->Q[i], x->nc * x->nv));
     TRY(dalloc(&x->G, x->nf * x->nv));
-    TRY(dalloc(&x->lg, x->nc * x->nv * x->dim));
+    TRY(dalloc(&x->rec, (int64_t)x->slots * x->nc * x->nv));
     TRY(dalloc(&x->scal, 16));
     TRY(dalloc(&x->part, std::max(x->nblk_c, x->nblk_f) * (x->nv + 1) + 16));
     TRY(dalloc(&x->fin_mid, 12 * (x->nblk_c / FIN_CHUNK + 2)));
     TRY(dalloc(&x->bad, 1));
     if (hipMemset(x->bad, 0, sizeof(int)) != hipSuccess || hipMemset(x->scal, 0, 16 * sizeof(double)) != hipSuccess ||
-        hipMemset(x->lg, 0, sizeof(double) * x->nc * x->nv * x->dim) != hipSuccess) {
+        hipMemset(x->rec, 0, sizeof(double) * x->slots * x->nc * x->nv) != hipSuccess) {
         release(x);
         return mst::fail(MST_E_HIP, "hipMemset failed");
     }
@@ -1055,8 +1043,10 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
     const double stages = ctx->cfg.rk;
     // faces: c0, c1, info, FP64 normal; cells: the arithmetic state; order 2: face-cell offsets
     // (FP64: face centres, partner centres and cell centres) and limited gradients
+    // order 2: the side records (each read once), the record indices, the raw c0 state of boundary faces
+    const double ninc = (double)ctx->n_inc, nbnd = (double)(ctx->nf - ctx->prep.P.n_interior);
     double flux_in = nf * (12 + 8 * D) + nc * r * NV;
-    if (second) flux_in += mixed ? nf * 8 * D + nc * r * NV * D : nf * 8 * D + np * 8 * D + nc * (8 * D + 8 * NV * D);
+    if (second) flux_in = nf * (12 + 8 * D + 8) + ninc * r * NV + nbnd * r * NV;
     // update: cell faces, stage state, volume, new state (+ FP32 shadow), Q^n on stages after the
     // first, the CFL half-sums on the last stage when the time step is reduced there
     double upd = ell + nc * (8 * NV + 8 + 8 * NV) + nc * 8 * NV * (stages - 1) / stages;
@@ -1066,11 +1056,9 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
         case MST_K_FLUX:
             *bytes = flux_in + nf * 8 * NV;
             break;
-        case MST_K_GRAD: {  // incidences: code, weight, normal, offset; cells: state, volume, limited gradient out
-            const double ninc = (double)ctx->n_inc;
-            *bytes = 4.0 * ctx->slots * nc + ninc * (r + 2 * r * D) + nc * (r * NV + 8 + r * NV * D);
+        case MST_K_GRAD:  // incidences: code, weight, normal, offset, record out; cells: state, volume
+            *bytes = 4.0 * ctx->slots * nc + ninc * (r + 2 * r * D + r * NV) + nc * (r * NV + 8);
             break;
-        }
         case MST_K_UPDATE:
             *bytes = upd + nf * 8 * NV;
             break;
