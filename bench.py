@@ -44,7 +44,9 @@ def build_workload(M, name):
 
 # HBM bytes per launch of a workload's dominant kernel from the committed rocprofv3 counter passes
 # (FETCH_SIZE and WRITE_SIZE scaled by a plain copy kernel of known size); None where not profiled
-TRAFFIC_PROFILES = {("vortex2d", "face", "k_grad"): ("profiles/hbm_traffic_vortex2d.md", "k_grad<2, 3, 2, double>")}
+TRAFFIC_PROFILES = {("vortex2d", "face", "k_grad"): ("profiles/hbm_traffic_vortex2d.md", "k_grad<2, 3, 2, double>"),
+                    ("vortex2d", "cell", "k_cell"): ("profiles/hbm_traffic_vortex2d.md", "k_cell<2, 0, true, double>"),
+                    ("vortex2d", "cell", "k_grad"): ("profiles/hbm_traffic_vortex2d.md", "k_grad<2, 3, 2, double>")}
 
 
 def committed_traffic(workload, strategy, kernel):
@@ -219,8 +221,9 @@ def main():
     ap.add_argument("--workload", default="vortex2d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--retune", action="store_true", help="batch workload: ignore the tuning cache")
-    ap.add_argument("--strategy", choices=("face", "cell"), default="face",
-                    help="face: flux kernel + gather/update kernel; cell: fused cell kernel")
+    ap.add_argument("--strategy", choices=("auto", "face", "cell"), default="auto",
+                    help="face: flux kernel + gather/update kernel; cell: fused cell kernel; auto: time both "
+                         "and report the faster")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison")
     ap.add_argument("--compare-steps", type=int, default=200,
                     help="steps of the FP64 vs mixed state comparison")
@@ -245,26 +248,39 @@ def main():
         copy = dict(bytes=args.calibrate_copy, ms=ms, gbs=2 * args.calibrate_copy / (ms * 1e-3) / 1e9)
     mesh, cfg_kw, desc, init = build_workload(M, args.workload)
     solver = M.Solver(mesh, M.make_config(**cfg_kw))
-    solver.strategy = M.STRATEGY_CELL if args.strategy == "cell" else M.STRATEGY_FACE
-    desc = dict(desc, strategy=args.strategy)
     Q0 = init()
-    solver.set_state(Q0)
-    wall, kern = measure(M, solver, args.steps, args.warmup)
     stages = cfg_kw["rk"]
+    names = {"face": M.STRATEGY_FACE, "cell": M.STRATEGY_CELL}
+    tried = ("face", "cell") if args.strategy == "auto" else (args.strategy,)
+    runs = {}
+    for st in tried:  # each strategy on the same context from the same initial field
+        solver.strategy = names[st]
+        solver.set_state(Q0)
+        wall, kern = measure(M, solver, args.steps, args.warmup)
+        dom = max(kern, key=lambda k: kern[k]["total_ms"])
+        runs[st] = dict(wall=wall, kernels=kern, dominant=dom)
+    best = max(runs, key=lambda st: 1.0 / runs[st]["wall"])
+    wall, kern, dom = runs[best]["wall"], runs[best]["kernels"], runs[best]["dominant"]
+    solver.strategy = names[best]
     value = mesh.n_faces * stages * args.steps / wall
-    dom = max(kern, key=lambda k: kern[k]["total_ms"])
-    d = kern[dom]
-    achieved = d["alg_bytes"] / (d["avg_us"] * 1e-6) / 1e9
-    traffic, traffic_src = committed_traffic(args.workload, args.strategy, dom)
+    desc = dict(desc, strategy=best)
+
+    def roofline(st):
+        r = runs[st]
+        d = r["kernels"][r["dominant"]]
+        ach = d["alg_bytes"] / (d["avg_us"] * 1e-6) / 1e9
+        traffic, src = committed_traffic(args.workload, st, r["dominant"])
+        return dict(bound="hbm", kernel=r["dominant"], achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=ach / HBM_PEAK_GBS, traffic=traffic, traffic_source=src, alg_bytes=d["alg_bytes"])
+
     out = dict(
         metric="face-flux updates/s (FP64 residual path); %HBM peak of the dominant kernel",
         value=value, unit="face-updates/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
         ms_per_step=1e3 * wall / args.steps, higher_is_better=True, scaling="weak", vs_baseline=None,
-        dtype="f64", data="synthetic", config=desc,
-        roofline=dict(bound="hbm", kernel=dom, achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=achieved / HBM_PEAK_GBS, traffic=traffic, traffic_source=traffic_src,
-                      alg_bytes=d["alg_bytes"]),
-        kernels=kern,
+        dtype="f64", data="synthetic", config=desc, roofline=roofline(best), kernels=kern,
+        strategies={st: dict(value=mesh.n_faces * stages * args.steps / runs[st]["wall"],
+                             ms_per_step=1e3 * runs[st]["wall"] / args.steps, roofline=roofline(st),
+                             kernels=runs[st]["kernels"]) for st in runs},
     )
     if copy:
         out["copy_probe"] = copy

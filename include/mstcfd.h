@@ -183,10 +183,13 @@ int mst_step(mst_ctx *ctx, int nsteps);
 /* NEW: kernel strategy of mst_step and mst_solve.  FACE: a face kernel writes every face flux once,
  * a cell kernel gathers them and updates (the reference's updateFaceFlux + solve split).  CELL: one
  * fused cell kernel solves each cell's own faces (every interior face twice, bitwise the same flux
- * on both sides) and updates, with no face-flux array in memory.  Results agree to rounding; the
- * default is FACE.  mst_face_flux / mst_residual_update keep the face split whatever the strategy. */
+ * on both sides) and updates, with no face-flux array in memory.  Both give the same result bit for
+ * bit.  AUTO (default): the first stage after creation or a configuration change times one stage of
+ * each on the context's own mesh and keeps the faster; mst_get_strategy then reports it.
+ * mst_face_flux / mst_residual_update keep the face split whatever the strategy. */
 #define MST_STRATEGY_FACE 0
 #define MST_STRATEGY_CELL 1
+#define MST_STRATEGY_AUTO 2
 int mst_set_strategy(mst_ctx *ctx, int strategy);
 int mst_get_strategy(const mst_ctx *ctx, int *strategy);
 /* NEW: steady driver.  Steps of mst_step until max_f L2(R/V)_f / ref_f <= tol (checked every
@@ -249,9 +252,13 @@ int mst_batch_run_to_steady(mst_batch *b, double tol, int max_steps, int check_e
 /* launch shape: block size (64, 128, 256) and cell tile per case-size bucket (faces: scaled by nf/nc) */
 int mst_batch_set_tuning(mst_batch *b, int block, const int32_t *tiles);
 int mst_batch_get_tuning(const mst_batch *b, int *block, int32_t *tiles);
+/* kernel strategy of the batched step (MST_STRATEGY_FACE or MST_STRATEGY_CELL, default CELL; the
+ * autotuner searches both); the results are the same bit for bit */
+int mst_batch_set_strategy(mst_batch *b, int strategy);
+int mst_batch_get_strategy(const mst_batch *b, int *strategy);
 /* 1: replay the captured step graph (default); 0: launch the kernels one by one */
 int mst_batch_set_graph(mst_batch *b, int on);
-/* measured search over block size and per-bucket tiles (steps timed per trial); the best shape is
+/* measured search over strategy, block size and per-bucket tiles (steps timed per trial); the best shape is
  * applied and stored in the JSON file cache_path under a key describing the batch; a cached entry
  * is applied without measuring unless force.  The state is restored afterwards. */
 int mst_batch_autotune(mst_batch *b, const char *cache_path, int force, int steps, double *ms_per_step);

@@ -165,6 +165,66 @@ __global__ __launch_bounds__(NT) void kb_update(const BArgs<T> a) {
     }
 }
 
+// cell strategy: each lane solves its cell's faces (scheme per case), sums and updates the stage
+template <int DIM, bool SECOND, typename T, int NT>
+__global__ __launch_bounds__(NT) void kb_cell(const BArgs<T> a) {
+    constexpr int NV = DIM + 2;
+    constexpr bool F32 = std::is_same<T, float>::value;
+    reset_others(a.cnt, a.own);
+    const bool L2 = a.u.l2 != 0, DT = a.u.dtred != 0;
+    for (;;) {
+        const int it = pull(a.cnt, a.own);
+        if (it >= a.n_items) break;
+        const Item I = a.items[it];
+        FluxArgs<T> fa = a.f;
+        fa.P = a.P[I.cs];
+        const bool roe = a.scheme[I.cs] == MST_SCHEME_ROE;
+        UpdArgs ua = a.u;
+        ua.bad = a.u.bad + I.cs;
+        ua.dt = a.dts + I.cs;
+        ua.gamma = a.gam[I.cs];
+        ua.cfl = a.cfl[I.cs];
+        double red[NV + 1];
+#pragma unroll
+        for (int k = 0; k < NV; k++) red[k] = 0;
+        red[NV] = INFINITY;
+        for (int64_t c = I.begin + (int)threadIdx.x; c < I.end; c += NT) {
+            double r[NV], cmp[NV];
+#pragma unroll
+            for (int k = 0; k < NV; k++) r[k] = cmp[k] = 0;
+            for (int sl = 0; sl < ua.maxf; sl++) {
+                const int32_t e = ua.ell[(int64_t)sl * ua.nc + c];
+                if (e < 0) break;
+                const int64_t f = e >> 1;
+                const int info = fa.finfo[f];
+                const int kind = info & 15;
+                if (kind >= mst::KIND_NONE) continue;
+                double g[NV];
+                if (roe)
+                    face_flux<DIM, MST_SCHEME_ROE, SECOND, T>(fa, f, kind, info >> 4, g);
+                else
+                    face_flux<DIM, MST_SCHEME_AUSM, SECOND, T>(fa, f, kind, info >> 4, g);
+                accumulate<NV, F32>(r, cmp, g, e & 1);
+            }
+            if constexpr (F32) {
+#pragma unroll
+                for (int k = 0; k < NV; k++) r[k] += cmp[k];
+            }
+            stage_cell<NV, F32>(ua, c, r, red, L2, DT);
+        }
+        if (L2) {
+            double v[NV];
+#pragma unroll
+            for (int k = 0; k < NV; k++) v[k] = red[k];
+            block_partial<NV, false, NT>(v, a.part_l2 + it, a.n_items);
+        }
+        if (DT) {
+            double v[1] = {red[NV]};
+            block_partial<1, true, NT>(v, a.part_dt + it, 1);
+        }
+    }
+}
+
 template <int DIM, typename T, int NT>
 __global__ __launch_bounds__(NT) void kb_cfl(const BArgs<T> a, const double *__restrict__ Q) {
     constexpr int NV = DIM + 2;
@@ -186,6 +246,7 @@ __global__ __launch_bounds__(NT) void kb_cfl(const BArgs<T> a, const double *__r
 
 // one block per case: its items' partials in item order -> L2 norms of R/V and the minimum time step
 struct BFin {
+    int *cnt;                  // work counters, zeroed here (the finish ends every step)
     const int32_t *ci0, *ci1;  // cell-item range of each case (empty: inactive)
     const double *part_l2, *part_dt;
     const double *ncells;
@@ -194,6 +255,7 @@ struct BFin {
 };
 
 __global__ __launch_bounds__(FIN_B) void kb_finish(const BFin a) {
+    if (blockIdx.x == 0 && (int)threadIdx.x < NCNT) a.cnt[threadIdx.x] = 0;
     const int cs = blockIdx.x;
     const int i0 = a.ci0[cs], i1 = a.ci1[cs];
     if (i0 >= i1) return;
@@ -270,6 +332,7 @@ struct mst_batch {
     bool shadow_ok[4] = {false, false, false, false};
     int block = 256;
     int tiles[MST_BATCH_BUCKETS] = {1024, 2048, 4096, 8192};
+    int strategy = MST_STRATEGY_CELL;  // FACE: flux + gather/update kernels; CELL: fused cell kernel
     std::vector<uint8_t> active;
     bool use_graph = true;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -483,8 +546,32 @@ int run_update(mst_batch *b, int is, int in, double ca, double cb, int stage0, i
     return launch_persistent(b, kb_update<NV, 8, T, NT>, NT, a.n_items, a);
 }
 
+// stage s of a step pulls from counter UPD (even s) or FLUX (odd s): without a gradient kernel between
+// them, consecutive cell kernels must not share a counter (each zeroes only the others at its start)
+template <int DIM, typename T, int NT>
+int run_cell(mst_batch *b, int stage, int is, int in, double ca, double cb, int stage0, int iout, bool l2, bool dt,
+             bool check) {
+    BArgs<T> a = base_args<T>(b, is);
+    a.items = b->citems;
+    a.n_items = b->n_citems;
+    a.own = (stage & 1) ? CNT_FLUX : CNT_UPD;
+    a.u.Qs = b->Q[is];
+    a.u.Qn = b->Q[in];
+    a.u.Qout = b->Q[iout];
+    a.u.Qout32 = b->mixed ? b->Q32[iout] : nullptr;
+    a.u.ca = ca;
+    a.u.cb = cb;
+    a.u.stage0 = stage0;
+    a.u.l2 = l2;
+    a.u.dtred = dt;
+    a.u.check = check;
+    if (b->order >= 2) return launch_persistent(b, kb_cell<DIM, true, T, NT>, NT, a.n_items, a);
+    return launch_persistent(b, kb_cell<DIM, false, T, NT>, NT, a.n_items, a);
+}
+
 int run_finish(mst_batch *b, bool l2, bool dt) {
     BFin f;
+    f.cnt = b->cnt;
     f.ci0 = b->ci0;
     f.ci1 = b->ci1;
     f.part_l2 = b->part;
@@ -530,10 +617,15 @@ int enqueue_step_t(mst_batch *b, int iold, int inew) {
             rc = run_grad<DIM, T, NT>(b, src);
             if (rc) return rc;
         }
-        rc = run_flux<DIM, T, NT>(b, src);
-        if (rc) return rc;
-        rc = run_update<DIM, T, NT>(b, src, iold, A[s], B[s], s == 0 ? 1 : 0, dst, s == 0, s == ns - 1, s == ns - 1);
-        if (rc) return rc;
+        if (b->strategy == MST_STRATEGY_CELL) {
+            rc = run_cell<DIM, T, NT>(b, s, src, iold, A[s], B[s], s == 0 ? 1 : 0, dst, s == 0, s == ns - 1, s == ns - 1);
+            if (rc) return rc;
+        } else {
+            rc = run_flux<DIM, T, NT>(b, src);
+            if (rc) return rc;
+            rc = run_update<DIM, T, NT>(b, src, iold, A[s], B[s], s == 0 ? 1 : 0, dst, s == 0, s == ns - 1, s == ns - 1);
+            if (rc) return rc;
+        }
         src = dst;
     }
     return run_finish(b, true, true);
@@ -646,8 +738,9 @@ std::string signature(const mst_batch *b) {
     return s.str();
 }
 
-// cache file: one JSON object, one entry per line  "key": {"block": B, "tiles": [t0, t1, t2, t3], "ms": x}
-bool cache_lookup(const std::string &path, const std::string &key, int *block, int *tiles, double *ms) {
+// cache file: one JSON object, one entry per line
+//   "key": {"block": B, "tiles": [t0, t1, t2, t3], "strategy": S, "ms": x}
+bool cache_lookup(const std::string &path, const std::string &key, int *block, int *tiles, int *strategy, double *ms) {
     std::ifstream in(path);
     if (!in) return false;
     std::string line, want = "\"" + key + "\"";
@@ -655,14 +748,15 @@ bool cache_lookup(const std::string &path, const std::string &key, int *block, i
         if (line.find(want) == std::string::npos) continue;
         const char *p = strstr(line.c_str(), "{\"block\":");
         if (!p) continue;
-        if (sscanf(p, "{\"block\": %d, \"tiles\": [%d, %d, %d, %d], \"ms\": %lf}", block, &tiles[0], &tiles[1],
-                   &tiles[2], &tiles[3], ms) == 6)
+        if (sscanf(p, "{\"block\": %d, \"tiles\": [%d, %d, %d, %d], \"strategy\": %d, \"ms\": %lf}", block,
+                   &tiles[0], &tiles[1], &tiles[2], &tiles[3], strategy, ms) == 7)
             return true;
     }
     return false;
 }
 
-void cache_store(const std::string &path, const std::string &key, int block, const int *tiles, double ms) {
+void cache_store(const std::string &path, const std::string &key, int block, const int *tiles, int strategy,
+                 double ms) {
     std::vector<std::string> lines;
     {
         std::ifstream in(path);
@@ -674,8 +768,8 @@ void cache_store(const std::string &path, const std::string &key, int block, con
             }
     }
     char buf[256];
-    snprintf(buf, sizeof(buf), "  \"%s\": {\"block\": %d, \"tiles\": [%d, %d, %d, %d], \"ms\": %.6f}", key.c_str(),
-             block, tiles[0], tiles[1], tiles[2], tiles[3], ms);
+    snprintf(buf, sizeof(buf), "  \"%s\": {\"block\": %d, \"tiles\": [%d, %d, %d, %d], \"strategy\": %d, \"ms\": %.6f}",
+             key.c_str(), block, tiles[0], tiles[1], tiles[2], tiles[3], strategy, ms);
     lines.push_back(buf);
     std::ofstream out(path, std::ios::trunc);
     out << "{\n";
@@ -1115,6 +1209,24 @@ int mst_batch_get_tuning(const mst_batch *b, int *block, int32_t *tiles) {
     return MST_OK;
 }
 
+int mst_batch_set_strategy(mst_batch *b, int strategy) {
+    B_GUARD(b);
+    if (strategy != MST_STRATEGY_FACE && strategy != MST_STRATEGY_CELL)
+        return mst::fail(MST_E_ARG, "batch strategy must be MST_STRATEGY_FACE or MST_STRATEGY_CELL");
+    if (strategy != b->strategy) {
+        b->strategy = strategy;
+        drop_graphs(b);
+    }
+    return MST_OK;
+}
+
+int mst_batch_get_strategy(const mst_batch *b, int *strategy) {
+    B_GUARD(b);
+    if (!strategy) return mst::fail(MST_E_ARG, "null argument");
+    *strategy = b->strategy;
+    return MST_OK;
+}
+
 int mst_batch_set_graph(mst_batch *b, int on) {
     B_GUARD(b);
     b->use_graph = on != 0;
@@ -1126,10 +1238,11 @@ int mst_batch_autotune(mst_batch *b, const char *cache_path, int force, int step
     if (steps < 1) return mst::fail(MST_E_ARG, "steps must be >= 1");
     const std::string key = signature(b);
     int tiles[MST_BATCH_BUCKETS];
-    int block = 0;
+    int block = 0, strat = MST_STRATEGY_CELL;
     double ms = 0;
-    if (!force && cache_path && cache_lookup(cache_path, key, &block, tiles, &ms)) {
-        int rc = mst_batch_set_tuning(b, block, tiles);
+    if (!force && cache_path && cache_lookup(cache_path, key, &block, tiles, &strat, &ms)) {
+        int rc = mst_batch_set_strategy(b, strat);
+        if (!rc) rc = mst_batch_set_tuning(b, block, tiles);
         if (rc) return rc;
         if (ms_per_step) *ms_per_step = ms;
         return MST_OK;
@@ -1146,11 +1259,11 @@ int mst_batch_autotune(mst_batch *b, const char *cache_path, int force, int step
         if (b->active[i]) hist[bucket_of(b->cstart[i + 1] - b->cstart[i])]++;
     static const int cand_tiles[] = {256, 512, 1024, 2048, 4096, 8192, 16384};
     static const int cand_blocks[] = {64, 128, 256};
-    int best_block = b->block, best_tiles[MST_BATCH_BUCKETS];
+    int best_block = b->block, best_strat = b->strategy, best_tiles[MST_BATCH_BUCKETS];
     for (int k = 0; k < MST_BATCH_BUCKETS; k++) best_tiles[k] = b->tiles[k];
     double best = INFINITY;
     int rc = MST_OK;
-    auto trial = [&](int blk, const int *t) -> int {
+    auto trial = [&](int strategy, int blk, const int *t) -> int {
         // every trial starts from the saved state, so all trials time the same steps
         HIPCHK(hipStreamSynchronize(b->stream));
         b->iold = iold0;
@@ -1159,6 +1272,7 @@ int mst_batch_autotune(mst_batch *b, const char *cache_path, int force, int step
         for (int k = 0; k < 4; k++) b->shadow_ok[k] = false;
         b->dt_valid = false;
         b->block = blk;
+        b->strategy = strategy;
         for (int k = 0; k < MST_BATCH_BUCKETS; k++) b->tiles[k] = t[k];
         double m = 0;
         int r = time_steps(b, steps, &m);
@@ -1166,15 +1280,21 @@ int mst_batch_autotune(mst_batch *b, const char *cache_path, int force, int step
         if (m < best) {
             best = m;
             best_block = blk;
+            best_strat = strategy;
             for (int k = 0; k < MST_BATCH_BUCKETS; k++) best_tiles[k] = t[k];
         }
         return MST_OK;
     };
-    // coordinate search: block size, then each populated bucket's tile, then block size again
+    // coordinate search: strategy x block size, then each populated bucket's tile, then the block
+    // size again
     for (int pass = 0; pass < 2 && !rc; pass++) {
-        for (int blk : cand_blocks) {
-            if (pass == 1 && blk == best_block) continue;
-            rc = trial(blk, best_tiles);
+        for (int st : {MST_STRATEGY_FACE, MST_STRATEGY_CELL}) {
+            if (pass == 1 && st != best_strat) continue;
+            for (int blk : cand_blocks) {
+                if (pass == 1 && blk == best_block) continue;
+                rc = trial(st, blk, best_tiles);
+                if (rc) break;
+            }
             if (rc) break;
         }
         if (pass == 1 || rc) break;
@@ -1185,7 +1305,7 @@ int mst_batch_autotune(mst_batch *b, const char *cache_path, int force, int step
                 int tt[MST_BATCH_BUCKETS];
                 for (int j = 0; j < MST_BATCH_BUCKETS; j++) tt[j] = best_tiles[j];
                 tt[k] = t;
-                rc = trial(best_block, tt);
+                rc = trial(best_strat, best_block, tt);
                 if (rc) break;
             }
         }
@@ -1202,9 +1322,10 @@ int mst_batch_autotune(mst_batch *b, const char *cache_path, int force, int step
     if (rc) return rc;
     int rc2 = check_bad(b);
     if (rc2) return rc2;
-    rc = mst_batch_set_tuning(b, best_block, best_tiles);
+    rc = mst_batch_set_strategy(b, best_strat);
+    if (!rc) rc = mst_batch_set_tuning(b, best_block, best_tiles);
     if (rc) return rc;
-    if (cache_path) cache_store(cache_path, key, best_block, best_tiles, best);
+    if (cache_path) cache_store(cache_path, key, best_block, best_tiles, best_strat, best);
     if (ms_per_step) *ms_per_step = best;
     return MST_OK;
 }

@@ -51,7 +51,8 @@ struct mst_ctx {
     Params<double> p64;
     Params<float> p32;
     bool mixed = false;
-    int strategy = MST_STRATEGY_FACE;
+    int strategy = MST_STRATEGY_AUTO;
+    int resolved = -1;  // strategy in use (AUTO: measured at the first stage after a configuration)
     hipStream_t stream = nullptr;
     int device = 0;
     int dim = 2, nv = 4, maxf = 3;
@@ -281,9 +282,15 @@ int run_cell_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int s
                   : launch(x, MST_K_CELL, k_cell<DIM, MST_SCHEME_AUSM, false, T>, x->nc, fa, a);
 }
 
+int resolve_strategy(mst_ctx *x);
+
 // one residual evaluation + stage update (mode includes UPD_STAGE) with the context's strategy
 int run_stage(mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
-    if (x->strategy == MST_STRATEGY_FACE) {
+    if (x->resolved < 0) {
+        int rc = resolve_strategy(x);
+        if (rc) return rc;
+    }
+    if (x->resolved == MST_STRATEGY_FACE) {
         int rc = run_face_flux(x, is);
         if (rc) return rc;
         return run_update(x, mode, is, in, ca, cb, stage0, iout);
@@ -521,6 +528,45 @@ int check_bad(mst_ctx *x) {
     return MST_OK;
 }
 
+// AUTO: time one stage of each strategy on this mesh and configuration (outputs go to a spare state
+// buffer; the Old and New states are untouched) and keep the faster
+int resolve_strategy(mst_ctx *x) {
+    if (x->strategy != MST_STRATEGY_AUTO) {
+        x->resolved = x->strategy;
+        return MST_OK;
+    }
+    int spare = 0;
+    while (spare == x->iold || spare == x->inew) spare++;
+    const bool timing = x->timing;
+    x->timing = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    float best = INFINITY;
+    int pick = MST_STRATEGY_FACE, rc = MST_OK;
+    for (int s : {MST_STRATEGY_FACE, MST_STRATEGY_CELL}) {
+        x->resolved = s;
+        rc = run_stage(x, UPD_STAGE, x->iold, x->iold, 0.0, 1.0, 1, spare);  // warm-up
+        if (rc) break;
+        if (hipEventRecord(e0, x->stream) != hipSuccess) break;
+        for (int r = 0; r < 3 && !rc; r++) rc = run_stage(x, UPD_STAGE, x->iold, x->iold, 0.0, 1.0, 1, spare);
+        if (rc || hipEventRecord(e1, x->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) break;
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best) {
+            best = ms;
+            pick = s;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    x->timing = timing;
+    x->resolved = pick;
+    // the trial stages may have flagged their (unchecked) outputs, e.g. with no time step set yet
+    HIPCHK(hipStreamSynchronize(x->stream));
+    HIPCHK(hipMemset(x->bad, 0, sizeof(int)));
+    return rc;
+}
+
 }  // namespace
 
 #define CTX_GUARD(x)                                           \
@@ -635,6 +681,7 @@ int mst_ctx_set_config(mst_ctx *ctx, const mst_config *cfg) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->cfg = *cfg;
     ctx->dt_valid = false;
+    if (ctx->strategy == MST_STRATEGY_AUTO) ctx->resolved = -1;  // the best strategy depends on the configuration
     return upload_config(ctx);
 }
 
@@ -694,15 +741,17 @@ int mst_solve(mst_ctx *ctx) {
 
 int mst_set_strategy(mst_ctx *ctx, int strategy) {
     CTX_GUARD(ctx);
-    if (strategy != MST_STRATEGY_FACE && strategy != MST_STRATEGY_CELL) return mst::fail(MST_E_ARG, "bad strategy");
+    if (strategy != MST_STRATEGY_FACE && strategy != MST_STRATEGY_CELL && strategy != MST_STRATEGY_AUTO)
+        return mst::fail(MST_E_ARG, "bad strategy");
     ctx->strategy = strategy;
+    ctx->resolved = strategy == MST_STRATEGY_AUTO ? -1 : strategy;
     return MST_OK;
 }
 
 int mst_get_strategy(const mst_ctx *ctx, int *strategy) {
     CTX_GUARD(ctx);
     if (!strategy) return mst::fail(MST_E_ARG, "null argument");
-    *strategy = ctx->strategy;
+    *strategy = ctx->resolved >= 0 ? ctx->resolved : ctx->strategy;
     return MST_OK;
 }
 
@@ -1063,7 +1112,7 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
             *bytes = upd + nf * 8 * NV;
             break;
         case MST_K_CELL:  // flux inputs + update, no face-flux array; FP64 reads the stage state once
-            *bytes = flux_in + upd - (mixed ? 0 : nc * 8 * NV);
+            *bytes = flux_in + upd - (mixed ? 0 : (second ? nbnd : nc) * 8 * NV);
             break;
         case MST_K_DT:
             *bytes = nc * (8 * NV + 8 + 8 * D);

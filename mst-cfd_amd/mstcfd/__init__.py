@@ -26,7 +26,7 @@ COMPAT_AUSM_ONESIDED = 16
 COMPAT_REFERENCE = COMPAT_GRAD_LASTINT | COMPAT_AUSM_LOCAL_A | COMPAT_AUSM_P_UNSCALED | COMPAT_AUSM_ONESIDED
 K_GRAD, K_FLUX, K_UPDATE, K_DT, K_CELL = 0, 1, 2, 3, 4
 KERNEL_NAMES = {K_GRAD: "k_grad", K_FLUX: "k_flux", K_UPDATE: "k_update", K_DT: "k_cfl", K_CELL: "k_cell"}
-STRATEGY_FACE, STRATEGY_CELL = 0, 1
+STRATEGY_FACE, STRATEGY_CELL, STRATEGY_AUTO = 0, 1, 2
 INIT_REFERENCE, INIT_UNIFORM, INIT_RIEMANN, INIT_VORTEX, INIT_PULSE, INIT_FREESTREAM = 0, 1, 2, 3, 4, 5
 BATCH_BUCKETS = 4
 
@@ -47,7 +47,7 @@ EXPORTS = [
     "mst_mlp_create", "mst_mlp_destroy", "mst_mlp_num_params", "mst_mlp_get_params", "mst_mlp_set_params",
     "mst_mlp_set_scaling", "mst_mlp_set_data", "mst_mlp_reset_optimizer", "mst_mlp_train", "mst_mlp_forward",
     "mst_mlp_gradient", "mst_ctx_warm_start", "mst_run_to_steady", "mst_batch_run_to_steady",
-    "mst_mesh_cell_centres",
+    "mst_mesh_cell_centres", "mst_batch_set_strategy", "mst_batch_get_strategy",
 ]
 
 
@@ -166,6 +166,8 @@ def lib():
         "mst_batch_set_tuning": (C.c_int, [vp, C.c_int, I32P]),
         "mst_batch_get_tuning": (C.c_int, [vp, C.POINTER(C.c_int), I32P]),
         "mst_batch_set_graph": (C.c_int, [vp, C.c_int]),
+        "mst_batch_set_strategy": (C.c_int, [vp, C.c_int]),
+        "mst_batch_get_strategy": (C.c_int, [vp, C.POINTER(C.c_int)]),
         "mst_batch_autotune": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
         "mst_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, pp]),
         "mst_mlp_destroy": (C.c_int, [vp]),
@@ -558,6 +560,16 @@ class Batch:
 
     def set_graph(self, on):
         _check(lib().mst_batch_set_graph(self.h, 1 if on else 0), "mst_batch_set_graph")
+
+    @property
+    def strategy(self):
+        v = C.c_int()
+        _check(lib().mst_batch_get_strategy(self.h, C.byref(v)), "mst_batch_get_strategy")
+        return v.value
+
+    @strategy.setter
+    def strategy(self, value):
+        _check(lib().mst_batch_set_strategy(self.h, int(value)), "mst_batch_set_strategy")
 
     def autotune(self, cache_path=None, force=False, steps=5):
         ms = C.c_double()

@@ -64,19 +64,36 @@ def test_batch_equals_single_case_contexts(precision):
         assert np.allclose(res[i], s.residual_norms(), rtol=1e-13, atol=0)
 
 
+@pytest.mark.parametrize("strategy", [M.STRATEGY_FACE, M.STRATEGY_CELL])
+@pytest.mark.parametrize("rk", [1, 3])
+def test_batch_first_order_equals_single(rk, strategy):
+    """Order 1 (no gradient kernel between the stage kernels) with an odd stage count."""
+    meshes, cfgs, Q0 = riemann_cases(5, seed=13)
+    cfgs = [dict(c, order=1, limiter=M.LIM_NONE, rk=rk, cfl=0.4) for c in cfgs]
+    b = make_batch(meshes, cfgs, Q0)
+    b.strategy = strategy
+    b.step(9)
+    for i in range(5):
+        q, _ = single(meshes[i], cfgs[i], Q0[i], 9)
+        assert np.array_equal(b.get_state(i), q), f"case {i}"
+
+
 def test_batch_tiling_block_graph_independence():
     meshes, cfgs, Q0 = riemann_cases(9, seed=5, cells=(100, 6000))
     out = []
-    for block, tiles, graph in [(256, [1024, 2048, 4096, 8192], True), (64, [64, 128, 256, 512], True),
-                                (128, [256, 256, 512, 16384], False)]:
+    for block, tiles, graph, strategy in [(256, [1024, 2048, 4096, 8192], True, M.STRATEGY_FACE),
+                                          (64, [64, 128, 256, 512], True, M.STRATEGY_CELL),
+                                          (128, [256, 256, 512, 16384], False, M.STRATEGY_CELL),
+                                          (128, [512, 1024, 64, 16384], False, M.STRATEGY_FACE)]:
         b = make_batch(meshes, cfgs, Q0)
         b.set_tuning(block, tiles)
         b.set_graph(graph)
-        assert b.get_tuning() == (block, tiles)
+        b.strategy = strategy
+        assert b.get_tuning() == (block, tiles) and b.strategy == strategy
         b.step(20)
         out.append([b.get_state(i) for i in range(len(meshes))])
         b.close()
-    for k in (1, 2):
+    for k in (1, 2, 3):
         for i in range(len(meshes)):
             assert np.array_equal(out[k][i], out[0][i])
 
@@ -114,11 +131,11 @@ def test_batch_autotune_cache_and_state(tmp_path):
     entries = json.loads(path.read_text())
     assert len(entries) == 1
     (e,) = entries.values()
-    assert e["block"] == block and e["tiles"] == tiles
+    assert e["block"] == block and e["tiles"] == tiles and e["strategy"] == b.strategy
     # a second batch of the same composition takes the cached shape without measuring
     b2 = make_batch(meshes, cfgs, Q0)
     assert b2.autotune(path, steps=3) == pytest.approx(e["ms"])
-    assert b2.get_tuning() == (block, tiles)
+    assert b2.get_tuning() == (block, tiles) and b2.strategy == b.strategy
     b.step(10)
     b2.step(10)
     for i in range(8):
