@@ -245,3 +245,62 @@ def test_cell_strategy_matches_face_strategy(mname, scheme, lim, precision):
     print(f"{mname} prec={precision}: cell vs face after 20 RK3 steps {err:.2e}", flush=True)
     assert err <= 1e-14
     assert np.allclose(out[1][1], out[0][1], rtol=1e-12, atol=0)
+
+
+def test_auto_strategy_resolves_and_matches():
+    """AUTO times both strategies on the first stage (outputs to a spare buffer) and keeps one; the
+    run equals an explicitly chosen strategy bit for bit."""
+    m, om, geo = mesh("periodic")
+    cg = M.make_config(scheme=ROE, order=2, limiter=M.LIM_VANLEER, rk=3, cfl=0.45)
+    Q = smooth_state(geo["ccen"], seed=6, mach=0.7)
+    s = M.Solver(m, cg)
+    assert s.strategy == M.STRATEGY_AUTO
+    s.set_state(Q)
+    s.step(5)
+    assert s.strategy in (M.STRATEGY_FACE, M.STRATEGY_CELL)
+    f = M.Solver(m, cg)
+    f.strategy = M.STRATEGY_FACE
+    f.set_state(Q)
+    f.step(5)
+    assert np.array_equal(s.get_state(), f.get_state())
+    # a configuration change measures again
+    s.set_config(M.make_config(scheme=AUSM, order=2, limiter=M.LIM_MINMOD, rk=2, cfl=0.4))
+    assert s.strategy == M.STRATEGY_AUTO
+
+
+@pytest.mark.parametrize("strategy", [M.STRATEGY_FACE, M.STRATEGY_CELL])
+def test_kernel_timing_counts_and_leaves_results_unchanged(strategy):
+    """Per-kernel timing (events carried by the dispatches) counts one interval per launch, each
+    positive, their sum below the wall time of the steps, and changes no result bit."""
+    import time
+    m, om, geo = mesh("periodic")
+    cg = M.make_config(scheme=ROE, order=2, limiter=M.LIM_VANLEER, rk=3, cfl=0.45)
+    Q = smooth_state(geo["ccen"], seed=8, mach=0.6)
+    plain = M.Solver(m, cg)
+    plain.strategy = strategy
+    plain.set_state(Q)
+    plain.step(4)
+    s = M.Solver(m, cg)
+    s.strategy = strategy
+    s.set_state(Q)
+    s.sync()
+    s.timing_reset()
+    s.timing(True)
+    t0 = time.perf_counter()
+    s.step(4)
+    s.sync()
+    wall_ms = 1e3 * (time.perf_counter() - t0)
+    s.timing(False)
+    assert np.array_equal(s.get_state(), plain.get_state())
+    fused = strategy == M.STRATEGY_CELL
+    want = {M.K_GRAD: 12, M.K_FLUX: 0 if fused else 12, M.K_UPDATE: 0 if fused else 12,
+            M.K_CELL: 12 if fused else 0, M.K_DT: 1}  # later steps take dt from the last stage's update
+    total = 0.0
+    for k, n_want in want.items():
+        n, ms = s.timing_get(k)
+        assert n == n_want, (M.KERNEL_NAMES[k], n)
+        assert (ms > 0) == (n > 0), (M.KERNEL_NAMES[k], ms)
+        total += ms
+    assert total < wall_ms
+    s.timing_reset()
+    assert s.timing_get(M.K_GRAD) == (0, 0.0)
