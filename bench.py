@@ -221,9 +221,11 @@ def main():
     ap.add_argument("--workload", default="vortex2d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--retune", action="store_true", help="batch workload: ignore the tuning cache")
-    ap.add_argument("--strategy", choices=("auto", "face", "cell"), default="auto",
-                    help="face: flux kernel + gather/update kernel; cell: fused cell kernel; auto: time both "
-                         "and report the faster")
+    ap.add_argument("--strategy", choices=("auto", "face", "cell", "patch"), default="auto",
+                    help="face: flux kernel + gather/update kernel; cell: fused cell kernel; patch: fused patch "
+                         "kernel (face fluxes of a 256-cell patch in LDS); auto: time each and report the fastest")
+    ap.add_argument("--order", choices=("hilbert", "rcm"), default="hilbert",
+                    help="cell order of the device layout (Hilbert-curve patches or reverse Cuthill-McKee)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison")
     ap.add_argument("--compare-steps", type=int, default=200,
                     help="steps of the FP64 vs mixed state comparison")
@@ -237,6 +239,7 @@ def main():
         sys.exit(2)
     import mstcfd as M
 
+    M.set_cell_order(M.ORDER_HILBERT if args.order == "hilbert" else M.ORDER_RCM)
     if args.workload == "batch":
         return bench_batch(M, args)
     if args.workload == "ramp":
@@ -250,8 +253,8 @@ def main():
     solver = M.Solver(mesh, M.make_config(**cfg_kw))
     Q0 = init()
     stages = cfg_kw["rk"]
-    names = {"face": M.STRATEGY_FACE, "cell": M.STRATEGY_CELL}
-    tried = ("face", "cell") if args.strategy == "auto" else (args.strategy,)
+    names = {"face": M.STRATEGY_FACE, "cell": M.STRATEGY_CELL, "patch": M.STRATEGY_PATCH}
+    tried = ("face", "cell", "patch") if args.strategy == "auto" else (args.strategy,)
     runs = {}
     for st in tried:  # each strategy on the same context from the same initial field
         solver.strategy = names[st]
@@ -263,7 +266,7 @@ def main():
     wall, kern, dom = runs[best]["wall"], runs[best]["kernels"], runs[best]["dominant"]
     solver.strategy = names[best]
     value = mesh.n_faces * stages * args.steps / wall
-    desc = dict(desc, strategy=best)
+    desc = dict(desc, strategy=best, cell_order=args.order)
 
     def roofline(st):
         r = runs[st]

@@ -129,9 +129,18 @@ int mst_mesh_cell_centres(const mst_mesh *m, double *ccen, int64_t n_cells);
 
 /* ---------------------------------------------------------------- host preprocessing (no device needed)
  * The one-time part of the reference's mesh set-up (mesh/MshBlock.cpp:272-334, mesh/Face.cpp:8-69,
- * mesh/Cell.cpp:6-61) plus the device layout: geometry in FP64, reverse Cuthill-McKee cell order,
- * faces sorted by kind then lowest cell, greedy face colouring, slot-major cell->face table. */
+ * mesh/Cell.cpp:6-61) plus the device layout: geometry in FP64, Hilbert-curve (or reverse
+ * Cuthill-McKee) cell order, faces sorted by kind then lowest cell, greedy face colouring, slot-major
+ * cell->face table. */
 typedef struct mst_prep mst_prep;
+/* NEW: cell order of every later preprocessing (process-wide, default HILBERT).  HILBERT sorts the
+ * cells along a Hilbert curve through their centres, so each run of consecutive cells is a compact
+ * patch (most faces of a 256-cell block join two cells of the block); RCM minimises the bandwidth of
+ * the cell graph.  Results agree to rounding (the gather order of a cell's faces is unchanged). */
+#define MST_ORDER_HILBERT 0
+#define MST_ORDER_RCM 1
+int mst_set_cell_order(int order);
+int mst_get_cell_order(int *order);
 int mst_prep_create(const mst_mesh *m, mst_prep **out);
 void mst_prep_destroy(mst_prep *p);
 /* out[0..7] = dim, nv, n_cells, n_faces, max faces per cell, colours, interior faces, periodic faces */
@@ -183,13 +192,17 @@ int mst_step(mst_ctx *ctx, int nsteps);
 /* NEW: kernel strategy of mst_step and mst_solve.  FACE: a face kernel writes every face flux once,
  * a cell kernel gathers them and updates (the reference's updateFaceFlux + solve split).  CELL: one
  * fused cell kernel solves each cell's own faces (every interior face twice, bitwise the same flux
- * on both sides) and updates, with no face-flux array in memory.  Both give the same result bit for
- * bit.  AUTO (default): the first stage after creation or a configuration change times one stage of
- * each on the context's own mesh and keeps the faster; mst_get_strategy then reports it.
- * mst_face_flux / mst_residual_update keep the face split whatever the strategy. */
+ * on both sides) and updates, with no face-flux array in memory.  PATCH: one kernel per block of 256
+ * consecutive cells (a compact patch in the Hilbert cell order) solves the patch's faces once into
+ * LDS, then sums and updates per cell; available for cells of at most 4 faces (MST_E_UNSUPPORTED
+ * otherwise).  All give the same result bit for bit.  AUTO (default): the first stage after creation
+ * or a configuration change times one stage of each available strategy on the context's own mesh
+ * and keeps the fastest; mst_get_strategy then reports it.  mst_face_flux / mst_residual_update keep
+ * the face split whatever the strategy. */
 #define MST_STRATEGY_FACE 0
 #define MST_STRATEGY_CELL 1
 #define MST_STRATEGY_AUTO 2
+#define MST_STRATEGY_PATCH 3
 int mst_set_strategy(mst_ctx *ctx, int strategy);
 int mst_get_strategy(const mst_ctx *ctx, int *strategy);
 /* NEW: steady driver.  Steps of mst_step until max_f L2(R/V)_f / ref_f <= tol (checked every
@@ -300,7 +313,8 @@ int mst_ctx_warm_start(mst_ctx *ctx, mst_mlp *m, const double *cond, int n_cond)
 #define MST_K_UPDATE 2
 #define MST_K_DT 3
 #define MST_K_CELL 4 /* fused flux + gather + update (MST_STRATEGY_CELL) */
-#define MST_K_COUNT 5
+#define MST_K_PATCH 5 /* patch fluxes in LDS + gather + update (MST_STRATEGY_PATCH) */
+#define MST_K_COUNT 6
 int mst_timing_enable(mst_ctx *ctx, int on);
 /* per kernel class: launches timed and total milliseconds since the last reset */
 int mst_timing_get(mst_ctx *ctx, int kernel, int64_t *count, double *total_ms);

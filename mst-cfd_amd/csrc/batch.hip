@@ -166,7 +166,7 @@ __global__ __launch_bounds__(NT) void kb_update(const BArgs<T> a) {
 }
 
 // cell strategy: each lane solves its cell's faces (scheme per case), sums and updates the stage
-template <int DIM, bool SECOND, typename T, int NT>
+template <int DIM, bool SECOND, typename T, int NT, int MAXF>
 __global__ __launch_bounds__(NT) void kb_cell(const BArgs<T> a) {
     constexpr int NV = DIM + 2;
     constexpr bool F32 = std::is_same<T, float>::value;
@@ -192,20 +192,7 @@ __global__ __launch_bounds__(NT) void kb_cell(const BArgs<T> a) {
             double r[NV], cmp[NV];
 #pragma unroll
             for (int k = 0; k < NV; k++) r[k] = cmp[k] = 0;
-            for (int sl = 0; sl < ua.maxf; sl++) {
-                const int32_t e = ua.ell[(int64_t)sl * ua.nc + c];
-                if (e < 0) break;
-                const int64_t f = e >> 1;
-                const int info = fa.finfo[f];
-                const int kind = info & 15;
-                if (kind >= mst::KIND_NONE) continue;
-                double g[NV];
-                if (roe)
-                    face_flux<DIM, MST_SCHEME_ROE, SECOND, T>(fa, f, kind, info >> 4, g);
-                else
-                    face_flux<DIM, MST_SCHEME_AUSM, SECOND, T>(fa, f, kind, info >> 4, g);
-                accumulate<NV, F32>(r, cmp, g, e & 1);
-            }
+            cell_fluxes<DIM, SCHEME_RUNTIME, SECOND, T, MAXF>(fa, ua.ell, ua.nc, roe, c, r, cmp);
             if constexpr (F32) {
 #pragma unroll
                 for (int k = 0; k < NV; k++) r[k] += cmp[k];
@@ -565,8 +552,13 @@ int run_cell(mst_batch *b, int stage, int is, int in, double ca, double cb, int 
     a.u.l2 = l2;
     a.u.dtred = dt;
     a.u.check = check;
-    if (b->order >= 2) return launch_persistent(b, kb_cell<DIM, true, T, NT>, NT, a.n_items, a);
-    return launch_persistent(b, kb_cell<DIM, false, T, NT>, NT, a.n_items, a);
+#define MST_BCELL(MF)                                                                                          \
+    if (b->order >= 2) return launch_persistent(b, kb_cell<DIM, true, T, NT, MF>, NT, a.n_items, a);          \
+    return launch_persistent(b, kb_cell<DIM, false, T, NT, MF>, NT, a.n_items, a);
+    if (b->slots == 3) { MST_BCELL(3) }
+    if (b->slots == 4) { MST_BCELL(4) }
+    MST_BCELL(8)
+#undef MST_BCELL
 }
 
 int run_finish(mst_batch *b, bool l2, bool dt) {

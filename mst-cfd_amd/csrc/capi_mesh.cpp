@@ -182,6 +182,18 @@ int mst_mesh_gen_tet(int nx, int ny, int nz, const double lo[3], const double hi
     return MST_OK;
 }
 
+int mst_set_cell_order(int order) {
+    if (order != MST_ORDER_HILBERT && order != MST_ORDER_RCM) return mst::fail(MST_E_ARG, "bad cell order");
+    mst::set_cell_order(order);
+    return MST_OK;
+}
+
+int mst_get_cell_order(int *order) {
+    if (!order) return mst::fail(MST_E_ARG, "null argument");
+    *order = mst::cell_order();
+    return MST_OK;
+}
+
 int mst_prep_create(const mst_mesh *m, mst_prep **out) {
     if (!m || !out) return mst::fail(MST_E_ARG, "null argument");
     *out = nullptr;

@@ -8,6 +8,8 @@
 //                        atomics -> bit-reproducible), then the Euler / SSP-RK stage update
 //   k_cell               (cell strategy, replaces k_flux + k_update) one lane per cell: solves its own
 //                        faces' Riemann problems, sums them and updates, without a face-flux array
+//   k_patch              (patch strategy, replaces k_flux + k_update) one block per patch of cells:
+//                        solves the patch's faces once into LDS, then sums and updates per cell
 // plus k_cfl (CFL time step) and one-block reductions (k_finish) of per-block partials.
 //
 // Precision forms: T = double is the plain FP64 path (the reference's arithmetic); T = float is the
@@ -52,67 +54,108 @@ struct FluxArgs {
     Params<T> P;
 };
 
-// Riemann left (A: c0 side) and right (B: c1 side or ghost) states of face f (RhoSolver::updateFaceFlux,
-// RhoSolver.cpp:90-233, and updateFaceFlux2ndOrder, :234-369): order 1 the cell states, order 2 the
-// reconstructed states the gradient kernel stored for the face's two incidences, and the ghost states
-// of the boundary faces, which combine the raw and the reconstructed interior state as the reference
-// does.  Every input is read from memory by face index, so the lanes of the two cells of a face (cell
-// strategy) compute bit-identical states.
+// Inputs of face f's Riemann problem, every memory read of the face (RhoSolver::updateFaceFlux,
+// RhoSolver.cpp:90-233, and updateFaceFlux2ndOrder, :234-369): the c0 outward normal, and X, Y = the
+// c0- and c1-side states of an interior or periodic face (order 1 the cell states, order 2 the
+// reconstructed states the gradient kernel stored for the face's two incidences), or X = the raw
+// interior state and Y = its reconstructed face state for a boundary face, whose ghost combines the
+// two as the reference does.  Everything is read by face index, so the lanes of the two cells of a
+// face (cell strategy) load and then compute bit-identical states.
 template <int DIM, bool SECOND, typename T>
-__device__ __forceinline__ void face_states(const FluxArgs<T> &a, int64_t f, int kind, const T *n, T (&A)[DIM + 2],
-                                            T (&B)[DIM + 2]) {
+__device__ __forceinline__ void face_load(const FluxArgs<T> &a, int64_t f, int kind, double (&n64)[DIM],
+                                          T (&X)[DIM + 2], T (&Y)[DIM + 2]) {
     constexpr int NV = DIM + 2;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) n64[d] = a.fn[f * DIM + d];
     if (kind == mst::KIND_INTERIOR || kind == mst::KIND_PERIODIC) {
         if constexpr (SECOND) {
-            ldq<NV>(a.rec + (int64_t)a.frec0[f] * NV, A);
-            ldq<NV>(a.rec + (int64_t)a.frec1[f] * NV, B);
+            ldq<NV>(a.rec + (int64_t)a.frec0[f] * NV, X);
+            ldq<NV>(a.rec + (int64_t)a.frec1[f] * NV, Y);
         } else {
-            ldq<NV>(a.Q + (int64_t)a.fc0[f] * NV, A);
-            ldq<NV>(a.Q + (int64_t)a.fc1[f] * NV, B);
+            ldq<NV>(a.Q + (int64_t)a.fc0[f] * NV, X);
+            ldq<NV>(a.Q + (int64_t)a.fc1[f] * NV, Y);
         }
         return;
     }
-    ldq<NV>(a.Q + (int64_t)a.fc0[f] * NV, A);
-    T Rc[NV];
+    ldq<NV>(a.Q + (int64_t)a.fc0[f] * NV, X);
     if constexpr (SECOND) {
-        ldq<NV>(a.rec + (int64_t)a.frec0[f] * NV, Rc);
+        ldq<NV>(a.rec + (int64_t)a.frec0[f] * NV, Y);
     } else {
 #pragma unroll
-        for (int k = 0; k < NV; k++) Rc[k] = A[k];
+        for (int k = 0; k < NV; k++) Y[k] = X[k];
+    }
+}
+
+// Riemann left (A: c0 side) and right (B: c1 side or ghost) states from face_load's inputs; registers only
+template <int DIM, typename T>
+__device__ __forceinline__ void face_bc(const Params<T> &P, int kind, const T *n, const T (&X)[DIM + 2],
+                                        const T (&Rc)[DIM + 2], T (&A)[DIM + 2], T (&B)[DIM + 2]) {
+    constexpr int NV = DIM + 2;
+    if (kind == mst::KIND_INTERIOR || kind == mst::KIND_PERIODIC) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            A[k] = X[k];
+            B[k] = Rc[k];
+        }
+        return;
     }
     if (kind == mst::KIND_WALL) {  // RhoSolver.cpp:142-179 / 285-319
 #pragma unroll
         for (int k = 0; k < NV; k++) A[k] = B[k] = Rc[k];
         reflect<DIM>(B, A, n);
-    } else if (kind == mst::KIND_SYMMETRY) {  // :184-211 / 322-350
+    } else if (kind == mst::KIND_SYMMETRY) {  // :184-211 / 322-350 (A keeps the raw interior state)
 #pragma unroll
-        for (int k = 0; k < NV; k++) B[k] = Rc[k];
+        for (int k = 0; k < NV; k++) {
+            A[k] = X[k];
+            B[k] = Rc[k];
+        }
         reflect<DIM>(B, A, n);
     } else if (kind == mst::KIND_INLET) {  // :121-137 / 265-281
 #pragma unroll
         for (int k = 0; k < NV; k++) {
             A[k] = Rc[k];
-            B[k] = a.P.qin[k];
+            B[k] = P.qin[k];
         }
-    } else {  // outlet :216-226 / 353-363
+    } else {  // outlet :216-226 / 353-363 (A keeps the raw interior state)
 #pragma unroll
-        for (int k = 0; k < NV; k++) B[k] = Rc[k];
+        for (int k = 0; k < NV; k++) {
+            A[k] = X[k];
+            B[k] = Rc[k];
+        }
     }
 }
 
-// G_f (c0 outward) of one face: states, dimension-split Riemann solves, FP64 projection
+// scheme chosen per call (a batch mixes Roe and AUSM cases) instead of per instantiation
+constexpr int SCHEME_RUNTIME = -1;
+
+// G_f (c0 outward) from face_load's inputs: ghost/boundary states, dimension-split Riemann solves,
+// FP64 projection
+template <int DIM, int SCHEME, typename T>
+__device__ __forceinline__ void face_solve(const Params<T> &P, bool roe, int kind, int mask, const double (&n64)[DIM],
+                                           const T (&X)[DIM + 2], const T (&Y)[DIM + 2], double (&g)[DIM + 2]) {
+    constexpr int NV = DIM + 2;
+    T n[DIM], A[NV], B[NV];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) n[d] = (T)n64[d];
+    face_bc<DIM, T>(P, kind, n, X, Y, A, B);
+    if constexpr (SCHEME == SCHEME_RUNTIME) {
+        if (roe)
+            riemann_face<DIM, MST_SCHEME_ROE, T>(A, B, n64, mask, P, g);
+        else
+            riemann_face<DIM, MST_SCHEME_AUSM, T>(A, B, n64, mask, P, g);
+    } else {
+        riemann_face<DIM, SCHEME, T>(A, B, n64, mask, P, g);
+    }
+}
+
+// G_f of one face: load, then solve
 template <int DIM, int SCHEME, bool SECOND, typename T>
 __device__ __forceinline__ void face_flux(const FluxArgs<T> &a, int64_t f, int kind, int mask, double (&g)[DIM + 2]) {
     constexpr int NV = DIM + 2;
     double n64[DIM];
-    T n[DIM], A[NV], B[NV];
-#pragma unroll
-    for (int d = 0; d < DIM; d++) {
-        n64[d] = a.fn[f * DIM + d];
-        n[d] = (T)n64[d];
-    }
-    face_states<DIM, SECOND, T>(a, f, kind, n, A, B);
-    riemann_face<DIM, SCHEME, T>(A, B, n64, mask, a.P, g);
+    T X[NV], Y[NV];
+    face_load<DIM, SECOND, T>(a, f, kind, n64, X, Y);
+    face_solve<DIM, SCHEME, T>(a.P, SCHEME == MST_SCHEME_ROE, kind, mask, n64, X, Y, g);
 }
 
 // face strategy: one lane per face writes G_f (RhoSolver::updateFaceFlux[2ndOrder])
@@ -468,11 +511,66 @@ __global__ __launch_bounds__(BLOCK) void k_update(const UpdArgs a) {
     stage_tail<NV, (MODE & UPD_STAGE) != 0, F32>(a, c, valid, r, blk, (MODE & UPD_L2) != 0, (MODE & UPD_DT) != 0);
 }
 
+// Sum over cell c's faces of +-G_f (accumulate: c0 adds, c1 subtracts), slots in order.  The face
+// records of a cell are read twice per kernel, once by each of the face's cells; they come from L2 the
+// second time only if the two reads are close in time.  For 2D cells of 3 or 4 slots (triangles,
+// quadrilaterals) every memory read of all the cell's faces is therefore issued before the first
+// Riemann solve (4 dependent load rounds per cell instead of 4 per face), so a face's two reads lie
+// about the dispatch distance of its two cells' blocks apart rather than up to a block's lifetime.
+// 3D cells and larger ones load face by face: their staged states would halve the resident waves.
+template <int DIM, int SCHEME, bool SECOND, typename T, int MAXF>
+__device__ __forceinline__ void cell_fluxes(const FluxArgs<T> &fa, const int32_t *__restrict__ ell, int64_t nc,
+                                            bool roe, int64_t c, double (&r)[DIM + 2], double (&cmp)[DIM + 2]) {
+    constexpr int NV = DIM + 2;
+    constexpr bool F32 = std::is_same<T, float>::value;
+    if constexpr (DIM == 2 && MAXF <= 4) {
+        int32_t e[MAXF];
+        int kind[MAXF], mask[MAXF];
+        double n64[MAXF][DIM];
+        T X[MAXF][NV], Y[MAXF][NV];
+#pragma unroll
+        for (int s = 0; s < MAXF; s++) {  // the slot table is padded to MAXF rows
+            e[s] = ell[(int64_t)s * nc + c];
+            kind[s] = mst::KIND_NONE;  // empty slot: no flux
+            mask[s] = 0;
+            if (e[s] >= 0) {
+                const int64_t f = e[s] >> 1;
+                const int info = fa.finfo[f];
+                kind[s] = info & 15;
+                mask[s] = info >> 4;
+                if (kind[s] < mst::KIND_NONE) face_load<DIM, SECOND, T>(fa, f, kind[s], n64[s], X[s], Y[s]);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < MAXF; s++) {
+            if (kind[s] >= mst::KIND_NONE) continue;  // empty slot or zero-flux face
+            double g[NV];
+            face_solve<DIM, SCHEME, T>(fa.P, roe, kind[s], mask[s], n64[s], X[s], Y[s], g);
+            accumulate<NV, F32>(r, cmp, g, e[s] & 1);
+        }
+    } else {
+        for (int s = 0; s < MAXF; s++) {
+            const int32_t e = ell[(int64_t)s * nc + c];
+            if (e < 0) break;
+            const int64_t f = e >> 1;
+            const int info = fa.finfo[f];
+            const int kind = info & 15;
+            if (kind >= mst::KIND_NONE) continue;  // zero-flux face
+            double n64[DIM];
+            T X[NV], Y[NV];
+            face_load<DIM, SECOND, T>(fa, f, kind, n64, X, Y);
+            double g[NV];
+            face_solve<DIM, SCHEME, T>(fa.P, roe, kind, info >> 4, n64, X, Y, g);
+            accumulate<NV, F32>(r, cmp, g, e & 1);
+        }
+    }
+}
+
 // Cell strategy: flux + gather + update fused, one lane per cell.  Each lane solves the Riemann
 // problems of its own faces (an interior face is solved by both of its cells, with bit-identical
 // inputs and instructions, so the two cells still receive exactly opposite fluxes) and never writes
 // face fluxes to memory: the face-flux array's write and re-read are traded for a second solve.
-template <int DIM, int SCHEME, bool SECOND, typename T>
+template <int DIM, int SCHEME, bool SECOND, typename T, int MAXF>
 __global__ __launch_bounds__(BLOCK) void k_cell(const FluxArgs<T> fa, const UpdArgs a) {
     constexpr int NV = DIM + 2;
     constexpr bool F32 = std::is_same<T, float>::value;
@@ -483,16 +581,59 @@ __global__ __launch_bounds__(BLOCK) void k_cell(const FluxArgs<T> fa, const UpdA
 #pragma unroll
     for (int k = 0; k < NV; k++) r[k] = cmp[k] = 0;
     if (valid) {
-        for (int s = 0; s < a.maxf; s++) {
-            const int32_t e = a.ell[(int64_t)s * a.nc + c];
-            if (e < 0) break;
-            const int64_t f = e >> 1;
-            const int info = fa.finfo[f];
-            const int kind = info & 15;
-            if (kind >= mst::KIND_NONE) continue;  // zero-flux face
+        cell_fluxes<DIM, SCHEME, SECOND, T, MAXF>(fa, a.ell, a.nc, SCHEME == MST_SCHEME_ROE, c, r, cmp);
+        if constexpr (F32) {
+#pragma unroll
+            for (int k = 0; k < NV; k++) r[k] += cmp[k];
+        }
+    }
+    stage_tail<NV, true, F32>(a, c, valid, r, blk, a.l2, a.dtred);
+}
+
+// ------------------------------------------------------------------------------------ patch strategy
+// With the cells in Hilbert order, the BLOCK cells of a block form a compact patch of the mesh: about
+// 9 in 10 of the faces it touches join two of its own cells.  The block solves every face of its
+// patch once into LDS (a face on the border of two patches is solved by both, with bit-identical
+// inputs), then each lane sums its cell's fluxes from LDS in slot order and updates: one kernel, no
+// face-flux array in memory, and about 1.65 Riemann problems per cell instead of the cell strategy's
+// three.  The sum is the face strategy's, term for term, so all strategies agree bit for bit.
+struct PatchArgs {
+    const int32_t *__restrict__ start;  // nblk + 1: each block's range in face
+    const int32_t *__restrict__ face;   // the faces of each patch that carry a flux, ascending
+    const int16_t *__restrict__ loc;    // slot-major per cell: (local face << 1) | is_c1, -1: no flux
+};
+
+template <int DIM, int SCHEME, bool SECOND, typename T, int MAXF>
+__global__ __launch_bounds__(BLOCK) void k_patch(const FluxArgs<T> fa, const UpdArgs a, const PatchArgs p) {
+    constexpr int NV = DIM + 2;
+    constexpr bool F32 = std::is_same<T, float>::value;
+    extern __shared__ double gsh[];  // NV x npf, component-major: consecutive lanes, consecutive banks
+    const int64_t blk = block_id();
+    const int32_t f0 = p.start[blk];
+    const int npf = p.start[blk + 1] - f0;
+    for (int i = threadIdx.x; i < npf; i += BLOCK) {
+        const int64_t f = p.face[f0 + i];
+        const int info = fa.finfo[f];
+        double g[NV];
+        face_flux<DIM, SCHEME, SECOND, T>(fa, f, info & 15, info >> 4, g);
+#pragma unroll
+        for (int k = 0; k < NV; k++) gsh[k * npf + i] = g[k];
+    }
+    __syncthreads();
+    const int64_t c = blk * BLOCK + threadIdx.x;
+    const bool valid = c < a.nc;
+    double r[NV], cmp[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) r[k] = cmp[k] = 0;
+    if (valid) {
+#pragma unroll
+        for (int s = 0; s < MAXF; s++) {  // the slot table is padded to MAXF rows
+            const int l = p.loc[(int64_t)s * a.nc + c];
+            if (l < 0) continue;
             double g[NV];
-            face_flux<DIM, SCHEME, SECOND, T>(fa, f, kind, info >> 4, g);
-            accumulate<NV, F32>(r, cmp, g, e & 1);
+#pragma unroll
+            for (int k = 0; k < NV; k++) g[k] = gsh[k * npf + (l >> 1)];
+            accumulate<NV, F32>(r, cmp, g, l & 1);
         }
         if constexpr (F32) {
 #pragma unroll

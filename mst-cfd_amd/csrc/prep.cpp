@@ -1,4 +1,5 @@
-// prep.cpp -- geometry, reverse Cuthill-McKee cell order, face order and colouring, SoA arrays.
+// prep.cpp -- geometry, Hilbert-curve or reverse Cuthill-McKee cell order, face order and colouring,
+// SoA arrays.
 //
 // Geometry follows the reference exactly (same operations in the same order, FP64, built with
 // -ffp-contract=off so it is bit-identical wherever it is compiled):
@@ -11,10 +12,11 @@
 //   eta0 = |x_c1 - x_f| / (|x_c0 - x_f| + |x_c1 - x_f|), 1 on boundaries  mesh/Face.cpp:62-69
 // New here: volumes of other polytopes by the divergence theorem (the reference's 3D formula,
 // mesh/Cell.cpp:52-59, lacks the 1/3 and is orientation dependent), eta0 of periodic faces through
-// the partner face, RCM ordering, face colouring and the slot-major cell->face table.
+// the partner face, Hilbert and RCM orderings, face colouring and the slot-major cell->face table.
 #include "prep.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <numeric>
 
@@ -257,6 +259,70 @@ void rcm(const mst_mesh &m, std::vector<int32_t> &old_of_new) {
     old_of_new = order;
 }
 
+// Hilbert index of a point with n = 2 or 3 coordinates of b bits each, by J. Skilling's transform
+// ("Programming the Hilbert curve", AIP Conf. Proc. 707, 2004): undo the excess work of the axes
+// (invert or exchange low bits from the top bit down), Gray-encode, then interleave the transposed
+// coordinates most significant bit first.
+uint64_t hilbert_index(uint32_t *X, int n, int b) {
+    const uint32_t M = 1u << (b - 1);
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+        for (int i = 0; i < n; i++) {
+            if (X[i] & Q) {
+                X[0] ^= P;
+            } else {
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    for (int i = 1; i < n; i++) X[i] ^= X[i - 1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[n - 1] & Q) t ^= Q - 1;
+    for (int i = 0; i < n; i++) X[i] ^= t;
+    uint64_t key = 0;
+    for (int bit = b - 1; bit >= 0; bit--)
+        for (int i = 0; i < n; i++) key = (key << 1) | ((X[i] >> bit) & 1u);
+    return key;
+}
+
+// Cells along the Hilbert curve through their centres (bounding box scaled to 2^b per axis, ties by
+// file index): consecutive runs of cells are compact patches of the mesh, so most faces of a
+// 256-cell block join two cells of that block.
+void hilbert(const Prepared &P, std::vector<int32_t> &old_of_new) {
+    const int64_t nc = P.nc;
+    const int n = P.dim, b = n == 2 ? 31 : 21;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t c = 0; c < nc; c++)
+        for (int d = 0; d < n; d++) {
+            const double x = P.ccen[c * n + d];
+            if (std::isfinite(x)) {
+                lo[d] = std::min(lo[d], x);
+                hi[d] = std::max(hi[d], x);
+            }
+        }
+    double ext = 0;
+    for (int d = 0; d < n; d++) ext = std::max(ext, hi[d] - lo[d]);
+    const double top = (double)((1u << b) - 1);
+    const double scale = ext > 0 ? top / ext : 0;
+    std::vector<uint64_t> key(nc);
+    for (int64_t c = 0; c < nc; c++) {
+        uint32_t X[3] = {0, 0, 0};
+        for (int d = 0; d < n; d++) {
+            const double x = P.ccen[c * n + d];
+            const double q = std::isfinite(x) ? std::min(std::max((x - lo[d]) * scale, 0.0), top) : 0.0;
+            X[d] = (uint32_t)q;
+        }
+        key[c] = hilbert_index(X, n, b);
+    }
+    old_of_new.resize(nc);
+    std::iota(old_of_new.begin(), old_of_new.end(), 0);
+    std::sort(old_of_new.begin(), old_of_new.end(),
+              [&](int32_t x, int32_t y) { return key[x] != key[y] ? key[x] < key[y] : x < y; });
+}
+
 int32_t kind_of(int32_t t) {
     switch (t) {
         case MST_BC_INTERIOR: return KIND_INTERIOR;
@@ -283,6 +349,13 @@ int cell_centres(const mst_mesh &m, std::vector<double> &ccen) {
     return MST_OK;
 }
 
+namespace {
+std::atomic<int> g_cell_order{MST_ORDER_HILBERT};
+}
+
+int cell_order() { return g_cell_order.load(); }
+void set_cell_order(int order) { g_cell_order.store(order); }
+
 int prepare(const mst_mesh &m, Prepared &P) {
     int rc = validate(m);
     if (rc) return rc;
@@ -303,8 +376,11 @@ int prepare(const mst_mesh &m, Prepared &P) {
             P.bad_cells++;
         }
 
-    // ---- cells: RCM
-    rcm(m, P.cell_old_of_new);
+    // ---- cells: Hilbert curve (default) or reverse Cuthill-McKee
+    if (cell_order() == MST_ORDER_RCM)
+        rcm(m, P.cell_old_of_new);
+    else
+        hilbert(P, P.cell_old_of_new);
     P.cell_new_of_old.assign(nc, -1);
     for (int64_t i = 0; i < nc; i++) P.cell_new_of_old[P.cell_old_of_new[i]] = (int32_t)i;
 

@@ -53,6 +53,10 @@ struct Prepared {
 
 int prepare(const mst_mesh &m, Prepared &P);
 
+// process-wide cell order of prepare (MST_ORDER_HILBERT or MST_ORDER_RCM)
+int cell_order();
+void set_cell_order(int order);
+
 // per new face: bit d set when c0 (or the interior-side state) is the Riemann left state along d
 void lr_masks(const Prepared &P, int compat, std::vector<int32_t> &mask);
 

@@ -66,6 +66,12 @@ struct mst_ctx {
     int iold = 0, inew = 1;
     double *G = nullptr, *rec = nullptr, *graw = nullptr, *phi = nullptr, *R = nullptr;
     int32_t *frec0 = nullptr, *frec1 = nullptr;  // incidence records of each face's c0 / c1 side
+    // patch strategy: per block of BLOCK cells its flux-carrying faces, per cell slot the local index
+    int32_t *pstart = nullptr, *pface = nullptr;
+    int16_t *ploc = nullptr;
+    int max_pf = 0;        // most faces of one patch (sizes the kernel's LDS)
+    int64_t n_psolve = 0;  // Riemann problems per patch launch
+    bool patch_ok = false;
     // mixed form: FP32 shadows of the states, FP32 geometry, offsets, face fluxes, limited gradients
     float *Q32[4] = {nullptr, nullptr, nullptr, nullptr};
     bool shadow_ok[4] = {false, false, false, false};
@@ -103,7 +109,7 @@ hipEvent_t take_event(mst_ctx *x) {
 // kernel's own start and end timestamps, with no marker packets between the kernels of a step, so
 // timing costs no throughput and the interval is the kernel alone.
 template <typename K, typename... Args>
-int launch(mst_ctx *x, int kclass, K kernel, int64_t nthreads, Args... args) {
+int launch_shm(mst_ctx *x, int kclass, K kernel, int64_t nthreads, size_t shm, Args... args) {
     if (nthreads <= 0) return MST_OK;
     const int64_t grid = (nthreads + BLOCK - 1) / BLOCK;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -111,14 +117,20 @@ int launch(mst_ctx *x, int kclass, K kernel, int64_t nthreads, Args... args) {
         e0 = take_event(x);
         e1 = take_event(x);
         if (!e0 || !e1) return mst::fail(MST_E_HIP, "hipEventCreate failed");
-        hipExtLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(BLOCK), 0, x->stream, e0, e1, 0, args...);
+        hipExtLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(BLOCK), (uint32_t)shm, x->stream, e0, e1, 0,
+                              args...);
         HIPCHK(hipGetLastError());
         x->ev[kclass].push_back({e0, e1});
         return MST_OK;
     }
-    hipLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(BLOCK), 0, x->stream, args...);
+    hipLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(BLOCK), (uint32_t)shm, x->stream, args...);
     HIPCHK(hipGetLastError());
     return MST_OK;
+}
+
+template <typename K, typename... Args>
+int launch(mst_ctx *x, int kclass, K kernel, int64_t nthreads, Args... args) {
+    return launch_shm(x, kclass, kernel, nthreads, 0, args...);
 }
 
 // FP32 shadow of state buffer i (mixed form), made on demand after a host write
@@ -279,11 +291,42 @@ int run_cell_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int s
     const FluxArgs<T> fa = flux_args<T>(x, is);
     const UpdArgs a = upd_args(x, mode, is, in, ca, cb, stage0, iout);
     const bool second = x->cfg.order >= 2;
-    if (x->cfg.scheme == MST_SCHEME_ROE)
-        return second ? launch(x, MST_K_CELL, k_cell<DIM, MST_SCHEME_ROE, true, T>, x->nc, fa, a)
-                      : launch(x, MST_K_CELL, k_cell<DIM, MST_SCHEME_ROE, false, T>, x->nc, fa, a);
-    return second ? launch(x, MST_K_CELL, k_cell<DIM, MST_SCHEME_AUSM, true, T>, x->nc, fa, a)
-                  : launch(x, MST_K_CELL, k_cell<DIM, MST_SCHEME_AUSM, false, T>, x->nc, fa, a);
+    const bool roe = x->cfg.scheme == MST_SCHEME_ROE;
+    // slot count of the padded slot-major table: 3 (triangles), 4 (quadrilaterals, tetrahedra), 8
+#define MST_CELL(MF)                                                                                            \
+    if (roe)                                                                                                    \
+        return second ? launch(x, MST_K_CELL, k_cell<DIM, MST_SCHEME_ROE, true, T, MF>, x->nc, fa, a)           \
+                      : launch(x, MST_K_CELL, k_cell<DIM, MST_SCHEME_ROE, false, T, MF>, x->nc, fa, a);         \
+    return second ? launch(x, MST_K_CELL, k_cell<DIM, MST_SCHEME_AUSM, true, T, MF>, x->nc, fa, a)              \
+                  : launch(x, MST_K_CELL, k_cell<DIM, MST_SCHEME_AUSM, false, T, MF>, x->nc, fa, a);
+    if (x->slots == 3) { MST_CELL(3) }
+    if (x->slots == 4) { MST_CELL(4) }
+    MST_CELL(8)
+#undef MST_CELL
+}
+
+// patch strategy: gradient + limiter (order 2), then the patch kernel (faces into LDS, sum, update)
+template <int DIM, typename T>
+int run_patch_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
+    if (!x->patch_ok) return mst::fail(MST_E_UNSUPPORTED, "no patch tables for this mesh");
+    const FluxArgs<T> fa = flux_args<T>(x, is);
+    const UpdArgs a = upd_args(x, mode, is, in, ca, cb, stage0, iout);
+    PatchArgs pa;
+    pa.start = x->pstart;
+    pa.face = x->pface;
+    pa.loc = x->ploc;
+    const size_t shm = (size_t)x->max_pf * x->nv * sizeof(double);
+    const bool second = x->cfg.order >= 2;
+    const bool roe = x->cfg.scheme == MST_SCHEME_ROE;
+#define MST_PATCH(MF)                                                                                          \
+    if (roe)                                                                                                   \
+        return second ? launch_shm(x, MST_K_PATCH, k_patch<DIM, MST_SCHEME_ROE, true, T, MF>, x->nc, shm, fa, a, pa) \
+                      : launch_shm(x, MST_K_PATCH, k_patch<DIM, MST_SCHEME_ROE, false, T, MF>, x->nc, shm, fa, a, pa); \
+    return second ? launch_shm(x, MST_K_PATCH, k_patch<DIM, MST_SCHEME_AUSM, true, T, MF>, x->nc, shm, fa, a, pa)    \
+                  : launch_shm(x, MST_K_PATCH, k_patch<DIM, MST_SCHEME_AUSM, false, T, MF>, x->nc, shm, fa, a, pa);
+    if (x->slots == 3) { MST_PATCH(3) }
+    MST_PATCH(4)
+#undef MST_PATCH
 }
 
 int resolve_strategy(mst_ctx *x);
@@ -305,12 +348,20 @@ int run_stage(mst_ctx *x, int mode, int is, int in, double ca, double cb, int st
         rc = run_grad(x, is, false);
         if (rc) return rc;
     }
-    if (x->mixed)
+    if (x->resolved == MST_STRATEGY_PATCH) {
+        if (x->mixed)
+            rc = x->dim == 2 ? run_patch_t<2, float>(x, mode, is, in, ca, cb, stage0, iout)
+                             : run_patch_t<3, float>(x, mode, is, in, ca, cb, stage0, iout);
+        else
+            rc = x->dim == 2 ? run_patch_t<2, double>(x, mode, is, in, ca, cb, stage0, iout)
+                             : run_patch_t<3, double>(x, mode, is, in, ca, cb, stage0, iout);
+    } else if (x->mixed) {
         rc = x->dim == 2 ? run_cell_t<2, float>(x, mode, is, in, ca, cb, stage0, iout)
                          : run_cell_t<3, float>(x, mode, is, in, ca, cb, stage0, iout);
-    else
+    } else {
         rc = x->dim == 2 ? run_cell_t<2, double>(x, mode, is, in, ca, cb, stage0, iout)
                          : run_cell_t<3, double>(x, mode, is, in, ca, cb, stage0, iout);
+    }
     if (rc == MST_OK && iout >= 0) x->shadow_ok[iout] = x->mixed;
     return rc;
 }
@@ -425,6 +476,52 @@ int build_incidence(mst_ctx *x) {
     return rc;
 }
 
+// Patch tables: for each block of BLOCK consecutive cells, the faces that carry a flux into one of its
+// cells (ascending; a face joining two of its cells once), and for each cell slot (local index << 1) |
+// is_c1, or -1 for an empty slot or a zero-flux face.  The kernel keeps a patch's fluxes in LDS
+// (NV doubles per face), so the strategy needs at most 4 slots and 64 KiB per patch.
+int build_patches(mst_ctx *x) {
+    const mst::Prepared &P = x->prep.P;
+    x->patch_ok = false;
+    if (x->slots > 4) return MST_OK;
+    const int64_t nc = x->nc, nb = (nc + BLOCK - 1) / BLOCK;
+    std::vector<int32_t> start(nb + 1, 0), face, list;
+    std::vector<int16_t> loc((size_t)x->slots * nc, -1);
+    face.reserve((size_t)(nc * 2));
+    size_t max_pf = 0;
+    for (int64_t b = 0; b < nb; b++) {
+        const int64_t ca = b * BLOCK, cz = std::min(nc, ca + BLOCK);
+        list.clear();
+        for (int s = 0; s < P.maxf; s++)
+            for (int64_t c = ca; c < cz; c++) {
+                const int32_t e = P.ell[(int64_t)s * nc + c];
+                if (e >= 0 && P.fkind[e >> 1] < mst::KIND_NONE) list.push_back(e >> 1);
+            }
+        std::sort(list.begin(), list.end());
+        list.erase(std::unique(list.begin(), list.end()), list.end());
+        max_pf = std::max(max_pf, list.size());
+        for (int s = 0; s < P.maxf; s++)
+            for (int64_t c = ca; c < cz; c++) {
+                const int32_t e = P.ell[(int64_t)s * nc + c];
+                if (e < 0 || P.fkind[e >> 1] >= mst::KIND_NONE) continue;
+                const int64_t l = std::lower_bound(list.begin(), list.end(), e >> 1) - list.begin();
+                loc[(size_t)s * nc + c] = (int16_t)((l << 1) | (e & 1));
+            }
+        start[b] = (int32_t)face.size();
+        face.insert(face.end(), list.begin(), list.end());
+    }
+    start[nb] = (int32_t)face.size();
+    if (max_pf * x->nv * sizeof(double) > 65536) return MST_OK;
+    int rc = dupload(&x->pstart, start);
+    if (!rc) rc = dupload(&x->pface, face);
+    if (!rc) rc = dupload(&x->ploc, loc);
+    if (rc) return rc;
+    x->max_pf = (int)max_pf;
+    x->n_psolve = (int64_t)face.size();
+    x->patch_ok = true;
+    return MST_OK;
+}
+
 // per incidence: FP64 eta of the face (with the last-interior-face quirk), FP32 the difference-form
 // weight (1 - eta on the c0 side, eta on the c1 side)
 int upload_incidence_weights(mst_ctx *x, const std::vector<double> &eta) {
@@ -489,7 +586,7 @@ void release(mst_ctx *x) {
                     x->cshalf, x->Q[0], x->Q[1], x->Q[2], x->Q[3], x->G, x->rec, x->graw, x->phi, x->R,
                     x->scal, x->part, x->fin_mid, x->bad, x->Q32[0], x->Q32[1], x->Q32[2], x->Q32[3], x->rec32,
                     x->frec0, x->frec1, x->inc_code, x->inc_w, x->inc_n, x->inc_dx, x->inc_w32, x->inc_n32,
-                    x->inc_dx32};
+                    x->inc_dx32, x->pstart, x->pface, x->ploc};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (int k = 0; k < MST_K_COUNT; k++)
@@ -548,7 +645,8 @@ int resolve_strategy(mst_ctx *x) {
     HIPCHK(hipEventCreate(&e1));
     float best = INFINITY;
     int pick = MST_STRATEGY_FACE, rc = MST_OK;
-    for (int s : {MST_STRATEGY_FACE, MST_STRATEGY_CELL}) {
+    for (int s : {MST_STRATEGY_FACE, MST_STRATEGY_CELL, MST_STRATEGY_PATCH}) {
+        if (s == MST_STRATEGY_PATCH && !x->patch_ok) continue;
         x->resolved = s;
         rc = run_stage(x, UPD_STAGE, x->iold, x->iold, 0.0, 1.0, 1, spare);  // warm-up
         if (rc) break;
@@ -665,6 +763,7 @@ int mst_ctx_create(const mst_mesh *m, const mst_config *cfg, mst_ctx **out) {
         return mst::fail(MST_E_HIP, "hipMemset failed");
     }
     TRY(build_incidence(x));
+    TRY(build_patches(x));
     TRY(upload_config(x));
 #undef TRY
     *out = x;
@@ -745,8 +844,11 @@ int mst_solve(mst_ctx *ctx) {
 
 int mst_set_strategy(mst_ctx *ctx, int strategy) {
     CTX_GUARD(ctx);
-    if (strategy != MST_STRATEGY_FACE && strategy != MST_STRATEGY_CELL && strategy != MST_STRATEGY_AUTO)
+    if (strategy != MST_STRATEGY_FACE && strategy != MST_STRATEGY_CELL && strategy != MST_STRATEGY_PATCH &&
+        strategy != MST_STRATEGY_AUTO)
         return mst::fail(MST_E_ARG, "bad strategy");
+    if (strategy == MST_STRATEGY_PATCH && !ctx->patch_ok)
+        return mst::fail(MST_E_UNSUPPORTED, "patch strategy: needs at most 4 faces per cell and 64 KiB of fluxes per patch");
     ctx->strategy = strategy;
     ctx->resolved = strategy == MST_STRATEGY_AUTO ? -1 : strategy;
     return MST_OK;
@@ -1117,6 +1219,10 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
             break;
         case MST_K_CELL:  // flux inputs + update, no face-flux array; FP64 reads the stage state once
             *bytes = flux_in + upd - (mixed ? 0 : (second ? nbnd : nc) * 8 * NV);
+            break;
+        case MST_K_PATCH:  // as the cell kernel, with the patch face lists and local slots for the table
+            *bytes = flux_in + upd - (mixed ? 0 : (second ? nbnd : nc) * 8 * NV) - ell +
+                     4.0 * (double)ctx->n_psolve + 2.0 * ctx->slots * nc;
             break;
         case MST_K_DT:
             *bytes = nc * (8 * NV + 8 + 8 * D);

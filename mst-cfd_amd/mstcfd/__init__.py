@@ -24,11 +24,13 @@ PREC_FP64, PREC_MIXED = 0, 1
 COMPAT_BRANCH2_LR, COMPAT_GRAD_LASTINT, COMPAT_AUSM_LOCAL_A, COMPAT_AUSM_P_UNSCALED = 1, 2, 4, 8
 COMPAT_AUSM_ONESIDED = 16
 COMPAT_REFERENCE = COMPAT_GRAD_LASTINT | COMPAT_AUSM_LOCAL_A | COMPAT_AUSM_P_UNSCALED | COMPAT_AUSM_ONESIDED
-K_GRAD, K_FLUX, K_UPDATE, K_DT, K_CELL = 0, 1, 2, 3, 4
-KERNEL_NAMES = {K_GRAD: "k_grad", K_FLUX: "k_flux", K_UPDATE: "k_update", K_DT: "k_cfl", K_CELL: "k_cell"}
-STRATEGY_FACE, STRATEGY_CELL, STRATEGY_AUTO = 0, 1, 2
+K_GRAD, K_FLUX, K_UPDATE, K_DT, K_CELL, K_PATCH = 0, 1, 2, 3, 4, 5
+KERNEL_NAMES = {K_GRAD: "k_grad", K_FLUX: "k_flux", K_UPDATE: "k_update", K_DT: "k_cfl", K_CELL: "k_cell",
+                K_PATCH: "k_patch"}
+STRATEGY_FACE, STRATEGY_CELL, STRATEGY_AUTO, STRATEGY_PATCH = 0, 1, 2, 3
 INIT_REFERENCE, INIT_UNIFORM, INIT_RIEMANN, INIT_VORTEX, INIT_PULSE, INIT_FREESTREAM = 0, 1, 2, 3, 4, 5
 BATCH_BUCKETS = 4
+ORDER_HILBERT, ORDER_RCM = 0, 1
 
 # public exported symbols of include/mstcfd.h (checked by the CPU tests)
 EXPORTS = [
@@ -47,7 +49,8 @@ EXPORTS = [
     "mst_mlp_create", "mst_mlp_destroy", "mst_mlp_num_params", "mst_mlp_get_params", "mst_mlp_set_params",
     "mst_mlp_set_scaling", "mst_mlp_set_data", "mst_mlp_reset_optimizer", "mst_mlp_train", "mst_mlp_forward",
     "mst_mlp_gradient", "mst_ctx_warm_start", "mst_run_to_steady", "mst_batch_run_to_steady",
-    "mst_mesh_cell_centres", "mst_batch_set_strategy", "mst_batch_get_strategy",
+    "mst_mesh_cell_centres", "mst_batch_set_strategy", "mst_batch_get_strategy", "mst_set_cell_order",
+    "mst_get_cell_order",
 ]
 
 
@@ -117,6 +120,8 @@ def lib():
                                        C.c_uint64, I32P, pp]),
         "mst_mesh_gen_tet": (C.c_int, [C.c_int, C.c_int, C.c_int, F64P, F64P, C.c_double, C.c_uint64, I32P, pp]),
         "mst_prep_create": (C.c_int, [vp, pp]),
+        "mst_set_cell_order": (C.c_int, [C.c_int]),
+        "mst_get_cell_order": (C.c_int, [C.POINTER(C.c_int)]),
         "mst_prep_destroy": (None, [vp]),
         "mst_prep_counts": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "mst_prep_permutation": (C.c_int, [vp, I32P, I32P, I32P]),
@@ -464,6 +469,17 @@ class Solver:
         b = C.c_double()
         _check(lib().mst_algorithmic_bytes(self.h, int(kernel), C.byref(b)), "mst_algorithmic_bytes")
         return b.value
+
+
+def set_cell_order(order):
+    """Cell order of every later Prep / Solver / Batch creation in this process (ORDER_HILBERT or ORDER_RCM)."""
+    _check(lib().mst_set_cell_order(int(order)), "mst_set_cell_order")
+
+
+def get_cell_order():
+    o = C.c_int()
+    _check(lib().mst_get_cell_order(C.byref(o)), "mst_get_cell_order")
+    return o.value
 
 
 def copy_probe(nbytes):

@@ -133,7 +133,7 @@ def test_gradient_and_limiter_parity(mname, lim):
     assert perr <= 1e-9
 
 
-STRATEGIES = [M.STRATEGY_FACE, M.STRATEGY_CELL]
+STRATEGIES = [M.STRATEGY_FACE, M.STRATEGY_CELL, M.STRATEGY_PATCH]
 
 
 @pytest.mark.parametrize("strategy", STRATEGIES)
@@ -226,8 +226,9 @@ def test_monitor_maxrel_matches_reference_rule():
 @pytest.mark.parametrize("mname,scheme,lim", [("periodic", ROE, M.LIM_VANLEER), ("allbc", AUSM, M.LIM_MINMOD),
                                               ("tet", ROE, M.LIM_MINMOD)])
 def test_cell_strategy_matches_face_strategy(mname, scheme, lim, precision):
-    """The fused cell kernel solves each interior face twice from the same memory inputs with the same
-    instructions, so it reproduces the face split (same fluxes, same summation order)."""
+    """The fused cell kernel solves each interior face twice, the patch kernel each face of a patch
+    once (a border face in both patches), always from the same memory inputs with the same
+    instructions, so both reproduce the face split bit for bit (same fluxes, same summation order)."""
     m, om, geo = mesh(mname)
     dim = m.dim
     cg = M.make_config(dim=dim, scheme=scheme, order=2, limiter=lim, rk=3, cfl=0.45, q_inlet=inlet(dim),
@@ -241,10 +242,12 @@ def test_cell_strategy_matches_face_strategy(mname, scheme, lim, precision):
         s.set_state(Q)
         s.step(20)
         out.append((s.get_state(), s.residual_norms()))
-    err = rel_state_error(out[1][0], out[0][0])
-    print(f"{mname} prec={precision}: cell vs face after 20 RK3 steps {err:.2e}", flush=True)
-    assert err <= 1e-14
-    assert np.allclose(out[1][1], out[0][1], rtol=1e-12, atol=0)
+    for k in range(1, len(STRATEGIES)):
+        err = rel_state_error(out[k][0], out[0][0])
+        print(f"{mname} prec={precision}: strategy {STRATEGIES[k]} vs face after 20 RK3 steps {err:.2e}",
+              flush=True)
+        assert np.array_equal(out[k][0], out[0][0])
+        assert np.array_equal(out[k][1], out[0][1])
 
 
 def test_auto_strategy_resolves_and_matches():
@@ -257,7 +260,7 @@ def test_auto_strategy_resolves_and_matches():
     assert s.strategy == M.STRATEGY_AUTO
     s.set_state(Q)
     s.step(5)
-    assert s.strategy in (M.STRATEGY_FACE, M.STRATEGY_CELL)
+    assert s.strategy in (M.STRATEGY_FACE, M.STRATEGY_CELL, M.STRATEGY_PATCH)
     f = M.Solver(m, cg)
     f.strategy = M.STRATEGY_FACE
     f.set_state(Q)
@@ -268,7 +271,7 @@ def test_auto_strategy_resolves_and_matches():
     assert s.strategy == M.STRATEGY_AUTO
 
 
-@pytest.mark.parametrize("strategy", [M.STRATEGY_FACE, M.STRATEGY_CELL])
+@pytest.mark.parametrize("strategy", STRATEGIES)
 def test_kernel_timing_counts_and_leaves_results_unchanged(strategy):
     """Per-kernel timing (events carried by the dispatches) counts one interval per launch, each
     positive, their sum below the wall time of the steps, and changes no result bit."""
@@ -292,9 +295,9 @@ def test_kernel_timing_counts_and_leaves_results_unchanged(strategy):
     wall_ms = 1e3 * (time.perf_counter() - t0)
     s.timing(False)
     assert np.array_equal(s.get_state(), plain.get_state())
-    fused = strategy == M.STRATEGY_CELL
-    want = {M.K_GRAD: 12, M.K_FLUX: 0 if fused else 12, M.K_UPDATE: 0 if fused else 12,
-            M.K_CELL: 12 if fused else 0, M.K_DT: 1}  # later steps take dt from the last stage's update
+    face, cell, patch = (strategy == st for st in STRATEGIES)
+    want = {M.K_GRAD: 12, M.K_FLUX: 12 * face, M.K_UPDATE: 12 * face, M.K_CELL: 12 * cell,
+            M.K_PATCH: 12 * patch, M.K_DT: 1}  # later steps take dt from the last stage's update
     total = 0.0
     for k, n_want in want.items():
         n, ms = s.timing_get(k)

@@ -146,8 +146,18 @@ def test_jitter_bounds_and_periodic_match():
     assert np.array_equal(g["S"][per], -g["S"][p[per]])  # exactly opposite area vectors
 
 
+@pytest.fixture
+def cell_order():
+    """Sets the process-wide cell order for one test and restores it afterwards."""
+    old = M.get_cell_order()
+    yield M.set_cell_order
+    M.set_cell_order(old)
+
+
+@pytest.mark.parametrize("order", [M.ORDER_RCM, M.ORDER_HILBERT])
 @pytest.mark.parametrize("shuffle", [False, True])
-def test_rcm_permutation_colouring_tables(shuffle):
+def test_rcm_permutation_colouring_tables(shuffle, order, cell_order):
+    cell_order(order)
     m = M.Mesh.gen_tri(40, 30, 0, 4, 0, 3, 0.1, 3, (M.BC_INLET, M.BC_OUTLET, M.BC_PERIODIC, M.BC_PERIODIC))
     raw = m.raw()
     if shuffle:  # relabel cells randomly: RCM must recover a narrow band
@@ -194,10 +204,84 @@ def test_rcm_permutation_colouring_tables(shuffle):
             assert key not in seen
             seen[key] = f
     assert p.ncolours <= 5
-    # bandwidth of the new order vs the input order
-    bw_new = np.abs(fc0[:p.n_interior] - fc1[:p.n_interior]).max()
-    bw_old = np.abs(raw["c0"][inter] - raw["c1"][inter]).max()
-    assert bw_new <= (bw_old if not shuffle else bw_old // 10)
+    if order == M.ORDER_RCM:  # bandwidth of the new order vs the input order
+        bw_new = np.abs(fc0[:p.n_interior] - fc1[:p.n_interior]).max()
+        bw_old = np.abs(raw["c0"][inter] - raw["c1"][inter]).max()
+        assert bw_new <= (bw_old if not shuffle else bw_old // 10)
+    else:  # the order depends on the cell centres alone, not on the input labels
+        assert block_internal_fraction(p) > 0.6
+
+
+def block_internal_fraction(p, block=256):
+    """Fraction of interior faces whose two cells lie in the same block of `block` consecutive cells."""
+    fc0, fc1, kind, _ = p.tables()
+    i = kind == 0
+    return float(((fc0[i] // block) == (fc1[i] // block)).mean())
+
+
+def quad_grid(n):
+    """n x n unit squares, interior faces first, walls around."""
+    nid = lambda i, j: j * (n + 1) + i
+    cid = lambda i, j: j * n + i
+    xy = np.array([(i, j) for j in range(n + 1) for i in range(n + 1)], np.float64)
+    fn, c0, c1 = [], [], []
+    for j in range(n):
+        for i in range(n - 1):
+            fn.append((nid(i + 1, j), nid(i + 1, j + 1)))
+            c0.append(cid(i, j))
+            c1.append(cid(i + 1, j))
+    for j in range(n - 1):
+        for i in range(n):
+            fn.append((nid(i, j + 1), nid(i + 1, j + 1)))
+            c0.append(cid(i, j))
+            c1.append(cid(i, j + 1))
+    nint = len(fn)
+    for k in range(n):
+        for a, b, c in ((nid(k, 0), nid(k + 1, 0), cid(k, 0)), (nid(k, n), nid(k + 1, n), cid(k, n - 1)),
+                        (nid(0, k), nid(0, k + 1), cid(0, k)), (nid(n, k), nid(n, k + 1), cid(n - 1, k))):
+            fn.append((a, b))
+            c0.append(c)
+            c1.append(-1)
+    nf = len(fn)
+    return dict(dim=2, xyz=xy, fnodes=np.array(fn, np.int32), c0=np.array(c0, np.int32), c1=np.array(c1, np.int32),
+                ftype=np.array([M.BC_INTERIOR] * nint + [M.BC_WALL] * (nf - nint), np.int32),
+                partner=np.full(nf, -1, np.int32), n_cells=n * n)
+
+
+@pytest.mark.parametrize("n", [2, 4, 16])
+def test_hilbert_order_is_a_unit_step_path_on_a_square_grid(n, cell_order):
+    """On a 2^k x 2^k grid the Hilbert curve visits every cell once, each step to an edge neighbour,
+    starting in the (0, 0) corner; RCM's diagonal fronts take longer steps."""
+    cell_order(M.ORDER_HILBERT)
+    m = M.Mesh.from_raw(quad_grid(n))
+    cn, _, _ = M.Prep(m).permutation()
+    old_of_new = np.argsort(cn)
+    ij = np.array([(c % n, c // n) for c in old_of_new])
+    step = np.abs(np.diff(ij, axis=0)).sum(1)
+    assert (step == 1).all()
+    assert tuple(ij[0]) == (0, 0)
+    if n >= 4:
+        cell_order(M.ORDER_RCM)
+        cn, _, _ = M.Prep(m).permutation()
+        ij = np.array([(c % n, c // n) for c in np.argsort(cn)])
+        assert (np.abs(np.diff(ij, axis=0)).sum(1) > 1).any()
+
+
+def test_hilbert_blocks_are_compact_in_3d(cell_order):
+    """Tetrahedra: most interior faces join two cells of one 256-cell block along the Hilbert curve,
+    few do along RCM's level sets."""
+    m = M.Mesh.gen_tet(10, 10, 10, (0, 0, 0), (1, 1, 1), 0.1, 5, (M.BC_WALL,) * 6)
+    cell_order(M.ORDER_HILBERT)
+    hil = block_internal_fraction(M.Prep(m))
+    cell_order(M.ORDER_RCM)
+    rcm = block_internal_fraction(M.Prep(m))
+    assert hil > 0.6 and hil > rcm + 0.2, (hil, rcm)
+
+
+def test_cell_order_setting_rejects_unknown_values(cell_order):
+    with pytest.raises(M.MstError):
+        M.set_cell_order(7)
+    assert M.get_cell_order() in (M.ORDER_HILBERT, M.ORDER_RCM)
 
 
 def test_quad_cells_geometry_matches_oracle():
