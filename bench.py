@@ -221,9 +221,10 @@ def main():
     ap.add_argument("--workload", default="vortex2d")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--retune", action="store_true", help="batch workload: ignore the tuning cache")
-    ap.add_argument("--strategy", choices=("auto", "face", "cell", "patch"), default="auto",
+    ap.add_argument("--strategy", choices=("auto", "face", "cell", "patch", "fused"), default="auto",
                     help="face: flux kernel + gather/update kernel; cell: fused cell kernel; patch: fused patch "
-                         "kernel (face fluxes of a 256-cell patch in LDS); auto: time each and report the fastest")
+                         "kernel (face fluxes of a 256-cell patch in LDS); fused: the patch kernel with the "
+                         "gradients and face states in LDS too; auto: time each and report the fastest")
     ap.add_argument("--order", choices=("hilbert", "rcm"), default="hilbert",
                     help="cell order of the device layout (Hilbert-curve patches or reverse Cuthill-McKee)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison")
@@ -253,8 +254,8 @@ def main():
     solver = M.Solver(mesh, M.make_config(**cfg_kw))
     Q0 = init()
     stages = cfg_kw["rk"]
-    names = {"face": M.STRATEGY_FACE, "cell": M.STRATEGY_CELL, "patch": M.STRATEGY_PATCH}
-    tried = ("face", "cell", "patch") if args.strategy == "auto" else (args.strategy,)
+    names = {"face": M.STRATEGY_FACE, "cell": M.STRATEGY_CELL, "patch": M.STRATEGY_PATCH, "fused": M.STRATEGY_FUSED}
+    tried = tuple(names) if args.strategy == "auto" else (args.strategy,)
     runs = {}
     for st in tried:  # each strategy on the same context from the same initial field
         solver.strategy = names[st]

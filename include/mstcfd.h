@@ -195,7 +195,9 @@ int mst_step(mst_ctx *ctx, int nsteps);
  * on both sides) and updates, with no face-flux array in memory.  PATCH: one kernel per block of 256
  * consecutive cells (a compact patch in the Hilbert cell order) solves the patch's faces once into
  * LDS, then sums and updates per cell; available for cells of at most 4 faces (MST_E_UNSUPPORTED
- * otherwise).  All give the same result bit for bit.  AUTO (default): the first stage after creation
+ * otherwise).  FUSED (order 2; order 1 runs PATCH): the patch kernel also computes the limited
+ * gradients of its cells and of its halo and keeps the face states in LDS, one kernel per stage.
+ * All give the same result bit for bit.  AUTO (default): the first stage after creation
  * or a configuration change times one stage of each available strategy on the context's own mesh
  * and keeps the fastest; mst_get_strategy then reports it.  mst_face_flux / mst_residual_update keep
  * the face split whatever the strategy. */
@@ -203,6 +205,7 @@ int mst_step(mst_ctx *ctx, int nsteps);
 #define MST_STRATEGY_CELL 1
 #define MST_STRATEGY_AUTO 2
 #define MST_STRATEGY_PATCH 3
+#define MST_STRATEGY_FUSED 4
 int mst_set_strategy(mst_ctx *ctx, int strategy);
 int mst_get_strategy(const mst_ctx *ctx, int *strategy);
 /* NEW: steady driver.  Steps of mst_step until max_f L2(R/V)_f / ref_f <= tol (checked every
@@ -314,7 +317,8 @@ int mst_ctx_warm_start(mst_ctx *ctx, mst_mlp *m, const double *cond, int n_cond)
 #define MST_K_DT 3
 #define MST_K_CELL 4 /* fused flux + gather + update (MST_STRATEGY_CELL) */
 #define MST_K_PATCH 5 /* patch fluxes in LDS + gather + update (MST_STRATEGY_PATCH) */
-#define MST_K_COUNT 6
+#define MST_K_FUSED 6 /* gradients + face states + fluxes in LDS + gather + update (MST_STRATEGY_FUSED) */
+#define MST_K_COUNT 7
 int mst_timing_enable(mst_ctx *ctx, int on);
 /* per kernel class: launches timed and total milliseconds since the last reset */
 int mst_timing_get(mst_ctx *ctx, int kernel, int64_t *count, double *total_ms);

@@ -201,10 +201,12 @@ struct GradArgs {
 // the c1 side), equal because the outward normals of a closed cell sum to zero, and free of the
 // cancellation an FP32 sum of whole states would suffer.
 template <int DIM, int MAXF, int LIM, typename T>
-__device__ __forceinline__ void grad_cell(const GradArgs<T> &a, const int64_t c) {
+__device__ __forceinline__ int grad_limited(const GradArgs<T> &a, const int64_t c, T (&qc)[DIM + 2],
+                                           T (&lgr)[(DIM + 2) * DIM], T (&dxs)[MAXF][DIM], T (&t)[DIM + 2][DIM],
+                                           T (&ph)[DIM + 2]) {
     constexpr int NV = DIM + 2;
     constexpr bool F32 = std::is_same<T, float>::value;
-    T qc[NV], qmax[NV], qmin[NV], t[NV][DIM];
+    T qmax[NV], qmin[NV];
     ldq<NV>(a.Q + c * NV, qc);
 #pragma unroll
     for (int k = 0; k < NV; k++) {
@@ -212,7 +214,6 @@ __device__ __forceinline__ void grad_cell(const GradArgs<T> &a, const int64_t c)
 #pragma unroll
         for (int d = 0; d < DIM; d++) t[k][d] = 0;
     }
-    T dxs[MAXF][DIM];
     int nfc = 0;
 #pragma unroll
     for (int s = 0; s < MAXF; s++) {
@@ -255,7 +256,6 @@ __device__ __forceinline__ void grad_cell(const GradArgs<T> &a, const int64_t c)
         nfc = s + 1;
     }
     const T iV = (T)frcp(a.vol[c]);
-    T ph[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         ph[k] = 1;
@@ -296,11 +296,19 @@ __device__ __forceinline__ void grad_cell(const GradArgs<T> &a, const int64_t c)
                 ph[k] = fmin(T(1), fdiv(T(2) * bn[k], bd[k] + bn[k]));  // (r+|r|)/(1+|r|) at r = bn/bd >= 0
         }
     }
-    T lgr[NV * DIM];  // limited gradient
 #pragma unroll
     for (int k = 0; k < NV; k++)
 #pragma unroll
         for (int d = 0; d < DIM; d++) lgr[k * DIM + d] = ph[k] * t[k][d];
+    return nfc;
+}
+
+// grad_limited, then the cell's face records and the optional raw gradient / limiter outputs
+template <int DIM, int MAXF, int LIM, typename T>
+__device__ __forceinline__ void grad_cell(const GradArgs<T> &a, const int64_t c) {
+    constexpr int NV = DIM + 2;
+    T qc[NV], lgr[NV * DIM], dxs[MAXF][DIM], t[NV][DIM], ph[NV];
+    const int nfc = grad_limited<DIM, MAXF, LIM, T>(a, c, qc, lgr, dxs, t, ph);
     // reconstructed states at the cell's faces (RhoSolver.cpp:250,254 with the limiter), one record
     // per incidence, coalesced over the cells of a slot
 #pragma unroll
@@ -633,6 +641,166 @@ __global__ __launch_bounds__(BLOCK) void k_patch(const FluxArgs<T> fa, const Upd
             double g[NV];
 #pragma unroll
             for (int k = 0; k < NV; k++) g[k] = gsh[k * npf + (l >> 1)];
+            accumulate<NV, F32>(r, cmp, g, l & 1);
+        }
+        if constexpr (F32) {
+#pragma unroll
+            for (int k = 0; k < NV; k++) r[k] += cmp[k];
+        }
+    }
+    stage_tail<NV, true, F32>(a, c, valid, r, blk, a.l2, a.dtred);
+}
+
+// Order 2 in one kernel: the patch also computes the limited gradients of its own cells and of its
+// halo (the cells across its border faces), and keeps the reconstructed face states in LDS instead
+// of the global record array, so a stage reads the state and the geometry once and writes only the
+// new state.  The halo's gradients are computed by every patch that borders them (a quarter more
+// gradient work in 2D), from the same inputs with the same instructions as the gradient kernel, so
+// the records, fluxes and sums are bit-identical to the other strategies'.
+struct HaloArgs {
+    const int32_t *__restrict__ start;  // nblk + 1: each block's range in cell
+    const int32_t *__restrict__ cell;   // halo cells of each patch
+    const int16_t *__restrict__ loc;    // per halo entry, MAXF slots: (local face << 1) | side, -1: not in the patch
+};
+
+// LDS of k_fused for npf faces: the two sides' face states, T[2][NV][npf], and the fluxes,
+// double[NV][npf]; FP64 fluxes overwrite the c0-side states of their own face (only the lane that
+// solves a face reads its states), FP32 ones follow the states.  16 * NV * npf bytes either way.
+template <typename T>
+struct FusedLds {
+    T *rec;
+    double *g;
+    int npf;
+    __device__ FusedLds(double *sh, int npf_, int nv) : npf(npf_) {
+        rec = reinterpret_cast<T *>(sh);
+        g = std::is_same<T, double>::value ? sh : sh + nv * npf_;
+    }
+    __device__ T &r(int side, int k, int l, int nv) { return rec[(side * nv + k) * npf + l]; }
+};
+
+// Resident waves per SIMD requested for the fused kernel: 4 (at most 128 VGPRs) where it fits without
+// spilling (2D with FP32 arithmetic, Roe, or unlimited AUSM: 96-123 VGPRs); there the compiler would
+// keep 3 waves and the kernel, bound by its dependent load rounds, runs about 15 % slower.  Elsewhere
+// the bound would spill, so the compiler chooses.
+template <int DIM, int SCHEME, int LIM, typename T>
+constexpr int fused_waves() {
+    return (DIM == 2 && (sizeof(T) == 4 || SCHEME == MST_SCHEME_ROE || LIM == MST_LIM_NONE)) ? 4 : 1;
+}
+
+template <int DIM, int SCHEME, int LIM, typename T, int MAXF>
+__global__ __launch_bounds__(BLOCK, (fused_waves<DIM, SCHEME, LIM, T>())) void k_fused(const GradArgs<T> ga, const FluxArgs<T> fa, const UpdArgs a,
+                                                 const PatchArgs p, const HaloArgs h) {
+    constexpr int NV = DIM + 2;
+    constexpr bool F32 = std::is_same<T, float>::value;
+    extern __shared__ double sh[];
+    const int64_t blk = block_id();
+    const int32_t f0 = p.start[blk];
+    const int npf = p.start[blk + 1] - f0;
+    FusedLds<T> L(sh, npf, NV);
+    // Inputs of the later phases that do not depend on the gradients are fetched first, so their
+    // latency overlaps the gradient phase: each lane's first PF patch faces (face, kind, normal) and
+    // its cell's slot entries.
+    constexpr int PF = 2;
+    int64_t pf_f[PF];
+    int pf_info[PF];
+    double pf_n[PF][DIM];
+#pragma unroll
+    for (int q = 0; q < PF; q++) {
+        const int i = threadIdx.x + q * BLOCK;
+        pf_f[q] = -1;
+        pf_info[q] = 0;
+        if (i < npf) {
+            pf_f[q] = p.face[f0 + i];
+            pf_info[q] = fa.finfo[pf_f[q]];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) pf_n[q][d] = fa.fn[pf_f[q] * DIM + d];
+        }
+    }
+    const int64_t c = blk * BLOCK + threadIdx.x;
+    const bool valid = c < a.nc;
+    int cl[MAXF];
+#pragma unroll
+    for (int s = 0; s < MAXF; s++) cl[s] = valid ? p.loc[(int64_t)s * a.nc + c] : -1;
+    // limited gradients of the patch cells, then of the halo cells; face states into LDS
+    const int32_t h0 = h.start[blk];
+    const int next = BLOCK + (h.start[blk + 1] - h0);
+    for (int i = threadIdx.x; i < next; i += BLOCK) {
+        int64_t c;
+        const int16_t *lp;
+        int64_t stride;
+        if (i < BLOCK) {
+            c = blk * BLOCK + i;
+            if (c >= a.nc) continue;
+            lp = p.loc + c;
+            stride = a.nc;
+        } else {
+            c = h.cell[h0 + i - BLOCK];
+            lp = h.loc + (int64_t)(h0 + i - BLOCK) * MAXF;
+            stride = 1;
+        }
+        T qc[NV], lgr[NV * DIM], dxs[MAXF][DIM], t[NV][DIM], ph[NV];
+        const int nfc = grad_limited<DIM, MAXF, LIM, T>(ga, c, qc, lgr, dxs, t, ph);
+#pragma unroll
+        for (int s = 0; s < MAXF; s++) {
+            if (s >= nfc) break;
+            const int l = lp[s * stride];
+            if (l < 0) continue;
+            T q[NV];
+#pragma unroll
+            for (int k = 0; k < NV; k++) q[k] = qc[k];
+            recon<DIM>(q, lgr, dxs[s]);
+#pragma unroll
+            for (int k = 0; k < NV; k++) L.r(l & 1, k, l >> 1, NV) = q[k];
+        }
+    }
+    __syncthreads();
+    // the patch's face fluxes (face_load's inputs from LDS: both sides' states, or for a boundary
+    // face the raw interior state from memory and its reconstructed state)
+    for (int q = 0, i = threadIdx.x; i < npf; q++, i += BLOCK) {
+        int64_t f;
+        int info;
+        double n64[DIM];
+        if (q < PF) {
+            f = pf_f[q];
+            info = pf_info[q];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) n64[d] = pf_n[q][d];
+        } else {
+            f = p.face[f0 + i];
+            info = fa.finfo[f];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) n64[d] = fa.fn[f * DIM + d];
+        }
+        const int kind = info & 15;
+        T X[NV], Y[NV];
+        if (kind == mst::KIND_INTERIOR || kind == mst::KIND_PERIODIC) {
+#pragma unroll
+            for (int k = 0; k < NV; k++) {
+                X[k] = L.r(0, k, i, NV);
+                Y[k] = L.r(1, k, i, NV);
+            }
+        } else {
+            ldq<NV>(fa.Q + (int64_t)fa.fc0[f] * NV, X);
+#pragma unroll
+            for (int k = 0; k < NV; k++) Y[k] = L.r(0, k, i, NV);
+        }
+        double g[NV];
+        face_solve<DIM, SCHEME, T>(fa.P, SCHEME == MST_SCHEME_ROE, kind, info >> 4, n64, X, Y, g);
+#pragma unroll
+        for (int k = 0; k < NV; k++) L.g[k * npf + i] = g[k];
+    }
+    __syncthreads();
+    double r[NV], cmp[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) r[k] = cmp[k] = 0;
+    if (valid) {
+#pragma unroll
+        for (int s = 0; s < MAXF; s++) {
+            const int l = cl[s];
+            if (l < 0) continue;
+            double g[NV];
+#pragma unroll
+            for (int k = 0; k < NV; k++) g[k] = L.g[k * npf + (l >> 1)];
             accumulate<NV, F32>(r, cmp, g, l & 1);
         }
         if constexpr (F32) {

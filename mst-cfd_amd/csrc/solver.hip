@@ -72,6 +72,10 @@ struct mst_ctx {
     int max_pf = 0;        // most faces of one patch (sizes the kernel's LDS)
     int64_t n_psolve = 0;  // Riemann problems per patch launch
     bool patch_ok = false;
+    // fused strategy: each patch's halo cells (across its border faces) and their slots' local faces
+    int32_t *hstart = nullptr, *hcell = nullptr;
+    int16_t *hloc = nullptr;
+    int64_t n_halo = 0;
     // mixed form: FP32 shadows of the states, FP32 geometry, offsets, face fluxes, limited gradients
     float *Q32[4] = {nullptr, nullptr, nullptr, nullptr};
     bool shadow_ok[4] = {false, false, false, false};
@@ -142,8 +146,8 @@ int ensure_shadow(mst_ctx *x, int i) {
     return MST_OK;
 }
 
-template <int DIM, typename T>
-int run_grad_t(mst_ctx *x, int isrc, bool debug) {
+template <typename T>
+GradArgs<T> grad_args(const mst_ctx *x, int isrc, bool debug) {
     GradArgs<T> a;
     a.nc = x->nc;
     a.inc_code = x->inc_code;
@@ -163,6 +167,12 @@ int run_grad_t(mst_ctx *x, int isrc, bool debug) {
     a.vol = x->cvol;
     a.graw = debug ? x->graw : nullptr;
     a.phi = debug ? x->phi : nullptr;
+    return a;
+}
+
+template <int DIM, typename T>
+int run_grad_t(mst_ctx *x, int isrc, bool debug) {
+    const GradArgs<T> a = grad_args<T>(x, isrc, debug);
     const int lim = x->cfg.limiter;
 #define MST_GRAD(MF)                                                                                    \
     if (lim == MST_LIM_NONE) return launch(x, MST_K_GRAD, k_grad<DIM, MF, MST_LIM_NONE, T>, x->nc, a);     \
@@ -329,6 +339,41 @@ int run_patch_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int 
 #undef MST_PATCH
 }
 
+// fused strategy (order 2): gradients, face states, fluxes, sums and update of a patch in one kernel
+template <int DIM, typename T>
+int run_fused_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
+    if (x->cfg.order < 2) return run_patch_t<DIM, T>(x, mode, is, in, ca, cb, stage0, iout);
+    if (!x->patch_ok) return mst::fail(MST_E_UNSUPPORTED, "no patch tables for this mesh");
+    const GradArgs<T> ga = grad_args<T>(x, is, false);
+    const FluxArgs<T> fa = flux_args<T>(x, is);
+    const UpdArgs a = upd_args(x, mode, is, in, ca, cb, stage0, iout);
+    PatchArgs pa;
+    pa.start = x->pstart;
+    pa.face = x->pface;
+    pa.loc = x->ploc;
+    HaloArgs ha;
+    ha.start = x->hstart;
+    ha.cell = x->hcell;
+    ha.loc = x->hloc;
+    const size_t shm = (size_t)16 * x->nv * x->max_pf;
+    const bool roe = x->cfg.scheme == MST_SCHEME_ROE;
+    const int lim = x->cfg.limiter;
+#define MST_FUSED_L(SC, LM, MF) \
+    return launch_shm(x, MST_K_FUSED, k_fused<DIM, SC, LM, T, MF>, x->nc, shm, ga, fa, a, pa, ha);
+#define MST_FUSED_S(SC, MF)                                                                   \
+    if (lim == MST_LIM_NONE) { MST_FUSED_L(SC, MST_LIM_NONE, MF) }                            \
+    if (lim == MST_LIM_MINMOD) { MST_FUSED_L(SC, MST_LIM_MINMOD, MF) }                        \
+    MST_FUSED_L(SC, MST_LIM_VANLEER, MF)
+#define MST_FUSED(MF)                          \
+    if (roe) { MST_FUSED_S(MST_SCHEME_ROE, MF) } \
+    MST_FUSED_S(MST_SCHEME_AUSM, MF)
+    if (x->slots == 3) { MST_FUSED(3) }
+    MST_FUSED(4)
+#undef MST_FUSED
+#undef MST_FUSED_S
+#undef MST_FUSED_L
+}
+
 int resolve_strategy(mst_ctx *x);
 
 // one residual evaluation + stage update (mode includes UPD_STAGE) with the context's strategy
@@ -344,6 +389,16 @@ int run_stage(mst_ctx *x, int mode, int is, int in, double ca, double cb, int st
     }
     int rc = ensure_shadow(x, is);
     if (rc) return rc;
+    if (x->resolved == MST_STRATEGY_FUSED) {
+        if (x->mixed)
+            rc = x->dim == 2 ? run_fused_t<2, float>(x, mode, is, in, ca, cb, stage0, iout)
+                             : run_fused_t<3, float>(x, mode, is, in, ca, cb, stage0, iout);
+        else
+            rc = x->dim == 2 ? run_fused_t<2, double>(x, mode, is, in, ca, cb, stage0, iout)
+                             : run_fused_t<3, double>(x, mode, is, in, ca, cb, stage0, iout);
+        if (rc == MST_OK && iout >= 0) x->shadow_ok[iout] = x->mixed;
+        return rc;
+    }
     if (x->cfg.order >= 2) {
         rc = run_grad(x, is, false);
         if (rc) return rc;
@@ -478,15 +533,16 @@ int build_incidence(mst_ctx *x) {
 
 // Patch tables: for each block of BLOCK consecutive cells, the faces that carry a flux into one of its
 // cells (ascending; a face joining two of its cells once), and for each cell slot (local index << 1) |
-// is_c1, or -1 for an empty slot or a zero-flux face.  The kernel keeps a patch's fluxes in LDS
-// (NV doubles per face), so the strategy needs at most 4 slots and 64 KiB per patch.
+// is_c1, or -1 for an empty slot or a zero-flux face; and each patch's halo for the fused kernel.
+// The kernels keep a patch's face states and fluxes in LDS, so the strategies need at most 4 slots and
+// 16 * NV bytes per face of the largest patch within 64 KiB.
 int build_patches(mst_ctx *x) {
     const mst::Prepared &P = x->prep.P;
     x->patch_ok = false;
     if (x->slots > 4) return MST_OK;
     const int64_t nc = x->nc, nb = (nc + BLOCK - 1) / BLOCK;
-    std::vector<int32_t> start(nb + 1, 0), face, list;
-    std::vector<int16_t> loc((size_t)x->slots * nc, -1);
+    std::vector<int32_t> start(nb + 1, 0), face, list, hstart(nb + 1, 0), hcell, halo;
+    std::vector<int16_t> loc((size_t)x->slots * nc, -1), hloc;
     face.reserve((size_t)(nc * 2));
     size_t max_pf = 0;
     for (int64_t b = 0; b < nb; b++) {
@@ -509,13 +565,42 @@ int build_patches(mst_ctx *x) {
             }
         start[b] = (int32_t)face.size();
         face.insert(face.end(), list.begin(), list.end());
+        // halo: the other cell of every patch face with one cell outside the patch, and for each of its
+        // slots the local index of the slot's face when the face is one of the patch's
+        halo.clear();
+        for (const int32_t f : list) {
+            if (P.fkind[f] != mst::KIND_INTERIOR && P.fkind[f] != mst::KIND_PERIODIC) continue;
+            for (const int32_t c : {P.fc0[f], P.fc1[f]})
+                if (c < ca || c >= cz) halo.push_back(c);
+        }
+        std::sort(halo.begin(), halo.end());
+        halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+        hstart[b] = (int32_t)hcell.size();
+        for (const int32_t c : halo) {
+            hcell.push_back(c);
+            for (int s = 0; s < x->slots; s++) {
+                const int32_t e = s < P.maxf ? P.ell[(int64_t)s * nc + c] : -1;
+                int16_t v = -1;
+                if (e >= 0 && P.fkind[e >> 1] < mst::KIND_NONE) {
+                    const auto it = std::lower_bound(list.begin(), list.end(), e >> 1);
+                    if (it != list.end() && *it == (e >> 1)) v = (int16_t)(((it - list.begin()) << 1) | (e & 1));
+                }
+                hloc.push_back(v);
+            }
+        }
     }
     start[nb] = (int32_t)face.size();
-    if (max_pf * x->nv * sizeof(double) > 65536) return MST_OK;
+    hstart[nb] = (int32_t)hcell.size();
+    // the fused kernel keeps 16 * NV bytes per patch face in LDS (the patch kernel half of that)
+    if (max_pf * x->nv * 16 > 65536) return MST_OK;
     int rc = dupload(&x->pstart, start);
     if (!rc) rc = dupload(&x->pface, face);
     if (!rc) rc = dupload(&x->ploc, loc);
+    if (!rc) rc = dupload(&x->hstart, hstart);
+    if (!rc) rc = dupload(&x->hcell, hcell);
+    if (!rc) rc = dupload(&x->hloc, hloc);
     if (rc) return rc;
+    x->n_halo = (int64_t)hcell.size();
     x->max_pf = (int)max_pf;
     x->n_psolve = (int64_t)face.size();
     x->patch_ok = true;
@@ -586,7 +671,7 @@ void release(mst_ctx *x) {
                     x->cshalf, x->Q[0], x->Q[1], x->Q[2], x->Q[3], x->G, x->rec, x->graw, x->phi, x->R,
                     x->scal, x->part, x->fin_mid, x->bad, x->Q32[0], x->Q32[1], x->Q32[2], x->Q32[3], x->rec32,
                     x->frec0, x->frec1, x->inc_code, x->inc_w, x->inc_n, x->inc_dx, x->inc_w32, x->inc_n32,
-                    x->inc_dx32, x->pstart, x->pface, x->ploc};
+                    x->inc_dx32, x->pstart, x->pface, x->ploc, x->hstart, x->hcell, x->hloc};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (int k = 0; k < MST_K_COUNT; k++)
@@ -645,8 +730,8 @@ int resolve_strategy(mst_ctx *x) {
     HIPCHK(hipEventCreate(&e1));
     float best = INFINITY;
     int pick = MST_STRATEGY_FACE, rc = MST_OK;
-    for (int s : {MST_STRATEGY_FACE, MST_STRATEGY_CELL, MST_STRATEGY_PATCH}) {
-        if (s == MST_STRATEGY_PATCH && !x->patch_ok) continue;
+    for (int s : {MST_STRATEGY_FACE, MST_STRATEGY_CELL, MST_STRATEGY_PATCH, MST_STRATEGY_FUSED}) {
+        if ((s == MST_STRATEGY_PATCH || s == MST_STRATEGY_FUSED) && !x->patch_ok) continue;
         x->resolved = s;
         rc = run_stage(x, UPD_STAGE, x->iold, x->iold, 0.0, 1.0, 1, spare);  // warm-up
         if (rc) break;
@@ -845,9 +930,9 @@ int mst_solve(mst_ctx *ctx) {
 int mst_set_strategy(mst_ctx *ctx, int strategy) {
     CTX_GUARD(ctx);
     if (strategy != MST_STRATEGY_FACE && strategy != MST_STRATEGY_CELL && strategy != MST_STRATEGY_PATCH &&
-        strategy != MST_STRATEGY_AUTO)
+        strategy != MST_STRATEGY_FUSED && strategy != MST_STRATEGY_AUTO)
         return mst::fail(MST_E_ARG, "bad strategy");
-    if (strategy == MST_STRATEGY_PATCH && !ctx->patch_ok)
+    if ((strategy == MST_STRATEGY_PATCH || strategy == MST_STRATEGY_FUSED) && !ctx->patch_ok)
         return mst::fail(MST_E_UNSUPPORTED, "patch strategy: needs at most 4 faces per cell and 64 KiB of fluxes per patch");
     ctx->strategy = strategy;
     ctx->resolved = strategy == MST_STRATEGY_AUTO ? -1 : strategy;
@@ -1224,6 +1309,12 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
             *bytes = flux_in + upd - (mixed ? 0 : (second ? nbnd : nc) * 8 * NV) - ell +
                      4.0 * (double)ctx->n_psolve + 2.0 * ctx->slots * nc;
             break;
+        case MST_K_FUSED: {  // gradient inputs + per-face inputs + update, each once; no records
+            const double grad_in = 4.0 * ctx->slots * nc + ninc * (r + 2 * r * D) + nc * (r * NV + 8);
+            const double flux_f = nf * (4 + 8 * D) + 4.0 * (double)ctx->n_psolve + nbnd * 4;
+            *bytes = grad_in + flux_f + upd - ell + 2.0 * ctx->slots * nc - (mixed ? 0 : nc * 8 * NV);
+            break;
+        }
         case MST_K_DT:
             *bytes = nc * (8 * NV + 8 + 8 * D);
             break;

@@ -24,10 +24,10 @@ PREC_FP64, PREC_MIXED = 0, 1
 COMPAT_BRANCH2_LR, COMPAT_GRAD_LASTINT, COMPAT_AUSM_LOCAL_A, COMPAT_AUSM_P_UNSCALED = 1, 2, 4, 8
 COMPAT_AUSM_ONESIDED = 16
 COMPAT_REFERENCE = COMPAT_GRAD_LASTINT | COMPAT_AUSM_LOCAL_A | COMPAT_AUSM_P_UNSCALED | COMPAT_AUSM_ONESIDED
-K_GRAD, K_FLUX, K_UPDATE, K_DT, K_CELL, K_PATCH = 0, 1, 2, 3, 4, 5
+K_GRAD, K_FLUX, K_UPDATE, K_DT, K_CELL, K_PATCH, K_FUSED = 0, 1, 2, 3, 4, 5, 6
 KERNEL_NAMES = {K_GRAD: "k_grad", K_FLUX: "k_flux", K_UPDATE: "k_update", K_DT: "k_cfl", K_CELL: "k_cell",
-                K_PATCH: "k_patch"}
-STRATEGY_FACE, STRATEGY_CELL, STRATEGY_AUTO, STRATEGY_PATCH = 0, 1, 2, 3
+                K_PATCH: "k_patch", K_FUSED: "k_fused"}
+STRATEGY_FACE, STRATEGY_CELL, STRATEGY_AUTO, STRATEGY_PATCH, STRATEGY_FUSED = 0, 1, 2, 3, 4
 INIT_REFERENCE, INIT_UNIFORM, INIT_RIEMANN, INIT_VORTEX, INIT_PULSE, INIT_FREESTREAM = 0, 1, 2, 3, 4, 5
 BATCH_BUCKETS = 4
 ORDER_HILBERT, ORDER_RCM = 0, 1

@@ -133,7 +133,7 @@ def test_gradient_and_limiter_parity(mname, lim):
     assert perr <= 1e-9
 
 
-STRATEGIES = [M.STRATEGY_FACE, M.STRATEGY_CELL, M.STRATEGY_PATCH]
+STRATEGIES = [M.STRATEGY_FACE, M.STRATEGY_CELL, M.STRATEGY_PATCH, M.STRATEGY_FUSED]
 
 
 @pytest.mark.parametrize("strategy", STRATEGIES)
@@ -260,7 +260,7 @@ def test_auto_strategy_resolves_and_matches():
     assert s.strategy == M.STRATEGY_AUTO
     s.set_state(Q)
     s.step(5)
-    assert s.strategy in (M.STRATEGY_FACE, M.STRATEGY_CELL, M.STRATEGY_PATCH)
+    assert s.strategy in STRATEGIES
     f = M.Solver(m, cg)
     f.strategy = M.STRATEGY_FACE
     f.set_state(Q)
@@ -295,9 +295,9 @@ def test_kernel_timing_counts_and_leaves_results_unchanged(strategy):
     wall_ms = 1e3 * (time.perf_counter() - t0)
     s.timing(False)
     assert np.array_equal(s.get_state(), plain.get_state())
-    face, cell, patch = (strategy == st for st in STRATEGIES)
-    want = {M.K_GRAD: 12, M.K_FLUX: 12 * face, M.K_UPDATE: 12 * face, M.K_CELL: 12 * cell,
-            M.K_PATCH: 12 * patch, M.K_DT: 1}  # later steps take dt from the last stage's update
+    face, cell, patch, fused = (strategy == st for st in STRATEGIES)
+    want = {M.K_GRAD: 12 * (not fused), M.K_FLUX: 12 * face, M.K_UPDATE: 12 * face, M.K_CELL: 12 * cell,
+            M.K_PATCH: 12 * patch, M.K_FUSED: 12 * fused, M.K_DT: 1}  # later steps: dt from the last stage
     total = 0.0
     for k, n_want in want.items():
         n, ms = s.timing_get(k)
