@@ -44,9 +44,14 @@ def build_workload(M, name):
 
 # HBM bytes per launch of a workload's dominant kernel from the committed rocprofv3 counter passes
 # (FETCH_SIZE and WRITE_SIZE scaled by a plain copy kernel of known size); None where not profiled
-TRAFFIC_PROFILES = {("vortex2d", "face", "k_grad"): ("profiles/hbm_traffic_vortex2d.md", "k_grad<2, 3, 2, double>"),
-                    ("vortex2d", "cell", "k_cell"): ("profiles/hbm_traffic_vortex2d.md", "k_cell<2, 0, true, double>"),
-                    ("vortex2d", "cell", "k_grad"): ("profiles/hbm_traffic_vortex2d.md", "k_grad<2, 3, 2, double>")}
+TRAFFIC_PROFILES = {("vortex2d", st, k): ("profiles/hbm_traffic_vortex2d.md", name)
+                    for st, k, name in (("face", "k_grad", "k_grad<2, 3, 2, double>"),
+                                        ("face", "k_flux", "k_flux<2, 0, true, double>"),
+                                        ("cell", "k_grad", "k_grad<2, 3, 2, double>"),
+                                        ("cell", "k_cell", "k_cell<2, 0, true, double, 3>"),
+                                        ("patch", "k_grad", "k_grad<2, 3, 2, double>"),
+                                        ("patch", "k_patch", "k_patch<2, 0, true, double, 3>"),
+                                        ("fused", "k_fused", "k_fused<2, 0, 2, double, 3>"))}
 
 
 def committed_traffic(workload, strategy, kernel):
