@@ -119,3 +119,37 @@ def flip_faces(raw, frac=0.5, seed=0):
     fn[sel] = fn[sel][:, ::-1]
     out["fnodes"] = np.ascontiguousarray(fn)
     return out, sel
+
+
+def quad_grid(n, bc=3, h=1.0, jitter=0.0, seed=0):
+    """n x n quadrilaterals of size h (interior nodes moved by up to jitter * h), interior faces
+    first, boundary faces of Fluent type bc around; the raw arrays of Mesh.from_raw."""
+    nid = lambda i, j: j * (n + 1) + i
+    cid = lambda i, j: j * n + i
+    xy = np.array([(i, j) for j in range(n + 1) for i in range(n + 1)], np.float64) * h
+    if jitter:
+        rng = np.random.default_rng(seed)
+        inner = np.array([0 < i < n and 0 < j < n for j in range(n + 1) for i in range(n + 1)])
+        xy[inner] += rng.uniform(-jitter * h, jitter * h, (int(inner.sum()), 2))
+    fn, c0, c1 = [], [], []
+    for j in range(n):
+        for i in range(n - 1):
+            fn.append((nid(i + 1, j), nid(i + 1, j + 1)))
+            c0.append(cid(i, j))
+            c1.append(cid(i + 1, j))
+    for j in range(n - 1):
+        for i in range(n):
+            fn.append((nid(i, j + 1), nid(i + 1, j + 1)))
+            c0.append(cid(i, j))
+            c1.append(cid(i, j + 1))
+    nint = len(fn)
+    for k in range(n):
+        for a, b, c in ((nid(k, 0), nid(k + 1, 0), cid(k, 0)), (nid(k, n), nid(k + 1, n), cid(k, n - 1)),
+                        (nid(0, k), nid(0, k + 1), cid(0, k)), (nid(n, k), nid(n, k + 1), cid(n - 1, k))):
+            fn.append((a, b))
+            c0.append(c)
+            c1.append(-1)
+    nf = len(fn)
+    return dict(dim=2, xyz=xy, fnodes=np.array(fn, np.int32), c0=np.array(c0, np.int32), c1=np.array(c1, np.int32),
+                ftype=np.array([2] * nint + [bc] * (nf - nint), np.int32), partner=np.full(nf, -1, np.int32),
+                n_cells=n * n, n_int=nint)

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import mstcfd as M
-from helpers import (cons, flip_faces, flux_scale, jumpy_state, normal_sums, rel_state_error, residual_parity_error,
+from helpers import (quad_grid, cons, flip_faces, flux_scale, jumpy_state, normal_sums, rel_state_error, residual_parity_error,
                      smooth_state, sod_state)
 from oracle import pyoracle as O
 
@@ -32,6 +32,8 @@ def mesh(name):
             m = M.Mesh.gen_tri(40, 30, 0, 4, 0, 2, 0.1, 7, (M.BC_INLET, M.BC_OUTLET, M.BC_WALL, M.BC_SYMMETRY))
         elif name == "periodic":
             m = M.Mesh.gen_tri(40, 40, -10, 10, -10, 10, 0.1, 3, (M.BC_PERIODIC,) * 4)
+        elif name == "quad":  # 4 faces per cell: the 4-slot kernels
+            m = M.Mesh.from_raw(quad_grid(40, bc=M.BC_WALL, h=0.05, jitter=0.2, seed=4))
         elif name == "tet":
             m = M.Mesh.gen_tet(6, 5, 4, (0, 0, 0), (1, 0.8, 0.7), 0.15, 11,
                                (M.BC_INLET, M.BC_OUTLET, M.BC_SYMMETRY, M.BC_WALL, M.BC_SYMMETRY, M.BC_SYMMETRY))
@@ -161,7 +163,7 @@ def test_euler_solve_matches_oracle(mname, scheme, strategy):
 @pytest.mark.parametrize("mname,scheme,order,lim,rk", [
     ("strip_wall", ROE, 2, M.LIM_MINMOD, 2), ("periodic", ROE, 2, M.LIM_VANLEER, 3),
     ("allbc", AUSM, 2, M.LIM_MINMOD, 2), ("tet", AUSM, 2, M.LIM_MINMOD, 2), ("allbc", ROE, 1, 0, 1),
-    ("tet", ROE, 2, M.LIM_VANLEER, 3)])
+    ("tet", ROE, 2, M.LIM_VANLEER, 3), ("quad", ROE, 2, M.LIM_VANLEER, 3), ("quad", AUSM, 2, M.LIM_MINMOD, 2)])
 def test_rk_steps_match_oracle(mname, scheme, order, lim, rk, strategy):
     m, om, geo = mesh(mname)
     dim = m.dim
@@ -224,7 +226,7 @@ def test_monitor_maxrel_matches_reference_rule():
 
 @pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_MIXED])
 @pytest.mark.parametrize("mname,scheme,lim", [("periodic", ROE, M.LIM_VANLEER), ("allbc", AUSM, M.LIM_MINMOD),
-                                              ("tet", ROE, M.LIM_MINMOD)])
+                                              ("tet", ROE, M.LIM_MINMOD), ("quad", AUSM, M.LIM_VANLEER)])
 def test_cell_strategy_matches_face_strategy(mname, scheme, lim, precision):
     """The fused cell kernel solves each interior face twice, the patch kernel each face of a patch
     once (a border face in both patches), always from the same memory inputs with the same

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import mstcfd as M
+from helpers import quad_grid
 from oracle import pyoracle as O
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -217,35 +218,6 @@ def block_internal_fraction(p, block=256):
     fc0, fc1, kind, _ = p.tables()
     i = kind == 0
     return float(((fc0[i] // block) == (fc1[i] // block)).mean())
-
-
-def quad_grid(n):
-    """n x n unit squares, interior faces first, walls around."""
-    nid = lambda i, j: j * (n + 1) + i
-    cid = lambda i, j: j * n + i
-    xy = np.array([(i, j) for j in range(n + 1) for i in range(n + 1)], np.float64)
-    fn, c0, c1 = [], [], []
-    for j in range(n):
-        for i in range(n - 1):
-            fn.append((nid(i + 1, j), nid(i + 1, j + 1)))
-            c0.append(cid(i, j))
-            c1.append(cid(i + 1, j))
-    for j in range(n - 1):
-        for i in range(n):
-            fn.append((nid(i, j + 1), nid(i + 1, j + 1)))
-            c0.append(cid(i, j))
-            c1.append(cid(i, j + 1))
-    nint = len(fn)
-    for k in range(n):
-        for a, b, c in ((nid(k, 0), nid(k + 1, 0), cid(k, 0)), (nid(k, n), nid(k + 1, n), cid(k, n - 1)),
-                        (nid(0, k), nid(0, k + 1), cid(0, k)), (nid(n, k), nid(n, k + 1), cid(n - 1, k))):
-            fn.append((a, b))
-            c0.append(c)
-            c1.append(-1)
-    nf = len(fn)
-    return dict(dim=2, xyz=xy, fnodes=np.array(fn, np.int32), c0=np.array(c0, np.int32), c1=np.array(c1, np.int32),
-                ftype=np.array([M.BC_INTERIOR] * nint + [M.BC_WALL] * (nf - nint), np.int32),
-                partner=np.full(nf, -1, np.int32), n_cells=n * n)
 
 
 @pytest.mark.parametrize("n", [2, 4, 16])
