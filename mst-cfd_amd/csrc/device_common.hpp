@@ -35,7 +35,7 @@ inline Params<T> make_params(const mst_config &c) {
 }
 
 // Each XCD receives every 8th workgroup; remap so that XCD x processes one contiguous slice of
-// the (RCM-ordered) index space and its L2 sees neighbouring cells.
+// the (Hilbert- or RCM-ordered) index space and its L2 sees neighbouring cells.
 __device__ __forceinline__ int64_t block_id() {
     const uint32_t G = gridDim.x, b = blockIdx.x;
     const uint32_t q = G >> 3, r = G & 7u, x = b & 7u, s = b >> 3;

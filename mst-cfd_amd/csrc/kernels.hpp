@@ -10,6 +10,8 @@
 //                        faces' Riemann problems, sums them and updates, without a face-flux array
 //   k_patch              (patch strategy, replaces k_flux + k_update) one block per patch of cells:
 //                        solves the patch's faces once into LDS, then sums and updates per cell
+//   k_fused              (fused strategy, replaces all of the above at order 2) the patch kernel with
+//                        the gradients of the patch and its halo, and the face states, in the block
 // plus k_cfl (CFL time step) and one-block reductions (k_finish) of per-block partials.
 //
 // Precision forms: T = double is the plain FP64 path (the reference's arithmetic); T = float is the

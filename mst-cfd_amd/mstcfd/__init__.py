@@ -297,7 +297,7 @@ class Mesh:
 
 
 class Prep:
-    """Host preprocessing (geometry, RCM order, face order, colouring) without a device."""
+    """Host preprocessing (geometry, cell order, face order, colouring) without a device."""
 
     def __init__(self, mesh=None, handle=None, owner=None):
         if handle is None:
