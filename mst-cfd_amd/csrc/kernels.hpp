@@ -681,16 +681,16 @@ struct FusedLds {
 };
 
 // Resident waves per SIMD requested for the fused kernel: 4 (at most 128 VGPRs) where it fits without
-// spilling (2D with FP32 arithmetic, Roe, or unlimited AUSM: 96-123 VGPRs); there the compiler would
+// spilling (2D with FP32 arithmetic, or triangles with Roe or unlimited AUSM); there the compiler would
 // keep 3 waves and the kernel, bound by its dependent load rounds, runs about 15 % slower.  Elsewhere
 // the bound would spill, so the compiler chooses.
-template <int DIM, int SCHEME, int LIM, typename T>
+template <int DIM, int SCHEME, int LIM, typename T, int MAXF>
 constexpr int fused_waves() {
-    return (DIM == 2 && (sizeof(T) == 4 || SCHEME == MST_SCHEME_ROE || LIM == MST_LIM_NONE)) ? 4 : 1;
+    return (DIM == 2 && (sizeof(T) == 4 || ((SCHEME == MST_SCHEME_ROE || LIM == MST_LIM_NONE) && MAXF == 3))) ? 4 : 1;
 }
 
 template <int DIM, int SCHEME, int LIM, typename T, int MAXF>
-__global__ __launch_bounds__(BLOCK, (fused_waves<DIM, SCHEME, LIM, T>())) void k_fused(const GradArgs<T> ga, const FluxArgs<T> fa, const UpdArgs a,
+__global__ __launch_bounds__(BLOCK, (fused_waves<DIM, SCHEME, LIM, T, MAXF>())) void k_fused(const GradArgs<T> ga, const FluxArgs<T> fa, const UpdArgs a,
                                                  const PatchArgs p, const HaloArgs h) {
     constexpr int NV = DIM + 2;
     constexpr bool F32 = std::is_same<T, float>::value;
@@ -700,8 +700,8 @@ __global__ __launch_bounds__(BLOCK, (fused_waves<DIM, SCHEME, LIM, T>())) void k
     const int npf = p.start[blk + 1] - f0;
     FusedLds<T> L(sh, npf, NV);
     // Inputs of the later phases that do not depend on the gradients are fetched first, so their
-    // latency overlaps the gradient phase: each lane's first PF patch faces (face, kind, normal) and
-    // its cell's slot entries.
+    // latency overlaps the gradient phase: each lane's first PF patch faces (face, kind, normal), its
+    // cell's slot entries and its first halo cell.
     constexpr int PF = 2;
     int64_t pf_f[PF];
     int pf_info[PF];
@@ -726,6 +726,7 @@ __global__ __launch_bounds__(BLOCK, (fused_waves<DIM, SCHEME, LIM, T>())) void k
     // limited gradients of the patch cells, then of the halo cells; face states into LDS
     const int32_t h0 = h.start[blk];
     const int next = BLOCK + (h.start[blk + 1] - h0);
+    const int32_t hc = BLOCK + (int)threadIdx.x < next ? h.cell[h0 + threadIdx.x] : 0;  // first halo pass
     for (int i = threadIdx.x; i < next; i += BLOCK) {
         int64_t c;
         const int16_t *lp;
@@ -736,7 +737,7 @@ __global__ __launch_bounds__(BLOCK, (fused_waves<DIM, SCHEME, LIM, T>())) void k
             lp = p.loc + c;
             stride = a.nc;
         } else {
-            c = h.cell[h0 + i - BLOCK];
+            c = i < 2 * BLOCK ? hc : h.cell[h0 + i - BLOCK];
             lp = h.loc + (int64_t)(h0 + i - BLOCK) * MAXF;
             stride = 1;
         }
