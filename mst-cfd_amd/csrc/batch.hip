@@ -212,6 +212,80 @@ __global__ __launch_bounds__(NT) void kb_cell(const BArgs<T> a) {
     }
 }
 
+// patch strategy: one item is one patch (BLOCK consecutive cells of a case, a compact patch in the
+// Hilbert order): its faces solved once into LDS (scheme per case), then sums in slot order and the
+// stage update per cell, as the single-case patch kernel does
+template <int DIM, bool SECOND, typename T, int MAXF>
+__global__ __launch_bounds__(BLOCK) void kb_patch(const BArgs<T> a, const PatchArgs p) {
+    constexpr int NV = DIM + 2;
+    constexpr bool F32 = std::is_same<T, float>::value;
+    extern __shared__ double gsh[];  // NV x patch faces, component-major
+    reset_others(a.cnt, a.own);
+    const bool L2 = a.u.l2 != 0, DT = a.u.dtred != 0;
+    for (;;) {
+        const int it = pull(a.cnt, a.own);  // (its barrier also ends the previous item's LDS reads)
+        if (it >= a.n_items) break;
+        const Item I = a.items[it];
+        FluxArgs<T> fa = a.f;
+        fa.P = a.P[I.cs];
+        const bool roe = a.scheme[I.cs] == MST_SCHEME_ROE;
+        const int32_t f0 = p.start[I.pad];
+        const int npf = p.start[I.pad + 1] - f0;
+        for (int i = threadIdx.x; i < npf; i += BLOCK) {
+            const int64_t f = p.face[f0 + i];
+            const int info = fa.finfo[f];
+            const int kind = info & 15;
+            double n64[DIM];
+            T X[NV], Y[NV];
+            face_load<DIM, SECOND, T>(fa, f, kind, n64, X, Y);
+            double g[NV];
+            face_solve<DIM, SCHEME_RUNTIME, T>(fa.P, roe, kind, info >> 4, n64, X, Y, g);
+#pragma unroll
+            for (int k = 0; k < NV; k++) gsh[k * npf + i] = g[k];
+        }
+        __syncthreads();
+        UpdArgs ua = a.u;
+        ua.bad = a.u.bad + I.cs;
+        ua.dt = a.dts + I.cs;
+        ua.gamma = a.gam[I.cs];
+        ua.cfl = a.cfl[I.cs];
+        double red[NV + 1];
+#pragma unroll
+        for (int k = 0; k < NV; k++) red[k] = 0;
+        red[NV] = INFINITY;
+        const int64_t c = I.begin + (int)threadIdx.x;
+        if (c < I.end) {
+            double r[NV], cmp[NV];
+#pragma unroll
+            for (int k = 0; k < NV; k++) r[k] = cmp[k] = 0;
+#pragma unroll
+            for (int s = 0; s < MAXF; s++) {
+                const int l = p.loc[(int64_t)s * ua.nc + c];
+                if (l < 0) continue;
+                double g[NV];
+#pragma unroll
+                for (int k = 0; k < NV; k++) g[k] = gsh[k * npf + (l >> 1)];
+                accumulate<NV, F32>(r, cmp, g, l & 1);
+            }
+            if constexpr (F32) {
+#pragma unroll
+                for (int k = 0; k < NV; k++) r[k] += cmp[k];
+            }
+            stage_cell<NV, F32>(ua, c, r, red, L2, DT);
+        }
+        if (L2) {
+            double v[NV];
+#pragma unroll
+            for (int k = 0; k < NV; k++) v[k] = red[k];
+            block_partial<NV, false, BLOCK>(v, a.part_l2 + it, a.n_items);
+        }
+        if (DT) {
+            double v[1] = {red[NV]};
+            block_partial<1, true, BLOCK>(v, a.part_dt + it, 1);
+        }
+    }
+}
+
 template <int DIM, typename T, int NT>
 __global__ __launch_bounds__(NT) void kb_cfl(const BArgs<T> a, const double *__restrict__ Q) {
     constexpr int NV = DIM + 2;
@@ -319,7 +393,14 @@ struct mst_batch {
     bool shadow_ok[4] = {false, false, false, false};
     int block = 256;
     int tiles[MST_BATCH_BUCKETS] = {1024, 2048, 4096, 8192};
-    int strategy = MST_STRATEGY_CELL;  // FACE: flux + gather/update kernels; CELL: fused cell kernel
+    int strategy = MST_STRATEGY_CELL;  // FACE: flux + gather/update kernels; CELL: fused cell kernel;
+                                       // PATCH: patch kernel over BLOCK-cell items
+    // patch tables (global indices): per patch its flux-carrying faces, per cell slot the local index
+    int32_t *pstart = nullptr, *pface = nullptr;
+    int16_t *ploc = nullptr;
+    std::vector<int32_t> patch_base;  // first patch of each case
+    int max_pf = 0;
+    bool patch_ok = false;
     std::vector<uint8_t> active;
     bool use_graph = true;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -358,11 +439,61 @@ void release(mst_batch *b) {
                     b->Q[0], b->Q[1], b->Q[2], b->Q[3], b->G, b->rec, b->Q32[0], b->Q32[1], b->Q32[2], b->Q32[3],
                     b->frec0, b->frec1, b->rec32, b->P64, b->P32, b->scheme, b->dts, b->gam,
                     b->cfl, b->l2, b->ncells, b->citems, b->fitems, b->ci0, b->ci1, b->part, b->cnt, b->bad,
-                    b->inc_code, b->inc_w, b->inc_n, b->inc_dx, b->inc_w32, b->inc_n32, b->inc_dx32};
+                    b->inc_code, b->inc_w, b->inc_n, b->inc_dx, b->inc_w32, b->inc_n32, b->inc_dx32,
+                    b->pstart, b->pface, b->ploc};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
+}
+
+// Patch tables of every case (global indices): each case's cells in runs of BLOCK (its patches), per
+// patch the faces that carry a flux into one of its cells (ascending), per cell slot the face's local
+// index in its patch.  The patch strategy needs at most 4 slots and each patch's fluxes (NV doubles
+// per face) within 64 KiB of LDS.
+int build_patches(mst_batch *b) {
+    b->patch_ok = false;
+    b->patch_base.assign(b->n + 1, 0);
+    if (b->slots > 4) return MST_OK;
+    std::vector<int32_t> start, face, list;
+    std::vector<int16_t> loc((size_t)b->slots * b->NC, -1);
+    size_t max_pf = 0;
+    for (int i = 0; i < b->n; i++) {
+        const mst::Prepared &P = b->preps[i];
+        const int64_t cs = b->cstart[i], fs = b->fstart[i];
+        b->patch_base[i] = (int32_t)start.size();
+        for (int64_t ca = 0; ca < P.nc; ca += BLOCK) {
+            const int64_t cz = std::min<int64_t>(P.nc, ca + BLOCK);
+            list.clear();
+            for (int s = 0; s < P.maxf; s++)
+                for (int64_t c = ca; c < cz; c++) {
+                    const int32_t e = P.ell[(int64_t)s * P.nc + c];
+                    if (e >= 0 && P.fkind[e >> 1] < mst::KIND_NONE) list.push_back(e >> 1);
+                }
+            std::sort(list.begin(), list.end());
+            list.erase(std::unique(list.begin(), list.end()), list.end());
+            max_pf = std::max(max_pf, list.size());
+            for (int s = 0; s < P.maxf; s++)
+                for (int64_t c = ca; c < cz; c++) {
+                    const int32_t e = P.ell[(int64_t)s * P.nc + c];
+                    if (e < 0 || P.fkind[e >> 1] >= mst::KIND_NONE) continue;
+                    const int64_t l = std::lower_bound(list.begin(), list.end(), e >> 1) - list.begin();
+                    loc[(size_t)s * b->NC + cs + c] = (int16_t)((l << 1) | (e & 1));
+                }
+            start.push_back((int32_t)face.size());
+            for (const int32_t f : list) face.push_back((int32_t)(f + fs));
+        }
+    }
+    b->patch_base[b->n] = (int32_t)start.size();
+    start.push_back((int32_t)face.size());
+    if (max_pf * b->nv * sizeof(double) > 65536) return MST_OK;
+    int rc = dupload(&b->pstart, start);
+    if (!rc) rc = dupload(&b->pface, face);
+    if (!rc) rc = dupload(&b->ploc, loc);
+    if (rc) return rc;
+    b->max_pf = (int)max_pf;
+    b->patch_ok = true;
+    return MST_OK;
 }
 
 // work items of the active cases for the current tiles; grows the device lists as needed
@@ -373,6 +504,13 @@ int build_items(mst_batch *b) {
         const int64_t nc = b->cstart[i + 1] - b->cstart[i], nf = b->fstart[i + 1] - b->fstart[i];
         c0[i] = c1[i] = (int32_t)ci.size();
         if (!b->active[i]) continue;
+        if (b->strategy == MST_STRATEGY_PATCH) {  // one item per patch, carrying the patch's index
+            for (int64_t s = 0, k = 0; s < nc; s += BLOCK, k++)
+                ci.push_back({i, (int32_t)(b->cstart[i] + s), (int32_t)(b->cstart[i] + std::min<int64_t>(nc, s + BLOCK)),
+                              (int32_t)(b->patch_base[i] + k)});
+            c1[i] = (int32_t)ci.size();
+            continue;
+        }
         const int64_t tc = b->tiles[bucket_of(nc)];
         const int64_t tf = std::max<int64_t>(64, ((tc * nf / std::max<int64_t>(nc, 1)) + 63) / 64 * 64);
         for (int64_t s = 0; s < nc; s += tc)
@@ -466,22 +604,27 @@ BArgs<T> base_args(const mst_batch *b, int isrc) {
 }
 
 template <typename K>
-int persistent_grid(const mst_batch *b, K kernel, int nt, int n_items, int *grid) {
+int persistent_grid(const mst_batch *b, K kernel, int nt, int n_items, int *grid, size_t shm = 0) {
     int per_cu = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, nt, 0));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, nt, shm));
     *grid = std::max(1, std::min(n_items, std::max(per_cu, 1) * b->ncu));
     return MST_OK;
 }
 
 template <typename K, typename... A>
-int launch_persistent(mst_batch *b, K kernel, int nt, int n_items, const A &...args) {
+int launch_persistent_shm(mst_batch *b, K kernel, int nt, size_t shm, int n_items, const A &...args) {
     if (n_items <= 0) return MST_OK;
     int grid = 0;
-    int rc = persistent_grid(b, kernel, nt, n_items, &grid);
+    int rc = persistent_grid(b, kernel, nt, n_items, &grid, shm);
     if (rc) return rc;
-    hipLaunchKernelGGL(kernel, dim3(grid), dim3(nt), 0, b->stream, args...);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(nt), (uint32_t)shm, b->stream, args...);
     HIPCHK(hipGetLastError());
     return MST_OK;
+}
+
+template <typename K, typename... A>
+int launch_persistent(mst_batch *b, K kernel, int nt, int n_items, const A &...args) {
+    return launch_persistent_shm(b, kernel, nt, 0, n_items, args...);
 }
 
 template <int DIM, typename T, int NT>
@@ -561,6 +704,36 @@ int run_cell(mst_batch *b, int stage, int is, int in, double ca, double cb, int 
 #undef MST_BCELL
 }
 
+template <int DIM, typename T>
+int run_patch(mst_batch *b, int stage, int is, int in, double ca, double cb, int stage0, int iout, bool l2, bool dt,
+              bool check) {
+    BArgs<T> a = base_args<T>(b, is);
+    a.items = b->citems;
+    a.n_items = b->n_citems;
+    a.own = (stage & 1) ? CNT_FLUX : CNT_UPD;
+    a.u.Qs = b->Q[is];
+    a.u.Qn = b->Q[in];
+    a.u.Qout = b->Q[iout];
+    a.u.Qout32 = b->mixed ? b->Q32[iout] : nullptr;
+    a.u.ca = ca;
+    a.u.cb = cb;
+    a.u.stage0 = stage0;
+    a.u.l2 = l2;
+    a.u.dtred = dt;
+    a.u.check = check;
+    PatchArgs pa;
+    pa.start = b->pstart;
+    pa.face = b->pface;
+    pa.loc = b->ploc;
+    const size_t shm = (size_t)b->max_pf * b->nv * sizeof(double);
+#define MST_BPATCH(MF)                                                                                        \
+    if (b->order >= 2) return launch_persistent_shm(b, kb_patch<DIM, true, T, MF>, BLOCK, shm, a.n_items, a, pa); \
+    return launch_persistent_shm(b, kb_patch<DIM, false, T, MF>, BLOCK, shm, a.n_items, a, pa);
+    if (b->slots == 3) { MST_BPATCH(3) }
+    MST_BPATCH(4)
+#undef MST_BPATCH
+}
+
 int run_finish(mst_batch *b, bool l2, bool dt) {
     BFin f;
     f.cnt = b->cnt;
@@ -609,7 +782,10 @@ int enqueue_step_t(mst_batch *b, int iold, int inew) {
             rc = run_grad<DIM, T, NT>(b, src);
             if (rc) return rc;
         }
-        if (b->strategy == MST_STRATEGY_CELL) {
+        if (b->strategy == MST_STRATEGY_PATCH) {
+            rc = run_patch<DIM, T>(b, s, src, iold, A[s], B[s], s == 0 ? 1 : 0, dst, s == 0, s == ns - 1, s == ns - 1);
+            if (rc) return rc;
+        } else if (b->strategy == MST_STRATEGY_CELL) {
             rc = run_cell<DIM, T, NT>(b, s, src, iold, A[s], B[s], s == 0 ? 1 : 0, dst, s == 0, s == ns - 1, s == ns - 1);
             if (rc) return rc;
         } else {
@@ -625,6 +801,7 @@ int enqueue_step_t(mst_batch *b, int iold, int inew) {
 
 template <int DIM, typename T>
 int enqueue_step_nt(mst_batch *b, int iold, int inew) {
+    if (b->strategy == MST_STRATEGY_PATCH) return enqueue_step_t<DIM, T, BLOCK>(b, iold, inew);  // patch = BLOCK cells
     if (b->block == 64) return enqueue_step_t<DIM, T, 64>(b, iold, inew);
     if (b->block == 128) return enqueue_step_t<DIM, T, 128>(b, iold, inew);
     return enqueue_step_t<DIM, T, 256>(b, iold, inew);
@@ -1003,6 +1180,7 @@ int mst_batch_create(int n_cases, const mst_mesh *const *meshes, const mst_confi
         release(b);
         return mst::fail(MST_E_HIP, "hipMemset failed");
     }
+    TRY(build_patches(b));
     TRY(build_items(b));
 #undef TRY
     *out = b;
@@ -1203,11 +1381,15 @@ int mst_batch_get_tuning(const mst_batch *b, int *block, int32_t *tiles) {
 
 int mst_batch_set_strategy(mst_batch *b, int strategy) {
     B_GUARD(b);
-    if (strategy != MST_STRATEGY_FACE && strategy != MST_STRATEGY_CELL)
-        return mst::fail(MST_E_ARG, "batch strategy must be MST_STRATEGY_FACE or MST_STRATEGY_CELL");
+    if (strategy != MST_STRATEGY_FACE && strategy != MST_STRATEGY_CELL && strategy != MST_STRATEGY_PATCH)
+        return mst::fail(MST_E_ARG, "batch strategy must be MST_STRATEGY_FACE, MST_STRATEGY_CELL or MST_STRATEGY_PATCH");
+    if (strategy == MST_STRATEGY_PATCH && !b->patch_ok)
+        return mst::fail(MST_E_UNSUPPORTED, "patch strategy: needs at most 4 faces per cell and 64 KiB of fluxes per patch");
     if (strategy != b->strategy) {
+        const bool items = strategy == MST_STRATEGY_PATCH || b->strategy == MST_STRATEGY_PATCH;
         b->strategy = strategy;
         drop_graphs(b);
+        if (items) return build_items(b);  // patch items replace the tiles
     }
     return MST_OK;
 }
@@ -1280,8 +1462,13 @@ int mst_batch_autotune(mst_batch *b, const char *cache_path, int force, int step
     // coordinate search: strategy x block size, then each populated bucket's tile, then the block
     // size again
     for (int pass = 0; pass < 2 && !rc; pass++) {
-        for (int st : {MST_STRATEGY_FACE, MST_STRATEGY_CELL}) {
+        for (int st : {MST_STRATEGY_FACE, MST_STRATEGY_CELL, MST_STRATEGY_PATCH}) {
             if (pass == 1 && st != best_strat) continue;
+            if (st == MST_STRATEGY_PATCH) {  // fixed shape: one BLOCK-cell patch per item
+                if (pass == 0 && b->patch_ok) rc = trial(st, BLOCK, best_tiles);
+                if (rc) break;
+                continue;
+            }
             for (int blk : cand_blocks) {
                 if (pass == 1 && blk == best_block) continue;
                 rc = trial(st, blk, best_tiles);
@@ -1290,6 +1477,7 @@ int mst_batch_autotune(mst_batch *b, const char *cache_path, int force, int step
             if (rc) break;
         }
         if (pass == 1 || rc) break;
+        if (best_strat == MST_STRATEGY_PATCH) break;  // tiles do not apply
         for (int k = 0; k < MST_BATCH_BUCKETS && !rc; k++) {
             if (!hist[k]) continue;
             for (int t : cand_tiles) {

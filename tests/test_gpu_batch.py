@@ -46,10 +46,12 @@ def single(m, cfg, q, steps, **extra):
     return s.get_state(), s.residual_norms()
 
 
+@pytest.mark.parametrize("strategy", [M.STRATEGY_CELL, M.STRATEGY_PATCH])
 @pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_MIXED])
-def test_batch_equals_single_case_contexts(precision):
+def test_batch_equals_single_case_contexts(precision, strategy):
     meshes, cfgs, Q0 = riemann_cases(10)
     b = make_batch(meshes, cfgs, Q0, precision=precision)
+    b.strategy = strategy
     n0 = b.counts()
     print(f"batch: {n0[0]} cases, {n0[1]} cells, {n0[3]} cell items, {n0[4]} face items", flush=True)
     dt0 = b.dt()
@@ -64,7 +66,7 @@ def test_batch_equals_single_case_contexts(precision):
         assert np.allclose(res[i], s.residual_norms(), rtol=1e-13, atol=0)
 
 
-@pytest.mark.parametrize("strategy", [M.STRATEGY_FACE, M.STRATEGY_CELL])
+@pytest.mark.parametrize("strategy", [M.STRATEGY_FACE, M.STRATEGY_CELL, M.STRATEGY_PATCH])
 @pytest.mark.parametrize("rk", [1, 3])
 def test_batch_first_order_equals_single(rk, strategy):
     """Order 1 (no gradient kernel between the stage kernels) with an odd stage count."""
@@ -84,7 +86,9 @@ def test_batch_tiling_block_graph_independence():
     for block, tiles, graph, strategy in [(256, [1024, 2048, 4096, 8192], True, M.STRATEGY_FACE),
                                           (64, [64, 128, 256, 512], True, M.STRATEGY_CELL),
                                           (128, [256, 256, 512, 16384], False, M.STRATEGY_CELL),
-                                          (128, [512, 1024, 64, 16384], False, M.STRATEGY_FACE)]:
+                                          (128, [512, 1024, 64, 16384], False, M.STRATEGY_FACE),
+                                          (64, [64, 128, 256, 512], True, M.STRATEGY_PATCH),
+                                          (256, [256, 256, 512, 16384], False, M.STRATEGY_PATCH)]:
         b = make_batch(meshes, cfgs, Q0)
         b.set_tuning(block, tiles)
         b.set_graph(graph)
@@ -93,14 +97,17 @@ def test_batch_tiling_block_graph_independence():
         b.step(20)
         out.append([b.get_state(i) for i in range(len(meshes))])
         b.close()
-    for k in (1, 2, 3):
+    for k in range(1, len(out)):
         for i in range(len(meshes)):
             assert np.array_equal(out[k][i], out[0][i])
 
 
-def test_batch_freeze_and_reactivate():
+@pytest.mark.parametrize("strategy", [M.STRATEGY_CELL, M.STRATEGY_PATCH])
+def test_batch_freeze_and_reactivate(strategy):
+    """Frozen cases leave the work lists (patch items keep their own patch's index)."""
     meshes, cfgs, Q0 = riemann_cases(6, seed=9)
     b = make_batch(meshes, cfgs, Q0)
+    b.strategy = strategy
     b.step(5)
     frozen = [b.get_state(i) for i in range(6)]
     active = np.array([1, 0, 1, 0, 1, 1], bool)
@@ -142,7 +149,8 @@ def test_batch_autotune_cache_and_state(tmp_path):
         assert np.array_equal(b.get_state(i), b2.get_state(i))
 
 
-def test_batch_3d_cases():
+@pytest.mark.parametrize("strategy", [M.STRATEGY_CELL, M.STRATEGY_PATCH])
+def test_batch_3d_cases(strategy):
     bc = (M.BC_INLET, M.BC_OUTLET) + (M.BC_SYMMETRY,) * 4
     meshes = [M.Mesh.gen_tet(n, 4, 4, jitter=0.1, seed=n, bc=bc) for n in (4, 9, 6)]
     qin = [1.0, 0.8, 0.0, 0.0, 1 / 0.4 / 1.4 + 0.32]
@@ -150,6 +158,7 @@ def test_batch_3d_cases():
             for s in (M.AUSM, M.ROE, M.AUSM)]
     Q0 = [m.init_state(M.INIT_PULSE, [1.4, 0.1, 0.15, 0.5, 0.5, 0.5, 0.8]) for m in meshes]
     b = make_batch(meshes, cfgs, Q0)
+    b.strategy = strategy
     b.step(12)
     for i in range(3):
         q, _ = single(meshes[i], cfgs[i], Q0[i], 12)
