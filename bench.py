@@ -177,8 +177,8 @@ def bench_batch(M, args):
         faces = b.n_faces
         out_forms[form] = dict(value=faces * 2 * args.steps / res["graph"], ms_per_step=1e3 * res["graph"] / args.steps,
                                direct_ms_per_step=1e3 * res["direct"] / args.steps, block=block, tiles=tiles,
-                               strategy={M.STRATEGY_FACE: "face", M.STRATEGY_CELL: "cell",
-                                         M.STRATEGY_PATCH: "patch"}.get(b.strategy, b.strategy),
+                               strategy={M.STRATEGY_FACE: "face", M.STRATEGY_CELL: "cell", M.STRATEGY_PATCH: "patch",
+                                         M.STRATEGY_FUSED: "fused"}.get(b.strategy, b.strategy),
                                autotune_ms_per_step=tune_ms, cell_items=n_items[3], face_items=n_items[4],
                                state_finite=bool(finite))
         b.close()

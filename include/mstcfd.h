@@ -268,10 +268,11 @@ int mst_batch_run_to_steady(mst_batch *b, double tol, int max_steps, int check_e
 /* launch shape: block size (64, 128, 256) and cell tile per case-size bucket (faces: scaled by nf/nc) */
 int mst_batch_set_tuning(mst_batch *b, int block, const int32_t *tiles);
 int mst_batch_get_tuning(const mst_batch *b, int *block, int32_t *tiles);
-/* kernel strategy of the batched step: MST_STRATEGY_FACE, MST_STRATEGY_CELL (default) or
- * MST_STRATEGY_PATCH (one work item per 256-cell patch, its fluxes in LDS; block and tiles do not
- * apply; MST_E_UNSUPPORTED for cells of more than 4 faces).  The autotuner searches all three; the
- * results are the same bit for bit. */
+/* kernel strategy of the batched step: MST_STRATEGY_FACE, MST_STRATEGY_CELL (default),
+ * MST_STRATEGY_PATCH (one work item per 256-cell patch, its fluxes in LDS) or MST_STRATEGY_FUSED (the
+ * patch item also computes its cells' and halo's gradients, order 2); block and tiles do not apply
+ * to the patch strategies, which return MST_E_UNSUPPORTED for cells of more than 4 faces.  The
+ * autotuner searches all four; the results are the same bit for bit. */
 int mst_batch_set_strategy(mst_batch *b, int strategy);
 int mst_batch_get_strategy(const mst_batch *b, int *strategy);
 /* 1: replay the captured step graph (default); 0: launch the kernels one by one */

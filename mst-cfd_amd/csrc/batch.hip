@@ -286,6 +286,54 @@ __global__ __launch_bounds__(BLOCK) void kb_patch(const BArgs<T> a, const PatchA
     }
 }
 
+// fused strategy (order 2): one item is one patch, whose cells' and halo's gradients, face states and
+// fluxes stay in the block (fused_patch), then sums in slot order and the stage update per cell
+template <int DIM, int LIM, typename T, int MAXF>
+__global__ __launch_bounds__(BLOCK) void kb_fused(const BArgs<T> a, const PatchArgs p, const HaloArgs h) {
+    constexpr int NV = DIM + 2;
+    constexpr bool F32 = std::is_same<T, float>::value;
+    extern __shared__ double sh[];
+    reset_others(a.cnt, a.own);
+    const bool L2 = a.u.l2 != 0, DT = a.u.dtred != 0;
+    for (;;) {
+        const int it = pull(a.cnt, a.own);  // (its barrier also ends the previous item's LDS reads)
+        if (it >= a.n_items) break;
+        const Item I = a.items[it];
+        FluxArgs<T> fa = a.f;
+        fa.P = a.P[I.cs];
+        const bool roe = a.scheme[I.cs] == MST_SCHEME_ROE;
+        const int32_t f0 = p.start[I.pad];
+        FusedLds<T> L(sh, p.start[I.pad + 1] - f0, NV);
+        int cl[MAXF];
+        fused_patch<DIM, SCHEME_RUNTIME, LIM, T, MAXF>(a.g, fa, roe, p, h, I.pad, I.begin, I.end, a.u.nc, f0, L, cl);
+        UpdArgs ua = a.u;
+        ua.bad = a.u.bad + I.cs;
+        ua.dt = a.dts + I.cs;
+        ua.gamma = a.gam[I.cs];
+        ua.cfl = a.cfl[I.cs];
+        double red[NV + 1];
+#pragma unroll
+        for (int k = 0; k < NV; k++) red[k] = 0;
+        red[NV] = INFINITY;
+        const int64_t c = I.begin + (int)threadIdx.x;
+        if (c < I.end) {
+            double r[NV];
+            patch_sum<NV, MAXF, F32>(L.g, L.npf, cl, r);
+            stage_cell<NV, F32>(ua, c, r, red, L2, DT);
+        }
+        if (L2) {
+            double v[NV];
+#pragma unroll
+            for (int k = 0; k < NV; k++) v[k] = red[k];
+            block_partial<NV, false, BLOCK>(v, a.part_l2 + it, a.n_items);
+        }
+        if (DT) {
+            double v[1] = {red[NV]};
+            block_partial<1, true, BLOCK>(v, a.part_dt + it, 1);
+        }
+    }
+}
+
 template <int DIM, typename T, int NT>
 __global__ __launch_bounds__(NT) void kb_cfl(const BArgs<T> a, const double *__restrict__ Q) {
     constexpr int NV = DIM + 2;
@@ -401,6 +449,10 @@ struct mst_batch {
     std::vector<int32_t> patch_base;  // first patch of each case
     int max_pf = 0;
     bool patch_ok = false;
+    // fused strategy: each patch's halo cells and their slots' local faces
+    int32_t *hstart = nullptr, *hcell = nullptr;
+    int16_t *hloc = nullptr;
+    bool fused_ok = false;
     std::vector<uint8_t> active;
     bool use_graph = true;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -440,7 +492,7 @@ void release(mst_batch *b) {
                     b->frec0, b->frec1, b->rec32, b->P64, b->P32, b->scheme, b->dts, b->gam,
                     b->cfl, b->l2, b->ncells, b->citems, b->fitems, b->ci0, b->ci1, b->part, b->cnt, b->bad,
                     b->inc_code, b->inc_w, b->inc_n, b->inc_dx, b->inc_w32, b->inc_n32, b->inc_dx32,
-                    b->pstart, b->pface, b->ploc};
+                    b->pstart, b->pface, b->ploc, b->hstart, b->hcell, b->hloc};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -452,11 +504,11 @@ void release(mst_batch *b) {
 // index in its patch.  The patch strategy needs at most 4 slots and each patch's fluxes (NV doubles
 // per face) within 64 KiB of LDS.
 int build_patches(mst_batch *b) {
-    b->patch_ok = false;
+    b->patch_ok = b->fused_ok = false;
     b->patch_base.assign(b->n + 1, 0);
     if (b->slots > 4) return MST_OK;
-    std::vector<int32_t> start, face, list;
-    std::vector<int16_t> loc((size_t)b->slots * b->NC, -1);
+    std::vector<int32_t> start, face, list, hstart, hcell, halo;
+    std::vector<int16_t> loc((size_t)b->slots * b->NC, -1), hloc;
     size_t max_pf = 0;
     for (int i = 0; i < b->n; i++) {
         const mst::Prepared &P = b->preps[i];
@@ -482,17 +534,45 @@ int build_patches(mst_batch *b) {
                 }
             start.push_back((int32_t)face.size());
             for (const int32_t f : list) face.push_back((int32_t)(f + fs));
+            // halo: the other cell of every patch face with one cell outside the patch, and for each of
+            // its slots the local index of the slot's face when the face is one of the patch's
+            halo.clear();
+            for (const int32_t f : list) {
+                if (P.fkind[f] != mst::KIND_INTERIOR && P.fkind[f] != mst::KIND_PERIODIC) continue;
+                for (const int32_t c : {P.fc0[f], P.fc1[f]})
+                    if (c < ca || c >= cz) halo.push_back(c);
+            }
+            std::sort(halo.begin(), halo.end());
+            halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+            hstart.push_back((int32_t)hcell.size());
+            for (const int32_t c : halo) {
+                hcell.push_back((int32_t)(c + cs));
+                for (int s = 0; s < b->slots; s++) {
+                    const int32_t e = s < P.maxf ? P.ell[(int64_t)s * P.nc + c] : -1;
+                    int16_t v = -1;
+                    if (e >= 0 && P.fkind[e >> 1] < mst::KIND_NONE) {
+                        const auto it = std::lower_bound(list.begin(), list.end(), e >> 1);
+                        if (it != list.end() && *it == (e >> 1)) v = (int16_t)(((it - list.begin()) << 1) | (e & 1));
+                    }
+                    hloc.push_back(v);
+                }
+            }
         }
     }
     b->patch_base[b->n] = (int32_t)start.size();
     start.push_back((int32_t)face.size());
+    hstart.push_back((int32_t)hcell.size());
     if (max_pf * b->nv * sizeof(double) > 65536) return MST_OK;
     int rc = dupload(&b->pstart, start);
     if (!rc) rc = dupload(&b->pface, face);
     if (!rc) rc = dupload(&b->ploc, loc);
+    if (!rc) rc = dupload(&b->hstart, hstart);
+    if (!rc) rc = dupload(&b->hcell, hcell);
+    if (!rc) rc = dupload(&b->hloc, hloc);
     if (rc) return rc;
     b->max_pf = (int)max_pf;
     b->patch_ok = true;
+    b->fused_ok = max_pf * b->nv * 16 <= 65536;  // the fused kernel keeps states and fluxes in LDS
     return MST_OK;
 }
 
@@ -504,7 +584,7 @@ int build_items(mst_batch *b) {
         const int64_t nc = b->cstart[i + 1] - b->cstart[i], nf = b->fstart[i + 1] - b->fstart[i];
         c0[i] = c1[i] = (int32_t)ci.size();
         if (!b->active[i]) continue;
-        if (b->strategy == MST_STRATEGY_PATCH) {  // one item per patch, carrying the patch's index
+        if (b->strategy == MST_STRATEGY_PATCH || b->strategy == MST_STRATEGY_FUSED) {  // one item per patch
             for (int64_t s = 0, k = 0; s < nc; s += BLOCK, k++)
                 ci.push_back({i, (int32_t)(b->cstart[i] + s), (int32_t)(b->cstart[i] + std::min<int64_t>(nc, s + BLOCK)),
                               (int32_t)(b->patch_base[i] + k)});
@@ -734,6 +814,43 @@ int run_patch(mst_batch *b, int stage, int is, int in, double ca, double cb, int
 #undef MST_BPATCH
 }
 
+template <int DIM, typename T>
+int run_fused(mst_batch *b, int stage, int is, int in, double ca, double cb, int stage0, int iout, bool l2, bool dt,
+              bool check) {
+    BArgs<T> a = base_args<T>(b, is);
+    a.items = b->citems;
+    a.n_items = b->n_citems;
+    a.own = (stage & 1) ? CNT_FLUX : CNT_UPD;
+    a.u.Qs = b->Q[is];
+    a.u.Qn = b->Q[in];
+    a.u.Qout = b->Q[iout];
+    a.u.Qout32 = b->mixed ? b->Q32[iout] : nullptr;
+    a.u.ca = ca;
+    a.u.cb = cb;
+    a.u.stage0 = stage0;
+    a.u.l2 = l2;
+    a.u.dtred = dt;
+    a.u.check = check;
+    PatchArgs pa;
+    pa.start = b->pstart;
+    pa.face = b->pface;
+    pa.loc = b->ploc;
+    HaloArgs ha;
+    ha.start = b->hstart;
+    ha.cell = b->hcell;
+    ha.loc = b->hloc;
+    const size_t shm = (size_t)16 * b->nv * b->max_pf;
+#define MST_BFUSED(LM, MF) return launch_persistent_shm(b, kb_fused<DIM, LM, T, MF>, BLOCK, shm, a.n_items, a, pa, ha);
+#define MST_BFUSED_L(MF)                                          \
+    if (b->limiter == MST_LIM_NONE) { MST_BFUSED(MST_LIM_NONE, MF) } \
+    if (b->limiter == MST_LIM_MINMOD) { MST_BFUSED(MST_LIM_MINMOD, MF) } \
+    MST_BFUSED(MST_LIM_VANLEER, MF)
+    if (b->slots == 3) { MST_BFUSED_L(3) }
+    MST_BFUSED_L(4)
+#undef MST_BFUSED_L
+#undef MST_BFUSED
+}
+
 int run_finish(mst_batch *b, bool l2, bool dt) {
     BFin f;
     f.cnt = b->cnt;
@@ -778,11 +895,17 @@ int enqueue_step_t(mst_batch *b, int iold, int inew) {
     for (int s = 0; s < ns; s++) {
         const int dst = (s == ns - 1) ? inew : spare[s & 1];
         int rc;
+        if (b->strategy == MST_STRATEGY_FUSED && b->order >= 2) {
+            rc = run_fused<DIM, T>(b, s, src, iold, A[s], B[s], s == 0 ? 1 : 0, dst, s == 0, s == ns - 1, s == ns - 1);
+            if (rc) return rc;
+            src = dst;
+            continue;
+        }
         if (b->order >= 2) {
             rc = run_grad<DIM, T, NT>(b, src);
             if (rc) return rc;
         }
-        if (b->strategy == MST_STRATEGY_PATCH) {
+        if (b->strategy == MST_STRATEGY_PATCH || b->strategy == MST_STRATEGY_FUSED) {  // FUSED at order 1: PATCH
             rc = run_patch<DIM, T>(b, s, src, iold, A[s], B[s], s == 0 ? 1 : 0, dst, s == 0, s == ns - 1, s == ns - 1);
             if (rc) return rc;
         } else if (b->strategy == MST_STRATEGY_CELL) {
@@ -801,7 +924,8 @@ int enqueue_step_t(mst_batch *b, int iold, int inew) {
 
 template <int DIM, typename T>
 int enqueue_step_nt(mst_batch *b, int iold, int inew) {
-    if (b->strategy == MST_STRATEGY_PATCH) return enqueue_step_t<DIM, T, BLOCK>(b, iold, inew);  // patch = BLOCK cells
+    if (b->strategy == MST_STRATEGY_PATCH || b->strategy == MST_STRATEGY_FUSED)
+        return enqueue_step_t<DIM, T, BLOCK>(b, iold, inew);  // a patch is BLOCK cells
     if (b->block == 64) return enqueue_step_t<DIM, T, 64>(b, iold, inew);
     if (b->block == 128) return enqueue_step_t<DIM, T, 128>(b, iold, inew);
     return enqueue_step_t<DIM, T, 256>(b, iold, inew);
@@ -1381,12 +1505,14 @@ int mst_batch_get_tuning(const mst_batch *b, int *block, int32_t *tiles) {
 
 int mst_batch_set_strategy(mst_batch *b, int strategy) {
     B_GUARD(b);
-    if (strategy != MST_STRATEGY_FACE && strategy != MST_STRATEGY_CELL && strategy != MST_STRATEGY_PATCH)
-        return mst::fail(MST_E_ARG, "batch strategy must be MST_STRATEGY_FACE, MST_STRATEGY_CELL or MST_STRATEGY_PATCH");
-    if (strategy == MST_STRATEGY_PATCH && !b->patch_ok)
-        return mst::fail(MST_E_UNSUPPORTED, "patch strategy: needs at most 4 faces per cell and 64 KiB of fluxes per patch");
+    if (strategy != MST_STRATEGY_FACE && strategy != MST_STRATEGY_CELL && strategy != MST_STRATEGY_PATCH &&
+        strategy != MST_STRATEGY_FUSED)
+        return mst::fail(MST_E_ARG, "batch strategy must be MST_STRATEGY_FACE, _CELL, _PATCH or _FUSED");
+    if ((strategy == MST_STRATEGY_PATCH && !b->patch_ok) || (strategy == MST_STRATEGY_FUSED && !b->fused_ok))
+        return mst::fail(MST_E_UNSUPPORTED, "patch strategies: need at most 4 faces per cell and a patch's face data within 64 KiB");
     if (strategy != b->strategy) {
-        const bool items = strategy == MST_STRATEGY_PATCH || b->strategy == MST_STRATEGY_PATCH;
+        auto patched = [](int st) { return st == MST_STRATEGY_PATCH || st == MST_STRATEGY_FUSED; };
+        const bool items = patched(strategy) != patched(b->strategy);
         b->strategy = strategy;
         drop_graphs(b);
         if (items) return build_items(b);  // patch items replace the tiles
@@ -1462,10 +1588,11 @@ int mst_batch_autotune(mst_batch *b, const char *cache_path, int force, int step
     // coordinate search: strategy x block size, then each populated bucket's tile, then the block
     // size again
     for (int pass = 0; pass < 2 && !rc; pass++) {
-        for (int st : {MST_STRATEGY_FACE, MST_STRATEGY_CELL, MST_STRATEGY_PATCH}) {
+        for (int st : {MST_STRATEGY_FACE, MST_STRATEGY_CELL, MST_STRATEGY_PATCH, MST_STRATEGY_FUSED}) {
             if (pass == 1 && st != best_strat) continue;
-            if (st == MST_STRATEGY_PATCH) {  // fixed shape: one BLOCK-cell patch per item
-                if (pass == 0 && b->patch_ok) rc = trial(st, BLOCK, best_tiles);
+            if (st == MST_STRATEGY_PATCH || st == MST_STRATEGY_FUSED) {  // fixed shape: one BLOCK-cell patch per item
+                const bool ok = st == MST_STRATEGY_PATCH ? b->patch_ok : (b->fused_ok && b->order >= 2);
+                if (pass == 0 && ok) rc = trial(st, BLOCK, best_tiles);
                 if (rc) break;
                 continue;
             }
@@ -1477,7 +1604,7 @@ int mst_batch_autotune(mst_batch *b, const char *cache_path, int force, int step
             if (rc) break;
         }
         if (pass == 1 || rc) break;
-        if (best_strat == MST_STRATEGY_PATCH) break;  // tiles do not apply
+        if (best_strat == MST_STRATEGY_PATCH || best_strat == MST_STRATEGY_FUSED) break;  // tiles do not apply
         for (int k = 0; k < MST_BATCH_BUCKETS && !rc; k++) {
             if (!hist[k]) continue;
             for (int t : cand_tiles) {

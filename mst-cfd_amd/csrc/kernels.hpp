@@ -689,16 +689,17 @@ constexpr int fused_waves() {
     return (DIM == 2 && (sizeof(T) == 4 || ((SCHEME == MST_SCHEME_ROE || LIM == MST_LIM_NONE) && MAXF == 3))) ? 4 : 1;
 }
 
+// Phases of one patch in the fused form, for a block of BLOCK lanes: the limited gradients of the
+// patch cells [cb, ce) and of the patch's halo, their face states into LDS, then the fluxes of the
+// patch faces (p.face from f0) into L.g.  Ends with a barrier.  cl receives the lane's cell's slot
+// entries ((local face << 1) | side, -1: none) for the caller's sums.  roe picks the scheme when
+// SCHEME is SCHEME_RUNTIME.
 template <int DIM, int SCHEME, int LIM, typename T, int MAXF>
-__global__ __launch_bounds__(BLOCK, (fused_waves<DIM, SCHEME, LIM, T, MAXF>())) void k_fused(const GradArgs<T> ga, const FluxArgs<T> fa, const UpdArgs a,
-                                                 const PatchArgs p, const HaloArgs h) {
+__device__ __forceinline__ void fused_patch(const GradArgs<T> &ga, const FluxArgs<T> &fa, bool roe, const PatchArgs &p,
+                                            const HaloArgs &h, int64_t patch, int64_t cb, int64_t ce, int64_t nc,
+                                            int32_t f0, FusedLds<T> &L, int (&cl)[MAXF]) {
     constexpr int NV = DIM + 2;
-    constexpr bool F32 = std::is_same<T, float>::value;
-    extern __shared__ double sh[];
-    const int64_t blk = block_id();
-    const int32_t f0 = p.start[blk];
-    const int npf = p.start[blk + 1] - f0;
-    FusedLds<T> L(sh, npf, NV);
+    const int npf = L.npf;
     // Inputs of the later phases that do not depend on the gradients are fetched first, so their
     // latency overlaps the gradient phase: each lane's first PF patch faces (face, kind, normal), its
     // cell's slot entries and its first halo cell.
@@ -718,31 +719,29 @@ __global__ __launch_bounds__(BLOCK, (fused_waves<DIM, SCHEME, LIM, T, MAXF>())) 
             for (int d = 0; d < DIM; d++) pf_n[q][d] = fa.fn[pf_f[q] * DIM + d];
         }
     }
-    const int64_t c = blk * BLOCK + threadIdx.x;
-    const bool valid = c < a.nc;
-    int cl[MAXF];
+    const int64_t c = cb + threadIdx.x;
 #pragma unroll
-    for (int s = 0; s < MAXF; s++) cl[s] = valid ? p.loc[(int64_t)s * a.nc + c] : -1;
+    for (int s = 0; s < MAXF; s++) cl[s] = c < ce ? p.loc[(int64_t)s * nc + c] : -1;
     // limited gradients of the patch cells, then of the halo cells; face states into LDS
-    const int32_t h0 = h.start[blk];
-    const int next = BLOCK + (h.start[blk + 1] - h0);
+    const int32_t h0 = h.start[patch];
+    const int next = BLOCK + (h.start[patch + 1] - h0);
     const int32_t hc = BLOCK + (int)threadIdx.x < next ? h.cell[h0 + threadIdx.x] : 0;  // first halo pass
     for (int i = threadIdx.x; i < next; i += BLOCK) {
-        int64_t c;
+        int64_t cc;
         const int16_t *lp;
         int64_t stride;
         if (i < BLOCK) {
-            c = blk * BLOCK + i;
-            if (c >= a.nc) continue;
-            lp = p.loc + c;
-            stride = a.nc;
+            cc = cb + i;
+            if (cc >= ce) continue;
+            lp = p.loc + cc;
+            stride = nc;
         } else {
-            c = i < 2 * BLOCK ? hc : h.cell[h0 + i - BLOCK];
+            cc = i < 2 * BLOCK ? hc : h.cell[h0 + i - BLOCK];
             lp = h.loc + (int64_t)(h0 + i - BLOCK) * MAXF;
             stride = 1;
         }
         T qc[NV], lgr[NV * DIM], dxs[MAXF][DIM], t[NV][DIM], ph[NV];
-        const int nfc = grad_limited<DIM, MAXF, LIM, T>(ga, c, qc, lgr, dxs, t, ph);
+        const int nfc = grad_limited<DIM, MAXF, LIM, T>(ga, cc, qc, lgr, dxs, t, ph);
 #pragma unroll
         for (int s = 0; s < MAXF; s++) {
             if (s >= nfc) break;
@@ -788,29 +787,51 @@ __global__ __launch_bounds__(BLOCK, (fused_waves<DIM, SCHEME, LIM, T, MAXF>())) 
             for (int k = 0; k < NV; k++) Y[k] = L.r(0, k, i, NV);
         }
         double g[NV];
-        face_solve<DIM, SCHEME, T>(fa.P, SCHEME == MST_SCHEME_ROE, kind, info >> 4, n64, X, Y, g);
+        face_solve<DIM, SCHEME, T>(fa.P, SCHEME == SCHEME_RUNTIME ? roe : SCHEME == MST_SCHEME_ROE, kind, info >> 4,
+                                   n64, X, Y, g);
 #pragma unroll
         for (int k = 0; k < NV; k++) L.g[k * npf + i] = g[k];
     }
     __syncthreads();
-    double r[NV], cmp[NV];
+}
+
+// sum of the lane's cell's face fluxes from LDS, in slot order (the face strategy's order)
+template <int NV, int MAXF, bool F32>
+__device__ __forceinline__ void patch_sum(const double *g, int npf, const int (&cl)[MAXF], double (&r)[NV]) {
+    double cmp[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) r[k] = cmp[k] = 0;
-    if (valid) {
 #pragma unroll
-        for (int s = 0; s < MAXF; s++) {
-            const int l = cl[s];
-            if (l < 0) continue;
-            double g[NV];
+    for (int s = 0; s < MAXF; s++) {
+        const int l = cl[s];
+        if (l < 0) continue;
+        double gf[NV];
 #pragma unroll
-            for (int k = 0; k < NV; k++) g[k] = L.g[k * npf + (l >> 1)];
-            accumulate<NV, F32>(r, cmp, g, l & 1);
-        }
-        if constexpr (F32) {
-#pragma unroll
-            for (int k = 0; k < NV; k++) r[k] += cmp[k];
-        }
+        for (int k = 0; k < NV; k++) gf[k] = g[k * npf + (l >> 1)];
+        accumulate<NV, F32>(r, cmp, gf, l & 1);
     }
+    if constexpr (F32) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) r[k] += cmp[k];
+    }
+}
+
+template <int DIM, int SCHEME, int LIM, typename T, int MAXF>
+__global__ __launch_bounds__(BLOCK, (fused_waves<DIM, SCHEME, LIM, T, MAXF>())) void k_fused(
+    const GradArgs<T> ga, const FluxArgs<T> fa, const UpdArgs a, const PatchArgs p, const HaloArgs h) {
+    constexpr int NV = DIM + 2;
+    constexpr bool F32 = std::is_same<T, float>::value;
+    extern __shared__ double sh[];
+    const int64_t blk = block_id();
+    const int32_t f0 = p.start[blk];
+    FusedLds<T> L(sh, p.start[blk + 1] - f0, NV);
+    const int64_t cb = blk * BLOCK, ce = cb + BLOCK < a.nc ? cb + BLOCK : a.nc;
+    int cl[MAXF];
+    fused_patch<DIM, SCHEME, LIM, T, MAXF>(ga, fa, SCHEME == MST_SCHEME_ROE, p, h, blk, cb, ce, a.nc, f0, L, cl);
+    const int64_t c = cb + threadIdx.x;
+    const bool valid = c < a.nc;
+    double r[NV];
+    patch_sum<NV, MAXF, F32>(L.g, L.npf, cl, r);
     stage_tail<NV, true, F32>(a, c, valid, r, blk, a.l2, a.dtred);
 }
 

@@ -46,7 +46,7 @@ def single(m, cfg, q, steps, **extra):
     return s.get_state(), s.residual_norms()
 
 
-@pytest.mark.parametrize("strategy", [M.STRATEGY_CELL, M.STRATEGY_PATCH])
+@pytest.mark.parametrize("strategy", [M.STRATEGY_CELL, M.STRATEGY_PATCH, M.STRATEGY_FUSED])
 @pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_MIXED])
 def test_batch_equals_single_case_contexts(precision, strategy):
     meshes, cfgs, Q0 = riemann_cases(10)
@@ -66,7 +66,7 @@ def test_batch_equals_single_case_contexts(precision, strategy):
         assert np.allclose(res[i], s.residual_norms(), rtol=1e-13, atol=0)
 
 
-@pytest.mark.parametrize("strategy", [M.STRATEGY_FACE, M.STRATEGY_CELL, M.STRATEGY_PATCH])
+@pytest.mark.parametrize("strategy", [M.STRATEGY_FACE, M.STRATEGY_CELL, M.STRATEGY_PATCH, M.STRATEGY_FUSED])
 @pytest.mark.parametrize("rk", [1, 3])
 def test_batch_first_order_equals_single(rk, strategy):
     """Order 1 (no gradient kernel between the stage kernels) with an odd stage count."""
@@ -88,7 +88,8 @@ def test_batch_tiling_block_graph_independence():
                                           (128, [256, 256, 512, 16384], False, M.STRATEGY_CELL),
                                           (128, [512, 1024, 64, 16384], False, M.STRATEGY_FACE),
                                           (64, [64, 128, 256, 512], True, M.STRATEGY_PATCH),
-                                          (256, [256, 256, 512, 16384], False, M.STRATEGY_PATCH)]:
+                                          (256, [256, 256, 512, 16384], False, M.STRATEGY_PATCH),
+                                          (128, [1024, 2048, 4096, 8192], True, M.STRATEGY_FUSED)]:
         b = make_batch(meshes, cfgs, Q0)
         b.set_tuning(block, tiles)
         b.set_graph(graph)
@@ -102,7 +103,7 @@ def test_batch_tiling_block_graph_independence():
             assert np.array_equal(out[k][i], out[0][i])
 
 
-@pytest.mark.parametrize("strategy", [M.STRATEGY_CELL, M.STRATEGY_PATCH])
+@pytest.mark.parametrize("strategy", [M.STRATEGY_CELL, M.STRATEGY_PATCH, M.STRATEGY_FUSED])
 def test_batch_freeze_and_reactivate(strategy):
     """Frozen cases leave the work lists (patch items keep their own patch's index)."""
     meshes, cfgs, Q0 = riemann_cases(6, seed=9)
@@ -149,7 +150,7 @@ def test_batch_autotune_cache_and_state(tmp_path):
         assert np.array_equal(b.get_state(i), b2.get_state(i))
 
 
-@pytest.mark.parametrize("strategy", [M.STRATEGY_CELL, M.STRATEGY_PATCH])
+@pytest.mark.parametrize("strategy", [M.STRATEGY_CELL, M.STRATEGY_PATCH, M.STRATEGY_FUSED])
 def test_batch_3d_cases(strategy):
     bc = (M.BC_INLET, M.BC_OUTLET) + (M.BC_SYMMETRY,) * 4
     meshes = [M.Mesh.gen_tet(n, 4, 4, jitter=0.1, seed=n, bc=bc) for n in (4, 9, 6)]
