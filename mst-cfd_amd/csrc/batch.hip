@@ -839,7 +839,7 @@ int run_fused(mst_batch *b, int stage, int is, int in, double ca, double cb, int
     ha.start = b->hstart;
     ha.cell = b->hcell;
     ha.loc = b->hloc;
-    const size_t shm = (size_t)16 * b->nv * b->max_pf;
+    const size_t shm = (size_t)FusedLds<T>::bytes_per_face_field * b->nv * b->max_pf;
 #define MST_BFUSED(LM, MF) return launch_persistent_shm(b, kb_fused<DIM, LM, T, MF>, BLOCK, shm, a.n_items, a, pa, ha);
 #define MST_BFUSED_L(MF)                                          \
     if (b->limiter == MST_LIM_NONE) { MST_BFUSED(MST_LIM_NONE, MF) } \

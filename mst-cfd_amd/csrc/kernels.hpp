@@ -665,9 +665,11 @@ struct HaloArgs {
     const int16_t *__restrict__ loc;    // per halo entry, MAXF slots: (local face << 1) | side, -1: not in the patch
 };
 
-// LDS of k_fused for npf faces: the two sides' face states, T[2][NV][npf], and the fluxes,
-// double[NV][npf]; FP64 fluxes overwrite the c0-side states of their own face (only the lane that
-// solves a face reads its states), FP32 ones follow the states.  16 * NV * npf bytes either way.
+// LDS of the fused kernels for npf faces: the two sides' face states and the fluxes, double[NV][npf].
+// Each face's flux overwrites its own states, which only the lane that solves the face reads.
+// FP64: states T[2][NV][npf], fluxes over the c0-side states, 16 * NV * npf bytes.  FP32: the two
+// sides' states of a component side by side (float2[NV][npf]), exactly the bytes of that
+// component's FP64 flux, 8 * NV * npf bytes.
 template <typename T>
 struct FusedLds {
     T *rec;
@@ -675,9 +677,15 @@ struct FusedLds {
     int npf;
     __device__ FusedLds(double *sh, int npf_, int nv) : npf(npf_) {
         rec = reinterpret_cast<T *>(sh);
-        g = std::is_same<T, double>::value ? sh : sh + nv * npf_;
+        g = sh;
     }
-    __device__ T &r(int side, int k, int l, int nv) { return rec[(side * nv + k) * npf + l]; }
+    __device__ T &r(int side, int k, int l, int nv) {
+        if constexpr (std::is_same<T, double>::value)
+            return rec[(side * nv + k) * npf + l];
+        else
+            return rec[((k * npf + l) << 1) + side];
+    }
+    static constexpr int bytes_per_face_field = std::is_same<T, double>::value ? 16 : 8;
 };
 
 // Resident waves per SIMD requested for the fused kernel: 4 (at most 128 VGPRs) where it fits without

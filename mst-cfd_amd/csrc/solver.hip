@@ -355,7 +355,7 @@ int run_fused_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int 
     ha.start = x->hstart;
     ha.cell = x->hcell;
     ha.loc = x->hloc;
-    const size_t shm = (size_t)16 * x->nv * x->max_pf;
+    const size_t shm = (size_t)FusedLds<T>::bytes_per_face_field * x->nv * x->max_pf;
     const bool roe = x->cfg.scheme == MST_SCHEME_ROE;
     const int lim = x->cfg.limiter;
 #define MST_FUSED_L(SC, LM, MF) \
