@@ -4,7 +4,8 @@ Default workload: 2D isentropic vortex (strength 5, Mach 0.5 freestream, gamma 1
 1000x1000 quads split into 2,000,000 jittered triangles (+-0.1 h, seed 12345), periodic on all
 sides; Roe + van Leer, SSP-RK3, CFL 0.5 (device-side time step), FP64.  One "step" is one RK3
 time step = 3 residual evaluations (gradient+limiter, face flux, gather+update each) + the CFL
-reduction.  face-updates/s = N_faces * 3 * steps / wall time, inputs resident in HBM.
+reduction.  face-updates/s = N_faces * 3 * steps / wall time, inputs resident in HBM.  Every kernel
+strategy (face, cell, patch, fused) is timed on the same context; `value` is the fastest.
 
 Prints ONE JSON line (see the keys below).  Only one GPU is used: --gpus N>1 exits with a message.
 """
