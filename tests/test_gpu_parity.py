@@ -309,3 +309,32 @@ def test_kernel_timing_counts_and_leaves_results_unchanged(strategy):
     assert total < wall_ms
     s.timing_reset()
     assert s.timing_get(M.K_GRAD) == (0, 0.0)
+
+
+@pytest.mark.parametrize("mname", ["allbc", "tet"])
+def test_rcm_cell_order_all_strategies_match_oracle(mname):
+    """Under the reverse Cuthill-McKee order the patches are strips, not compact: every strategy must
+    still match the oracle, and the strategies one another bit for bit."""
+    old = M.get_cell_order()
+    M.set_cell_order(M.ORDER_RCM)
+    try:
+        m, om, geo = mesh(mname)
+        dim = m.dim
+        cg, co = both_cfg(dim=dim, scheme=ROE, order=2, limiter=M.LIM_VANLEER, rk=3, cfl=0.4, q_inlet=inlet(dim))
+        Q = smooth_state(geo["ccen"], seed=12, mach=0.6)
+        Qr = Q.copy()
+        for _ in range(5):
+            Qr, _res = om.rk_step(co, Qr, om.cfl_dt(co, Qr))
+        out = []
+        for strategy in STRATEGIES:
+            s = M.Solver(m, cg)
+            s.strategy = strategy
+            s.set_state(Q)
+            s.step(5)
+            out.append(s.get_state())
+            s.close()
+    finally:
+        M.set_cell_order(old)
+    assert rel_state_error(out[0], Qr) <= TOL
+    for k in range(1, len(out)):
+        assert np.array_equal(out[k], out[0]), STRATEGIES[k]
