@@ -688,13 +688,16 @@ struct FusedLds {
     static constexpr int bytes_per_face_field = std::is_same<T, double>::value ? 16 : 8;
 };
 
-// Resident waves per SIMD requested for the fused kernel: 4 (at most 128 VGPRs) where it fits without
-// spilling (2D with FP32 arithmetic, or triangles with Roe or unlimited AUSM); there the compiler would
-// keep 3 waves and the kernel, bound by its dependent load rounds, runs about 15 % slower.  Elsewhere
-// the bound would spill, so the compiler chooses.
+// Resident waves per SIMD requested for the fused kernel, where the bound costs no more than a register
+// of spill: FP64 triangles with Roe or unlimited AUSM 4 (at most 128 VGPRs; the compiler would keep 3
+// and the kernel, bound by its dependent load rounds, runs about 15 % slower), FP32 triangles 6 (at
+// most 80 VGPRs, 3 % faster than 5), other 2D FP32 cells 4.  Elsewhere the bound would spill, so the
+// compiler chooses.
 template <int DIM, int SCHEME, int LIM, typename T, int MAXF>
 constexpr int fused_waves() {
-    return (DIM == 2 && (sizeof(T) == 4 || ((SCHEME == MST_SCHEME_ROE || LIM == MST_LIM_NONE) && MAXF == 3))) ? 4 : 1;
+    if constexpr (DIM != 2) return 1;
+    if constexpr (sizeof(T) == 4) return MAXF == 3 ? 6 : 4;
+    return ((SCHEME == MST_SCHEME_ROE || LIM == MST_LIM_NONE) && MAXF == 3) ? 4 : 1;
 }
 
 // Phases of one patch in the fused form, for a block of BLOCK lanes: the limited gradients of the
