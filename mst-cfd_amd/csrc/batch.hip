@@ -305,7 +305,9 @@ __global__ __launch_bounds__(BLOCK) void kb_fused(const BArgs<T> a, const PatchA
         const int32_t f0 = p.start[I.pad];
         FusedLds<T> L(sh, p.start[I.pad + 1] - f0, NV);
         int cl[MAXF];
-        fused_patch<DIM, SCHEME_RUNTIME, LIM, T, MAXF>(a.g, fa, roe, p, h, I.pad, I.begin, I.end, a.u.nc, f0, L, cl);
+        // (no early fetch of the update's inputs here: with both schemes inlined it would cost a wave)
+        fused_patch<DIM, SCHEME_RUNTIME, LIM, T, MAXF>(a.g, fa, roe, p, h, I.pad, I.begin, I.end, a.u.nc, f0, L, cl,
+                                                       []() {});
         UpdArgs ua = a.u;
         ua.bad = a.u.bad + I.cs;
         ua.dt = a.dts + I.cs;

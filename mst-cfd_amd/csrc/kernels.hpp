@@ -429,22 +429,33 @@ __device__ __forceinline__ void accumulate(double (&r)[NV], double (&cmp)[NV], c
 
 // Stage update of cell c from its residual r (RhoSolver::solve, RhoSolver.cpp:57-66, in SSP-RK stage
 // form) and the non-physical-state flag; red[] receives this cell's |R/V|^2 (L2) and CFL time step (DT)
+// the inputs of cell c's stage update that do not depend on its residual: volume, stage state and
+// (after the first stage) the state at the start of the step
+template <int NV>
+struct StageIn {
+    double V, qs[NV], qn[NV];
+};
+
+template <int NV>
+__device__ __forceinline__ void stage_load(const UpdArgs &a, int64_t c, StageIn<NV> &in) {
+    in.V = a.vol[c];
+    ldq<NV>(a.Qs + c * NV, in.qs);
+    if (!a.stage0) ldq<NV>(a.Qn + c * NV, in.qn);
+}
+
 template <int NV, bool F32>
-__device__ __forceinline__ void stage_cell(const UpdArgs &a, int64_t c, const double (&r)[NV], double (&red)[NV + 1],
-                                           const bool L2, const bool DT) {
+__device__ __forceinline__ void stage_cell(const UpdArgs &a, int64_t c, const StageIn<NV> &in, const double (&r)[NV],
+                                           double (&red)[NV + 1], const bool L2, const bool DT) {
     constexpr int DIM = NV - 2;
-    const double V = a.vol[c];
+    const double V = in.V;
     const double dtv = a.dt[0] / V;
-    double qs[NV], out[NV];
-    ldq<NV>(a.Qs + c * NV, qs);
+    double out[NV];
     if (a.stage0) {
 #pragma unroll
-        for (int k = 0; k < NV; k++) out[k] = qs[k] - dtv * r[k];
+        for (int k = 0; k < NV; k++) out[k] = in.qs[k] - dtv * r[k];
     } else {
-        double qn[NV];
-        ldq<NV>(a.Qn + c * NV, qn);
 #pragma unroll
-        for (int k = 0; k < NV; k++) out[k] = a.ca * qn[k] + a.cb * (qs[k] - dtv * r[k]);
+        for (int k = 0; k < NV; k++) out[k] = a.ca * in.qn[k] + a.cb * (in.qs[k] - dtv * r[k]);
     }
     stq<NV>(a.Qout + c * NV, out);
     if constexpr (F32) stq<NV>(a.Qout32 + c * NV, out);
@@ -467,11 +478,19 @@ __device__ __forceinline__ void stage_cell(const UpdArgs &a, int64_t c, const do
     if (DT) red[NV] = fmin(red[NV], local_dt<DIM>(out, a.sh, c, V, a.gamma, a.cfl));
 }
 
+template <int NV, bool F32>
+__device__ __forceinline__ void stage_cell(const UpdArgs &a, int64_t c, const double (&r)[NV], double (&red)[NV + 1],
+                                           const bool L2, const bool DT) {
+    StageIn<NV> in;
+    stage_load<NV>(a, c, in);
+    stage_cell<NV, F32>(a, c, in, r, red, L2, DT);
+}
+
 // stage_cell for the lanes of one block, then the block's partials of |R/V|^2 and the CFL time step.
 // Every lane of the block must call it (block-level reductions).
 template <int NV, bool STAGE, bool F32>
 __device__ __forceinline__ void stage_tail(const UpdArgs &a, int64_t c, bool valid, const double (&r)[NV],
-                                           int64_t blk, const bool L2, const bool DT) {
+                                           int64_t blk, const bool L2, const bool DT, const StageIn<NV> *in = nullptr) {
     if constexpr (!STAGE) {
         if (valid) stq<NV>(a.R + c * NV, r);
         return;
@@ -480,7 +499,12 @@ __device__ __forceinline__ void stage_tail(const UpdArgs &a, int64_t c, bool val
 #pragma unroll
         for (int k = 0; k < NV; k++) red[k] = 0;
         red[NV] = INFINITY;
-        if (valid) stage_cell<NV, F32>(a, c, r, red, L2, DT);
+        if (valid) {
+            if (in)
+                stage_cell<NV, F32>(a, c, *in, r, red, L2, DT);
+            else
+                stage_cell<NV, F32>(a, c, r, red, L2, DT);
+        }
         if (L2) {  // uniform over the launch
             double v[NV];
 #pragma unroll
@@ -704,11 +728,13 @@ constexpr int fused_waves() {
 // patch cells [cb, ce) and of the patch's halo, their face states into LDS, then the fluxes of the
 // patch faces (p.face from f0) into L.g.  Ends with a barrier.  cl receives the lane's cell's slot
 // entries ((local face << 1) | side, -1: none) for the caller's sums.  roe picks the scheme when
-// SCHEME is SCHEME_RUNTIME.
-template <int DIM, int SCHEME, int LIM, typename T, int MAXF>
+// SCHEME is SCHEME_RUNTIME.  pre() runs between the gradient and the flux phases, when the gradient's
+// registers are free: the caller fetches its update's inputs there, so their latency overlaps the
+// face solves.
+template <int DIM, int SCHEME, int LIM, typename T, int MAXF, typename Pre>
 __device__ __forceinline__ void fused_patch(const GradArgs<T> &ga, const FluxArgs<T> &fa, bool roe, const PatchArgs &p,
                                             const HaloArgs &h, int64_t patch, int64_t cb, int64_t ce, int64_t nc,
-                                            int32_t f0, FusedLds<T> &L, int (&cl)[MAXF]) {
+                                            int32_t f0, FusedLds<T> &L, int (&cl)[MAXF], Pre pre) {
     constexpr int NV = DIM + 2;
     const int npf = L.npf;
     // Inputs of the later phases that do not depend on the gradients are fetched first, so their
@@ -767,6 +793,7 @@ __device__ __forceinline__ void fused_patch(const GradArgs<T> &ga, const FluxArg
         }
     }
     __syncthreads();
+    pre();
     // the patch's face fluxes (face_load's inputs from LDS: both sides' states, or for a boundary
     // face the raw interior state from memory and its reconstructed state)
     for (int q = 0, i = threadIdx.x; i < npf; q++, i += BLOCK) {
@@ -838,12 +865,15 @@ __global__ __launch_bounds__(BLOCK, (fused_waves<DIM, SCHEME, LIM, T, MAXF>())) 
     FusedLds<T> L(sh, p.start[blk + 1] - f0, NV);
     const int64_t cb = blk * BLOCK, ce = cb + BLOCK < a.nc ? cb + BLOCK : a.nc;
     int cl[MAXF];
-    fused_patch<DIM, SCHEME, LIM, T, MAXF>(ga, fa, SCHEME == MST_SCHEME_ROE, p, h, blk, cb, ce, a.nc, f0, L, cl);
     const int64_t c = cb + threadIdx.x;
     const bool valid = c < a.nc;
+    StageIn<NV> in;
+    fused_patch<DIM, SCHEME, LIM, T, MAXF>(ga, fa, SCHEME == MST_SCHEME_ROE, p, h, blk, cb, ce, a.nc, f0, L, cl, [&]() {
+        if (valid) stage_load<NV>(a, c, in);
+    });
     double r[NV];
     patch_sum<NV, MAXF, F32>(L.g, L.npf, cl, r);
-    stage_tail<NV, true, F32>(a, c, valid, r, blk, a.l2, a.dtred);
+    stage_tail<NV, true, F32>(a, c, valid, r, blk, a.l2, a.dtred, &in);
 }
 
 // ------------------------------------------------------------------------------------ reductions
