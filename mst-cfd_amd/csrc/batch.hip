@@ -255,22 +255,11 @@ __global__ __launch_bounds__(BLOCK) void kb_patch(const BArgs<T> a, const PatchA
         red[NV] = INFINITY;
         const int64_t c = I.begin + (int)threadIdx.x;
         if (c < I.end) {
-            double r[NV], cmp[NV];
+            int cl[MAXF];
 #pragma unroll
-            for (int k = 0; k < NV; k++) r[k] = cmp[k] = 0;
-#pragma unroll
-            for (int s = 0; s < MAXF; s++) {
-                const int l = p.loc[(int64_t)s * ua.nc + c];
-                if (l < 0) continue;
-                double g[NV];
-#pragma unroll
-                for (int k = 0; k < NV; k++) g[k] = gsh[k * npf + (l >> 1)];
-                accumulate<NV, F32>(r, cmp, g, l & 1);
-            }
-            if constexpr (F32) {
-#pragma unroll
-                for (int k = 0; k < NV; k++) r[k] += cmp[k];
-            }
+            for (int s = 0; s < MAXF; s++) cl[s] = p.loc[(int64_t)s * ua.nc + c];
+            double r[NV];
+            patch_sum<NV, MAXF, F32>(gsh, npf, cl, r);
             stage_cell<NV, F32>(ua, c, r, red, L2, DT);
         }
         if (L2) {

@@ -1,9 +1,10 @@
 // kernels.hpp -- gfx950 kernels of the density-based explicit residual path.
 //
 // Per residual evaluation (one RK stage), on the context's stream:
-//   k_grad    (order 2)  one lane per cell: Green-Gauss gradient + limiter -> limited gradient
-//   k_flux               one lane per face: reconstruct, ghost state, dimension-split Roe/AUSM,
-//                        G_f = sum_d n_d F_d  (c0 outward normal)
+//   k_grad    (order 2)  one lane per cell: Green-Gauss gradient + limiter -> the reconstructed state
+//                        at each of the cell's faces (face-state records)
+//   k_flux               one lane per face: its two records (or cell states), ghost state,
+//                        dimension-split Roe/AUSM, G_f = sum_d n_d F_d  (c0 outward normal)
 //   k_update             one lane per cell: R_c = sum +-G_f over its faces in a fixed order (no float
 //                        atomics -> bit-reproducible), then the Euler / SSP-RK stage update
 //   k_cell               (cell strategy, replaces k_flux + k_update) one lane per cell: solves its own
@@ -637,6 +638,27 @@ struct PatchArgs {
     const int16_t *__restrict__ loc;    // slot-major per cell: (local face << 1) | is_c1, -1: no flux
 };
 
+// sum of the lane's cell's face fluxes from LDS, in slot order (the face strategy's order)
+template <int NV, int MAXF, bool F32>
+__device__ __forceinline__ void patch_sum(const double *g, int npf, const int (&cl)[MAXF], double (&r)[NV]) {
+    double cmp[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) r[k] = cmp[k] = 0;
+#pragma unroll
+    for (int s = 0; s < MAXF; s++) {
+        const int l = cl[s];
+        if (l < 0) continue;
+        double gf[NV];
+#pragma unroll
+        for (int k = 0; k < NV; k++) gf[k] = g[k * npf + (l >> 1)];
+        accumulate<NV, F32>(r, cmp, gf, l & 1);
+    }
+    if constexpr (F32) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) r[k] += cmp[k];
+    }
+}
+
 template <int DIM, int SCHEME, bool SECOND, typename T, int MAXF>
 __global__ __launch_bounds__(BLOCK) void k_patch(const FluxArgs<T> fa, const UpdArgs a, const PatchArgs p) {
     constexpr int NV = DIM + 2;
@@ -656,24 +678,11 @@ __global__ __launch_bounds__(BLOCK) void k_patch(const FluxArgs<T> fa, const Upd
     __syncthreads();
     const int64_t c = blk * BLOCK + threadIdx.x;
     const bool valid = c < a.nc;
-    double r[NV], cmp[NV];
+    int cl[MAXF];
 #pragma unroll
-    for (int k = 0; k < NV; k++) r[k] = cmp[k] = 0;
-    if (valid) {
-#pragma unroll
-        for (int s = 0; s < MAXF; s++) {  // the slot table is padded to MAXF rows
-            const int l = p.loc[(int64_t)s * a.nc + c];
-            if (l < 0) continue;
-            double g[NV];
-#pragma unroll
-            for (int k = 0; k < NV; k++) g[k] = gsh[k * npf + (l >> 1)];
-            accumulate<NV, F32>(r, cmp, g, l & 1);
-        }
-        if constexpr (F32) {
-#pragma unroll
-            for (int k = 0; k < NV; k++) r[k] += cmp[k];
-        }
-    }
+    for (int s = 0; s < MAXF; s++) cl[s] = valid ? p.loc[(int64_t)s * a.nc + c] : -1;  // padded to MAXF rows
+    double r[NV];
+    patch_sum<NV, MAXF, F32>(gsh, npf, cl, r);
     stage_tail<NV, true, F32>(a, c, valid, r, blk, a.l2, a.dtred);
 }
 
@@ -831,27 +840,6 @@ __device__ __forceinline__ void fused_patch(const GradArgs<T> &ga, const FluxArg
         for (int k = 0; k < NV; k++) L.g[k * npf + i] = g[k];
     }
     __syncthreads();
-}
-
-// sum of the lane's cell's face fluxes from LDS, in slot order (the face strategy's order)
-template <int NV, int MAXF, bool F32>
-__device__ __forceinline__ void patch_sum(const double *g, int npf, const int (&cl)[MAXF], double (&r)[NV]) {
-    double cmp[NV];
-#pragma unroll
-    for (int k = 0; k < NV; k++) r[k] = cmp[k] = 0;
-#pragma unroll
-    for (int s = 0; s < MAXF; s++) {
-        const int l = cl[s];
-        if (l < 0) continue;
-        double gf[NV];
-#pragma unroll
-        for (int k = 0; k < NV; k++) gf[k] = g[k * npf + (l >> 1)];
-        accumulate<NV, F32>(r, cmp, gf, l & 1);
-    }
-    if constexpr (F32) {
-#pragma unroll
-        for (int k = 0; k < NV; k++) r[k] += cmp[k];
-    }
 }
 
 template <int DIM, int SCHEME, int LIM, typename T, int MAXF>
