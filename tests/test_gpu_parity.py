@@ -338,3 +338,33 @@ def test_rcm_cell_order_all_strategies_match_oracle(mname):
     assert rel_state_error(out[0], Qr) <= TOL
     for k in range(1, len(out)):
         assert np.array_equal(out[k], out[0]), STRATEGIES[k]
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_tiny_meshes_all_strategies_match_oracle(dim):
+    """One partial patch of a few cells (4 triangles, 6 tetrahedra), walls or inlet/outlet around:
+    tiny face lists and halos, most faces on the boundary."""
+    if dim == 2:
+        m = M.Mesh.gen_tri(2, 1, 0, 1, 0, 0.5, 0.1, 21, (M.BC_INLET, M.BC_OUTLET, M.BC_WALL, M.BC_SYMMETRY))
+    else:
+        m = M.Mesh.gen_tet(1, 1, 1, (0, 0, 0), (1, 1, 1), 0.1, 21,
+                           (M.BC_INLET, M.BC_OUTLET, M.BC_WALL, M.BC_SYMMETRY, M.BC_WALL, M.BC_WALL))
+    om = O.OracleMesh(m.raw())
+    geo = M.Prep(m).geometry()
+    cg, co = both_cfg(dim=dim, scheme=ROE, order=2, limiter=M.LIM_MINMOD, rk=2, cfl=0.4, q_inlet=inlet(dim))
+    Q = smooth_state(geo["ccen"], seed=4, mach=0.5)
+    Qr = Q.copy()
+    for _ in range(3):
+        Qr, _res = om.rk_step(co, Qr, om.cfl_dt(co, Qr))
+    out = []
+    for strategy in STRATEGIES:
+        s = M.Solver(m, cg)
+        s.strategy = strategy
+        s.set_state(Q)
+        s.step(3)
+        out.append(s.get_state())
+        s.close()
+    print(f"dim {dim}: {m.n_cells} cells, err vs oracle {rel_state_error(out[0], Qr):.2e}", flush=True)
+    assert rel_state_error(out[0], Qr) <= TOL
+    for k in range(1, len(out)):
+        assert np.array_equal(out[k], out[0]), STRATEGIES[k]
