@@ -208,11 +208,31 @@ int mst_step(mst_ctx *ctx, int nsteps);
 #define MST_STRATEGY_FUSED 4
 int mst_set_strategy(mst_ctx *ctx, int strategy);
 int mst_get_strategy(const mst_ctx *ctx, int *strategy);
-/* NEW: steady driver.  Steps of mst_step until max_f L2(R/V)_f / ref_f <= tol (checked every
+/* NEW: steady driver.  Steps of mst_step (or LU-SGS iterations after
+ * mst_set_steady_driver(MST_STEADY_LUSGS)) until max_f L2(R/V)_f / ref_f <= tol (checked every
  * check_every steps) or max_steps.  ref[nv]: reference norms; when ref[0] <= 0 on entry the first
  * step's norms are taken and written back.  steps_done / drop: steps taken and the final ratio. */
 int mst_run_to_steady(mst_ctx *ctx, double tol, int max_steps, int check_every, double *ref, int *steps_done,
                       double *drop);
+/* NEW: implicit steady driver.  The reference's LU-SGS (SparseSolverNUM::solveILUSGS,
+ * lusolver/SparseSolverNUM.cpp:144-196) factorises assembled scalar systems of its pressure-based
+ * path; here the same (D+L) D^-1 (D+U) factorisation is applied matrix-free to the density-based
+ * residual (Jameson-Yoon scalar diagonal, Rusanov-split face Jacobians), over a greedy colouring of
+ * the cell graph: one launch per colour and sweep direction, deterministic.
+ * Per iteration: R(Q) with the configured scheme/order/limiter (its L2(R/V) norms as mst_step's),
+ * r_f = max over the face's cells of |u.S_f| + a|S_f|, D_i = (1/cfl + omega/2) sum_f r_f,
+ * forward sweep (-R - sum_L) / D, backward sweep, Q += dQ.  cfl is the local pseudo-time CFL number
+ * (large values approach Newton-like LU-SGS); omega >= 1 keeps D dominant.
+ * mst_set_steady_driver: MST_STEADY_EXPLICIT (mst_step, the default) or MST_STEADY_LUSGS for
+ * mst_run_to_steady. */
+#define MST_STEADY_EXPLICIT 0
+#define MST_STEADY_LUSGS 1
+int mst_set_steady_driver(mst_ctx *ctx, int driver, double cfl, double omega);
+/* n_iter LU-SGS iterations (Old -> New, then swap, like mst_step); cfl/omega from
+ * mst_set_steady_driver (defaults 1e4 and 1.0) */
+int mst_lusgs_step(mst_ctx *ctx, int n_iter);
+/* the cell colouring the sweeps use, file order (colour[n_cells]), and the number of colours */
+int mst_lusgs_colours(mst_ctx *ctx, int32_t *colour, int64_t n_cells, int *n_colours);
 /* NEW: dt = cfl * min_c V_c / sum_d (|u_d| + a) Shalf_c,d on the Old state */
 int mst_compute_dt(mst_ctx *ctx, double cfl, double *dt);
 /* NEW: L2 norm over cells of R(Q_old)/V per field (nv values) from the first stage of the last step */
@@ -321,7 +341,8 @@ int mst_ctx_warm_start(mst_ctx *ctx, mst_mlp *m, const double *cond, int n_cond)
 #define MST_K_CELL 4 /* fused flux + gather + update (MST_STRATEGY_CELL) */
 #define MST_K_PATCH 5 /* patch fluxes in LDS + gather + update (MST_STRATEGY_PATCH) */
 #define MST_K_FUSED 6 /* gradients + face states + fluxes in LDS + gather + update (MST_STRATEGY_FUSED) */
-#define MST_K_COUNT 7
+#define MST_K_LUSGS 7 /* LU-SGS face spectral radii and colour sweeps (mst_lusgs_step) */
+#define MST_K_COUNT 8
 int mst_timing_enable(mst_ctx *ctx, int on);
 /* per kernel class: launches timed and total milliseconds since the last reset */
 int mst_timing_get(mst_ctx *ctx, int kernel, int64_t *count, double *total_ms);

@@ -494,6 +494,20 @@ __device__ __forceinline__ void stage_tail(const UpdArgs &a, int64_t c, bool val
                                            int64_t blk, const bool L2, const bool DT, const StageIn<NV> *in = nullptr) {
     if constexpr (!STAGE) {
         if (valid) stq<NV>(a.R + c * NV, r);
+        if (L2) {  // residual-only evaluation with its norms (the implicit steady driver)
+            double v[NV];
+#pragma unroll
+            for (int k = 0; k < NV; k++) v[k] = 0;
+            if (valid) {
+                const double iV = frcp(a.vol[c]);
+#pragma unroll
+                for (int k = 0; k < NV; k++) {
+                    const double x = r[k] * iV;
+                    v[k] = x * x;
+                }
+            }
+            block_partial<NV, false>(v, a.part_l2 + blk, gridDim.x);
+        }
         return;
     } else {
         double red[NV + 1];

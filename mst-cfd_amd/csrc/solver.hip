@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "lusgs.hpp"
 #include "mlp.hpp"
 #include "mstcfd.h"
 #include "prep.hpp"
@@ -91,6 +92,13 @@ struct mst_ctx {
     double *fin_mid = nullptr;  // first-level results of the two-level finish
     bool dt_valid = false;   // scal[0] holds the CFL time step of the current Old state
     int *bad = nullptr;
+    // LU-SGS steady driver (built on first use): cell colours, the cells of each colour in ascending
+    // order (colour k: lu_cells[lu_start[k] .. lu_start[k + 1])), face spectral radii, the update
+    int32_t *lu_colour = nullptr, *lu_cells = nullptr;
+    std::vector<int64_t> lu_start;
+    double *lu_rf = nullptr, *lu_dQ = nullptr;
+    int steady = MST_STEADY_EXPLICIT;
+    double lu_cfl = 1e4, lu_omega = 1.0;
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[MST_K_COUNT];
     std::vector<hipEvent_t> pool;
@@ -271,6 +279,7 @@ int run_update_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int
 #define MST_UPD(MF)                                                                                             \
     switch (mode & ~UPD_CHECK) {                                                                                             \
         case UPD_RESIDUAL: return launch(x, MST_K_UPDATE, k_update<NV, MF, UPD_RESIDUAL, GT>, x->nc, a);        \
+        case UPD_L2: return launch(x, MST_K_UPDATE, k_update<NV, MF, UPD_L2, GT>, x->nc, a);                    \
         case UPD_STAGE: return launch(x, MST_K_UPDATE, k_update<NV, MF, UPD_STAGE, GT>, x->nc, a);              \
         case UPD_STAGE | UPD_L2: return launch(x, MST_K_UPDATE, k_update<NV, MF, UPD_STAGE | UPD_L2, GT>, x->nc, a); \
         case UPD_STAGE | UPD_DT: return launch(x, MST_K_UPDATE, k_update<NV, MF, UPD_STAGE | UPD_DT, GT>, x->nc, a); \
@@ -671,7 +680,8 @@ void release(mst_ctx *x) {
                     x->cshalf, x->Q[0], x->Q[1], x->Q[2], x->Q[3], x->G, x->rec, x->graw, x->phi, x->R,
                     x->scal, x->part, x->fin_mid, x->bad, x->Q32[0], x->Q32[1], x->Q32[2], x->Q32[3], x->rec32,
                     x->frec0, x->frec1, x->inc_code, x->inc_w, x->inc_n, x->inc_dx, x->inc_w32, x->inc_n32,
-                    x->inc_dx32, x->pstart, x->pface, x->ploc, x->hstart, x->hcell, x->hloc};
+                    x->inc_dx32, x->pstart, x->pface, x->ploc, x->hstart, x->hcell, x->hloc,
+                    x->lu_colour, x->lu_cells, x->lu_rf, x->lu_dQ};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (int k = 0; k < MST_K_COUNT; k++)
@@ -712,6 +722,93 @@ int check_bad(mst_ctx *x) {
         return mst::fail(MST_E_STATE, "non-physical state (rho<=0, p<=0 or NaN, or no positive time step) produced by the update");
     }
     return MST_OK;
+}
+
+// LU-SGS: greedy colouring of the cell graph in device order (each cell takes the smallest colour
+// none of its lower-numbered neighbours has; at most maxf + 1 colours) and each colour's cells in
+// ascending order.  Built once per context, on the first implicit iteration.
+int build_colours(mst_ctx *x) {
+    if (x->lu_colour) return MST_OK;
+    const auto &P = x->prep.P;
+    const int64_t nc = x->nc;
+    std::vector<int32_t> col(nc, -1);
+    int ncol = 0;
+    for (int64_t c = 0; c < nc; c++) {
+        uint32_t used = 0;
+        for (int s = 0; s < P.maxf; s++) {
+            const int32_t e = P.ell[(int64_t)s * nc + c];
+            if (e < 0) break;
+            const int32_t f = e >> 1;
+            const int32_t o = (e & 1) ? P.fc0[f] : P.fc1[f];
+            if (o >= 0 && o != c && col[o] >= 0) used |= 1u << col[o];
+        }
+        int k = 0;
+        while ((used >> k) & 1u) k++;
+        col[c] = k;
+        ncol = std::max(ncol, k + 1);
+    }
+    std::vector<int64_t> start(ncol + 1, 0);
+    for (int64_t c = 0; c < nc; c++) start[col[c] + 1]++;
+    for (int k = 0; k < ncol; k++) start[k + 1] += start[k];
+    std::vector<int32_t> cells(nc);
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    for (int64_t c = 0; c < nc; c++) cells[fill[col[c]]++] = (int32_t)c;
+    int rc = dupload(&x->lu_colour, col);
+    if (rc == MST_OK) rc = dupload(&x->lu_cells, cells);
+    if (rc == MST_OK) rc = dalloc(&x->lu_rf, (size_t)std::max<int64_t>(x->nf, 1));
+    if (rc == MST_OK) rc = dalloc(&x->lu_dQ, (size_t)(nc * x->nv));
+    if (rc == MST_OK && !x->R) rc = dalloc(&x->R, (size_t)(nc * x->nv));
+    if (rc) return rc;
+    x->lu_start = start;
+    return MST_OK;
+}
+
+template <int DIM, int MAXF>
+int lu_sweeps(mst_ctx *x, const LuArgs &a) {
+    int rc = launch(x, MST_K_LUSGS, k_lu_radius<DIM>, x->nf, a, x->nf, x->lu_rf);
+    const int ncol = (int)x->lu_start.size() - 1;
+    for (int k = 0; k < ncol && rc == MST_OK; k++)
+        rc = launch(x, MST_K_LUSGS, k_lu_sweep<DIM, true, MAXF>, x->lu_start[k + 1] - x->lu_start[k], a,
+                    (const int32_t *)(x->lu_cells + x->lu_start[k]), x->lu_start[k + 1] - x->lu_start[k], k);
+    for (int k = ncol - 1; k >= 0 && rc == MST_OK; k--)
+        rc = launch(x, MST_K_LUSGS, k_lu_sweep<DIM, false, MAXF>, x->lu_start[k + 1] - x->lu_start[k], a,
+                    (const int32_t *)(x->lu_cells + x->lu_start[k]), x->lu_start[k + 1] - x->lu_start[k], k);
+    return rc;
+}
+
+// one LU-SGS iteration: residual of Old (+ its L2 norms), spectral radii, the two sweeps into New
+int lu_iteration(mst_ctx *x) {
+    int rc = run_face_flux(x, x->iold);
+    if (rc) return rc;
+    rc = run_update(x, UPD_L2, x->iold, -1, 0, 0, 1, -1);
+    if (rc) return rc;
+    rc = run_finish(x, x->nv, 0);
+    if (rc) return rc;
+    LuArgs a;
+    a.nc = x->nc;
+    a.ell = x->ell;
+    a.fc0 = x->fc0;
+    a.fc1 = x->fc1;
+    a.fn = x->fn;
+    a.colour = x->lu_colour;
+    a.rf = x->lu_rf;
+    a.Q = x->Q[x->iold];
+    a.R = x->R;
+    a.dQ = x->lu_dQ;
+    a.Qout = x->Q[x->inew];
+    a.bad = x->bad;
+    a.gamma = x->cfg.gamma;
+    a.omega = x->lu_omega;
+    a.dscale = 1.0 / x->lu_cfl + 0.5 * x->lu_omega;
+    // the sweeps unroll over the padded slot count of the device table (3, 4 or 8)
+    if (x->dim == 2) {
+        if (x->slots == 3) return lu_sweeps<2, 3>(x, a);
+        if (x->slots == 4) return lu_sweeps<2, 4>(x, a);
+        return lu_sweeps<2, 8>(x, a);
+    }
+    if (x->slots == 3) return lu_sweeps<3, 3>(x, a);
+    if (x->slots == 4) return lu_sweeps<3, 4>(x, a);
+    return lu_sweeps<3, 8>(x, a);
 }
 
 // AUTO: time one stage of each strategy on this mesh and configuration (outputs go to a spare state
@@ -993,12 +1090,14 @@ int mst_run_to_steady(mst_ctx *ctx, double tol, int max_steps, int check_every, 
                       double *drop) {
     CTX_GUARD(ctx);
     if (!ref || !(tol > 0) || max_steps < 1 || check_every < 1) return mst::fail(MST_E_ARG, "bad arguments");
-    if (!(ctx->cfg.cfl > 0)) return mst::fail(MST_E_ARG, "steady runs advance with the CFL time step (cfl > 0)");
+    const bool lu = ctx->steady == MST_STEADY_LUSGS;
+    if (!lu && !(ctx->cfg.cfl > 0))
+        return mst::fail(MST_E_ARG, "explicit steady runs advance with the CFL time step (cfl > 0)");
     const int nv = ctx->nv;
     double r[5];
     int steps = 0;
     if (!(ref[0] > 0)) {  // reference: the residual of the first step
-        int rc = mst_step(ctx, 1);
+        int rc = lu ? mst_lusgs_step(ctx, 1) : mst_step(ctx, 1);
         if (rc) return rc;
         rc = mst_residual_norms(ctx, ref);
         if (rc) return rc;
@@ -1007,7 +1106,7 @@ int mst_run_to_steady(mst_ctx *ctx, double tol, int max_steps, int check_every, 
     double ratio = INFINITY;
     while (steps < max_steps) {
         const int k = std::min(check_every, max_steps - steps);
-        int rc = mst_step(ctx, k);
+        int rc = lu ? mst_lusgs_step(ctx, k) : mst_step(ctx, k);
         if (rc) return rc;
         steps += k;
         rc = mst_residual_norms(ctx, r);
@@ -1018,6 +1117,48 @@ int mst_run_to_steady(mst_ctx *ctx, double tol, int max_steps, int check_every, 
     }
     if (steps_done) *steps_done = steps;
     if (drop) *drop = ratio;
+    return MST_OK;
+}
+
+int mst_set_steady_driver(mst_ctx *ctx, int driver, double cfl, double omega) {
+    CTX_GUARD(ctx);
+    if (driver != MST_STEADY_EXPLICIT && driver != MST_STEADY_LUSGS)
+        return mst::fail(MST_E_ARG, "driver must be MST_STEADY_EXPLICIT or MST_STEADY_LUSGS");
+    if (driver == MST_STEADY_LUSGS && (!(cfl > 0) || !(omega > 0) || !std::isfinite(cfl)))
+        return mst::fail(MST_E_ARG, "LU-SGS needs a finite cfl > 0 and omega > 0");
+    ctx->steady = driver;
+    if (driver == MST_STEADY_LUSGS) {
+        ctx->lu_cfl = cfl;
+        ctx->lu_omega = omega;
+    }
+    return MST_OK;
+}
+
+int mst_lusgs_step(mst_ctx *ctx, int n_iter) {
+    CTX_GUARD(ctx);
+    if (n_iter < 0) return mst::fail(MST_E_ARG, "n_iter must be >= 0");
+    int rc = build_colours(ctx);
+    if (rc) return rc;
+    for (int it = 0; it < n_iter; it++) {
+        rc = lu_iteration(ctx);
+        if (rc) return rc;
+        ctx->shadow_ok[ctx->inew] = false;
+        std::swap(ctx->iold, ctx->inew);
+    }
+    if (n_iter > 0) ctx->dt_valid = false;
+    return check_bad(ctx);
+}
+
+int mst_lusgs_colours(mst_ctx *ctx, int32_t *colour, int64_t n_cells, int *n_colours) {
+    CTX_GUARD(ctx);
+    if (!colour || !n_colours || n_cells != ctx->nc) return mst::fail(MST_E_ARG, "size does not match the mesh");
+    int rc = build_colours(ctx);
+    if (rc) return rc;
+    std::vector<int32_t> buf(ctx->nc);
+    HIPCHK(hipMemcpy(buf.data(), ctx->lu_colour, sizeof(int32_t) * buf.size(), hipMemcpyDeviceToHost));
+    const auto &new_of_old = ctx->prep.P.cell_new_of_old;
+    for (int64_t c = 0; c < ctx->nc; c++) colour[c] = buf[new_of_old[c]];
+    *n_colours = (int)ctx->lu_start.size() - 1;
     return MST_OK;
 }
 

@@ -24,13 +24,14 @@ PREC_FP64, PREC_MIXED = 0, 1
 COMPAT_BRANCH2_LR, COMPAT_GRAD_LASTINT, COMPAT_AUSM_LOCAL_A, COMPAT_AUSM_P_UNSCALED = 1, 2, 4, 8
 COMPAT_AUSM_ONESIDED = 16
 COMPAT_REFERENCE = COMPAT_GRAD_LASTINT | COMPAT_AUSM_LOCAL_A | COMPAT_AUSM_P_UNSCALED | COMPAT_AUSM_ONESIDED
-K_GRAD, K_FLUX, K_UPDATE, K_DT, K_CELL, K_PATCH, K_FUSED = 0, 1, 2, 3, 4, 5, 6
+K_GRAD, K_FLUX, K_UPDATE, K_DT, K_CELL, K_PATCH, K_FUSED, K_LUSGS = 0, 1, 2, 3, 4, 5, 6, 7
 KERNEL_NAMES = {K_GRAD: "k_grad", K_FLUX: "k_flux", K_UPDATE: "k_update", K_DT: "k_cfl", K_CELL: "k_cell",
-                K_PATCH: "k_patch", K_FUSED: "k_fused"}
+                K_PATCH: "k_patch", K_FUSED: "k_fused", K_LUSGS: "k_lu_sweep"}
 STRATEGY_FACE, STRATEGY_CELL, STRATEGY_AUTO, STRATEGY_PATCH, STRATEGY_FUSED = 0, 1, 2, 3, 4
 INIT_REFERENCE, INIT_UNIFORM, INIT_RIEMANN, INIT_VORTEX, INIT_PULSE, INIT_FREESTREAM = 0, 1, 2, 3, 4, 5
 BATCH_BUCKETS = 4
 ORDER_HILBERT, ORDER_RCM = 0, 1
+STEADY_EXPLICIT, STEADY_LUSGS = 0, 1
 
 # public exported symbols of include/mstcfd.h (checked by the CPU tests)
 EXPORTS = [
@@ -50,7 +51,7 @@ EXPORTS = [
     "mst_mlp_set_scaling", "mst_mlp_set_data", "mst_mlp_reset_optimizer", "mst_mlp_train", "mst_mlp_forward",
     "mst_mlp_gradient", "mst_ctx_warm_start", "mst_run_to_steady", "mst_batch_run_to_steady",
     "mst_mesh_cell_centres", "mst_batch_set_strategy", "mst_batch_get_strategy", "mst_set_cell_order",
-    "mst_get_cell_order",
+    "mst_get_cell_order", "mst_set_steady_driver", "mst_lusgs_step", "mst_lusgs_colours",
 ]
 
 
@@ -190,6 +191,9 @@ def lib():
                                         C.POINTER(C.c_double)]),
         "mst_batch_run_to_steady": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, F64P, I32P, F64P]),
         "mst_mesh_cell_centres": (C.c_int, [vp, F64P, C.c_int64]),
+        "mst_set_steady_driver": (C.c_int, [vp, C.c_int, C.c_double, C.c_double]),
+        "mst_lusgs_step": (C.c_int, [vp, C.c_int]),
+        "mst_lusgs_colours": (C.c_int, [vp, I32P, C.c_int64, C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -372,6 +376,21 @@ class Solver:
         _check(lib().mst_run_to_steady(self.h, float(tol), int(max_steps), int(check_every), r, C.byref(n),
                                        C.byref(d)), "mst_run_to_steady")
         return n.value, d.value, r
+
+    def set_steady_driver(self, driver, cfl=1e4, omega=1.0):
+        """MST_STEADY_EXPLICIT (mst_step) or MST_STEADY_LUSGS (implicit, pseudo-time CFL cfl) for run_to_steady."""
+        _check(lib().mst_set_steady_driver(self.h, int(driver), float(cfl), float(omega)), "mst_set_steady_driver")
+
+    def lusgs_step(self, n=1):
+        """n LU-SGS iterations (Old -> New, swapped)."""
+        _check(lib().mst_lusgs_step(self.h, int(n)), "mst_lusgs_step")
+
+    def lusgs_colours(self):
+        """(colour per cell in file order, number of colours) of the LU-SGS sweeps."""
+        col = np.zeros(self.n_cells, np.int32)
+        n = C.c_int()
+        _check(lib().mst_lusgs_colours(self.h, col, self.n_cells, C.byref(n)), "mst_lusgs_colours")
+        return col, n.value
 
     def warm_start(self, mlp, cond):
         """Old state := the network's decoded prediction at the cell centres (mst_ctx_warm_start)."""

@@ -841,3 +841,138 @@ int orc_monitor_maxrel(int64_t n_cells, int nv, const double *Qnew, const double
         }
     return 0;
 }
+
+/* ---------------------------------------------------------------- LU-SGS steady driver (NEW)
+ * Matrix-free LU-SGS pseudo-time iteration.  The factorised operator (D+L) D^-1 (D+U) -- a forward
+ * sweep with the lower part, a division by D, a backward sweep with the upper part -- is the one
+ * SparseSolverNUM::solveILUSGS applies to its assembled scalar systems
+ * (lusolver/SparseSolverNUM.cpp:144-196).  Here the blocks are never assembled: they come from the
+ * Rusanov split of the face-flux Jacobian (Jameson & Yoon, AIAA J. 25(7), 929-935, 1987):
+ *   r_f  = max over the face's cells of |u.S_f| + a |S_f|     (a boundary face: its one cell)
+ *   D_i  = (1/CFL + omega/2) sum_f r_f                         (local step dt_i = CFL V_i / sum_f r_f)
+ *   (i,j) block times dQ_j = 1/2 [F_n(Q_j + dQ_j) - F_n(Q_j)] - 1/2 omega r_f dQ_j,
+ *        n = the outward normal of cell i at f, F_n = sum_d n_d F_d (physical Euler flux)
+ * L / U hold the neighbours of lower / higher colour.  Boundary faces enter D only.  Cells are swept
+ * colour by colour, ascending then descending; no two cells of one colour are neighbours, so the
+ * order inside a colour does not change the result. */
+static double lu_radius(const double *q, const double *n, int dim, double gamma) {
+    double un = 0, s2 = 0;
+    for (int d = 0; d < dim; d++) {
+        un += q[1 + d] * n[d];
+        s2 += n[d] * n[d];
+    }
+    const double p = orc_pressure(q, dim, gamma);
+    return fabs(un / q[0]) + sqrt(gamma * p / q[0]) * sqrt(s2);
+}
+
+static void lu_flux(const double *q, const double *n, int dim, double gamma, double *F) {
+    const int nv = dim + 2;
+    double un = 0;
+    for (int d = 0; d < dim; d++) un += q[1 + d] * n[d];
+    un /= q[0];
+    const double p = orc_pressure(q, dim, gamma);
+    F[0] = q[0] * un;
+    for (int d = 0; d < dim; d++) F[1 + d] = q[1 + d] * un + p * n[d];
+    F[nv - 1] = (q[nv - 1] + p) * un;
+}
+
+/* the cell across face f from cell c (periodic: the partner face's cell), -1 on a boundary */
+static int64_t lu_neighbour(const orc_mesh *m, int64_t c, int32_t f) {
+    if (m->c1[f] >= 0) return c == m->c0[f] ? m->c1[f] : m->c0[f];
+    if (m->ftype[f] == ORC_PERIODIC && m->partner[f] >= 0) return m->c0[m->partner[f]];
+    return -1;
+}
+
+/* s -= 1/2 [F_n(q + dq) - F_n(q)] - 1/2 omega r dq */
+static void lu_offdiag(const double *q, const double *dq, const double *n, double r, int dim, double gamma,
+                       double omega, double *s) {
+    const int nv = dim + 2;
+    double qp[MAXNV], F1[MAXNV], F0[MAXNV];
+    for (int k = 0; k < nv; k++) qp[k] = q[k] + dq[k];
+    lu_flux(qp, n, dim, gamma, F1);
+    lu_flux(q, n, dim, gamma, F0);
+    for (int k = 0; k < nv; k++) s[k] -= 0.5 * ((F1[k] - F0[k]) - omega * r * dq[k]);
+}
+
+int orc_lusgs_sweep(const orc_mesh *m, const orc_config *cfg, const double *Q, const double *R,
+                    const int32_t *colour, double cfl, double omega, double *dQ) {
+    const int nv = m->nv, dim = m->dim;
+    const double g = cfg->gamma;
+    if (!(cfl > 0) || !(omega > 0)) return -1;
+    int ncol = 0;
+    for (int64_t c = 0; c < m->nc; c++) {
+        if (colour[c] < 0) return -1;
+        if (colour[c] + 1 > ncol) ncol = colour[c] + 1;
+    }
+    double *rf = (double *)malloc(sizeof(double) * (m->nf > 0 ? m->nf : 1));
+    if (!rf) return -1;
+    for (int64_t f = 0; f < m->nf; f++) {
+        const double *n = m->S + f * dim;
+        double r = lu_radius(Q + (int64_t)m->c0[f] * nv, n, dim, g);
+        const int64_t o = lu_neighbour(m, m->c0[f], (int32_t)f);
+        if (o >= 0) {
+            const double r1 = lu_radius(Q + o * nv, n, dim, g);
+            if (r1 > r) r = r1;
+        }
+        rf[f] = r;
+    }
+    const double dscale = 1.0 / cfl + 0.5 * omega;
+    for (int pass = 0; pass < 2; pass++) {
+        for (int step = 0; step < ncol; step++) {
+            const int col = pass == 0 ? step : ncol - 1 - step;
+            for (int64_t c = 0; c < m->nc; c++) {
+                if (colour[c] != col) continue;
+                double s[MAXNV], nrm[MAXD], rsum = 0;
+                for (int k = 0; k < nv; k++) s[k] = pass == 0 ? -R[c * nv + k] : 0.0;
+                for (int64_t j = m->cptr[c]; j < m->cptr[c + 1]; j++) {
+                    const int32_t f = m->cface[j];
+                    rsum += rf[f];
+                    const int64_t o = lu_neighbour(m, c, f);
+                    if (o < 0 || o == c) continue;
+                    if (pass == 0 ? colour[o] < col : colour[o] > col) {
+                        for (int d = 0; d < dim; d++) nrm[d] = nout(m, c, f, d);
+                        lu_offdiag(Q + o * nv, dQ + o * nv, nrm, rf[f], dim, g, omega, s);
+                    }
+                }
+                const double D = dscale * rsum;
+                for (int k = 0; k < nv; k++) {
+                    if (pass == 0)
+                        dQ[c * nv + k] = s[k] / D;
+                    else
+                        dQ[c * nv + k] += s[k] / D;
+                }
+            }
+        }
+    }
+    free(rf);
+    return 0;
+}
+
+int orc_lusgs_step(const orc_mesh *m, const orc_config *cfg, double *Q, const int32_t *colour, double cfl,
+                   double omega, double *res_l2) {
+    const int nv = m->nv;
+    const size_t n = (size_t)m->nc * nv;
+    double *R = (double *)malloc(sizeof(double) * (n ? n : 1));
+    double *dQ = (double *)malloc(sizeof(double) * (n ? n : 1));
+    if (!R || !dQ) {
+        free(R);
+        free(dQ);
+        return -1;
+    }
+    orc_residual(m, cfg, Q, R, NULL);
+    if (res_l2) {
+        for (int k = 0; k < nv; k++) res_l2[k] = 0;
+        for (int64_t c = 0; c < m->nc; c++)
+            for (int k = 0; k < nv; k++) {
+                const double r = R[c * nv + k] / m->vol[c];
+                res_l2[k] += r * r;
+            }
+        for (int k = 0; k < nv; k++) res_l2[k] = sqrt(res_l2[k] / (double)m->nc);
+    }
+    int rc = orc_lusgs_sweep(m, cfg, Q, R, colour, cfl, omega, dQ);
+    if (rc == 0)
+        for (size_t i = 0; i < n; i++) Q[i] += dQ[i];
+    free(R);
+    free(dQ);
+    return rc;
+}

@@ -87,6 +87,14 @@ double orc_cfl_dt(const orc_mesh *m, const orc_config *cfg, const double *Q);
 int orc_rk_step(const orc_mesh *m, const orc_config *cfg, double *Q, double dt, double *res_l2);
 int orc_monitor_maxrel(int64_t n_cells, int nv, const double *Qnew, const double *Qold, double *out);
 
+/* NEW steady driver: matrix-free LU-SGS over a cell colouring (colour[c] >= 0 per file-order cell,
+ * no two neighbours alike).  sweep: dQ from a given residual R; step: R(Q), res_l2 (as orc_rk_step),
+ * the sweeps and Q += dQ. */
+int orc_lusgs_sweep(const orc_mesh *m, const orc_config *cfg, const double *Q, const double *R,
+                    const int32_t *colour, double cfl, double omega, double *dQ);
+int orc_lusgs_step(const orc_mesh *m, const orc_config *cfg, double *Q, const int32_t *colour, double cfl,
+                   double omega, double *res_l2);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,6 +92,8 @@ def lib():
     L.orc_cfl_dt.restype = C.c_double
     L.orc_rk_step.argtypes = [C.c_void_p, C.POINTER(Config), F64P, C.c_double, F64P]
     L.orc_monitor_maxrel.argtypes = [C.c_int64, C.c_int, F64P, F64P, F64P]
+    L.orc_lusgs_sweep.argtypes = [C.c_void_p, C.POINTER(Config), F64P, F64P, I32P, C.c_double, C.c_double, F64P]
+    L.orc_lusgs_step.argtypes = [C.c_void_p, C.POINTER(Config), F64P, I32P, C.c_double, C.c_double, F64P]
     _LIB = L
     return L
 
@@ -176,6 +178,24 @@ class OracleMesh:
         Qc = np.array(Q, dtype=np.float64, copy=True).reshape(self.n_cells, self.nv)
         res = np.zeros(self.nv)
         lib().orc_rk_step(self.h, C.byref(cfg), Qc.reshape(-1), float(dt), res)
+        return Qc, res
+
+    def lusgs_sweep(self, cfg, Q, R, colour, cfl, omega):
+        dQ = np.zeros((self.n_cells, self.nv))
+        rc = lib().orc_lusgs_sweep(self.h, C.byref(cfg), np.ascontiguousarray(Q, np.float64).reshape(-1),
+                                   np.ascontiguousarray(R, np.float64).reshape(-1),
+                                   np.ascontiguousarray(colour, np.int32), float(cfl), float(omega), dQ.reshape(-1))
+        if rc:
+            raise ValueError("orc_lusgs_sweep: bad colouring, cfl or omega")
+        return dQ
+
+    def lusgs_step(self, cfg, Q, colour, cfl, omega):
+        Qc = np.array(Q, dtype=np.float64, copy=True).reshape(self.n_cells, self.nv)
+        res = np.zeros(self.nv)
+        rc = lib().orc_lusgs_step(self.h, C.byref(cfg), Qc.reshape(-1), np.ascontiguousarray(colour, np.int32),
+                                  float(cfl), float(omega), res)
+        if rc:
+            raise ValueError("orc_lusgs_step: bad colouring, cfl or omega")
         return Qc, res
 
 
