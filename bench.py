@@ -200,25 +200,118 @@ def bench_ramp(M, args):
     """Learned warm start (compression ramp): 256 solver-generated steady cases (75,000 triangles
     each, solved in one batch), a 4 x 128 network trained on the GPU, then 16 held-out cases on the
     300,000-triangle ramp from the freestream and from the network's prediction to the same
-    residual level (1e-8 x the freestream run's first-step L2(R/V), every field)."""
+    residual level (1e-8 x the freestream run's first-step L2(R/V), every field), with the explicit
+    SSP-RK2 driver and with the LU-SGS driver.  value: the warm start's gain under LU-SGS; the
+    summary also holds the explicit driver's warm-start gain, the LU-SGS driver's gain over the
+    explicit one (both from the freestream) and the two combined."""
     from mstcfd import ramp as RP
     t0 = time.perf_counter()
     train_cases = RP.sample_cases(256, 2024)
     X, T, gen = RP.generate(train_cases, 300, 125, max_steps=20000, check_every=500)
     net, tr = RP.train(X, T)
     test_cases = RP.sample_cases(16, 2025)
-    rows = RP.evaluate(net, test_cases, 600, 250, max_steps=40000)
+    rows = RP.evaluate(net, test_cases, 600, 250, max_steps=40000, lusgs=(1e4, 1.0))
     sm = RP.summary(rows)
-    out = dict(metric="time-to-1e-8 residual drop, freestream start / learned warm start",
-               value=sm["geomean_speedup"], unit="x (geometric mean over held-out cases that converge)",
+    out = dict(metric="time-to-1e-8 residual drop, freestream start / learned warm start (LU-SGS steady driver)",
+               value=sm.get("lusgs_geomean_warm_speedup"), unit="x (geometric mean over held-out cases that converge)",
                n_gpus=1, steps=None, warmup=None, ms_per_step=None, higher_is_better=True, scaling="strong",
                vs_baseline=None, dtype="f64 solver / f32 network", data="synthetic (solver-generated)",
-               config=dict(workload="steady supersonic compression ramp, 600x250x2 = 300,000 triangles, first-order Roe, "
-                                    "SSP-RK2, CFL 0.8; M ~ U[1.5,3], theta ~ U[5,20] deg",
+               config=dict(workload="steady supersonic compression ramp, 600x250x2 = 300,000 triangles, first-order Roe; "
+                                    "LU-SGS (CFL 1e4, omega 1) and SSP-RK2 (CFL 0.8) steady drivers; "
+                                    "M ~ U[1.5,3], theta ~ U[5,20] deg",
                            training=gen, network=dict(tr, layers="4-128-128-128-128-4 tanh"),
                            wall_s=time.perf_counter() - t0),
                summary=sm, cases=rows)
     print(json.dumps(out))
+
+
+def bench_steady(M, args):
+    """Steady supersonic compression ramp (the C5 mesh: 600x250x2 = 300,000 triangles; M 2.51,
+    theta 9.3 deg, first-order Roe) from the freestream to 1e-8 x the first-step L2(R/V) of every
+    field, with the explicit SSP-RK2 driver (CFL 0.8, device time step) and with the LU-SGS driver
+    (pseudo-time CFL 1e4, omega 1).  value: the LU-SGS time to that level (inputs resident in HBM,
+    the colouring built before the timed run); the line also holds the explicit time, the
+    iteration counts, the per-kernel times of LU-SGS iterations and the sweeps' roofline."""
+    from mstcfd import ramp as RP
+    mach, theta, tol = 2.51, 9.3, 1e-8
+    m = RP.mesh(theta, 600, 250)
+    s = M.Solver(m, RP.config(m, mach))
+    q0 = RP.freestream(m, mach)
+    runs = {}
+    ref = None
+    for name, drv in (("explicit", M.STEADY_EXPLICIT), ("lusgs", M.STEADY_LUSGS)):
+        s.set_steady_driver(drv, 1e4, 1.0)
+        if drv == M.STEADY_LUSGS:
+            _, ncol = s.lusgs_colours()
+        s.set_state(q0)
+        s.sync()
+        t0 = time.perf_counter()
+        steps, drop, r = s.run_to_steady(tol, 60000 if drv == M.STEADY_EXPLICIT else 20000,
+                                         200 if drv == M.STEADY_EXPLICIT else 50, ref)
+        s.sync()
+        runs[name] = dict(seconds=time.perf_counter() - t0, iterations=int(steps), drop=float(drop),
+                          converged=bool(drop <= tol))
+        ref = r if ref is None else ref
+    # per-kernel times of LU-SGS iterations (dispatch-carried HIP events on the solver's stream)
+    s.set_state(q0)
+    s.lusgs_step(args.warmup)
+    s.sync()
+    s.timing_reset()
+    s.timing(True)
+    s.lusgs_step(args.steps)
+    s.sync()
+    s.timing(False)
+    kern = {}
+    for k, name in M.KERNEL_NAMES.items():
+        n, ms = s.timing_get(k)
+        if n:
+            kern[name] = dict(launches=n, avg_us=1e3 * ms / n, per_iteration_us=1e3 * ms / args.steps)
+            if k == M.K_LUSGS:  # the residual kernels run in residual-only mode here
+                b = s.algorithmic_bytes(k)
+                kern[name].update(alg_bytes=b, gbs=b / (1e-3 * ms / n) / 1e9)
+    sw = kern["k_lu_sweep"]
+    ach = sw["alg_bytes"] / (sw["avg_us"] * 1e-6) / 1e9
+    lu, ex = runs["lusgs"], runs["explicit"]
+    out = dict(metric="time-to-1e-8 residual drop from the freestream (steady ramp, LU-SGS driver)",
+               value=lu["seconds"], unit="s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+               ms_per_step=1e3 * lu["seconds"] / max(lu["iterations"], 1), higher_is_better=False,
+               scaling="strong", vs_baseline=None, dtype="f64", data="synthetic",
+               config=dict(workload="steady supersonic compression ramp, 600x250x2 = 300,000 triangles, "
+                                    f"M {mach}, theta {theta} deg, first-order Roe; LU-SGS CFL 1e4, omega 1, "
+                                    f"{ncol} colours; explicit SSP-RK2 CFL 0.8 alongside",
+                           cells=m.n_cells, faces=m.n_faces),
+               drivers=runs, speedup_vs_explicit=ex["seconds"] / lu["seconds"],
+               roofline=dict(bound="hbm", kernel="k_lu_sweep", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
+                             frac=ach / HBM_PEAK_GBS, traffic=None, alg_bytes=sw["alg_bytes"]),
+               kernels=kern)
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = lusgs_cpu_baseline(m, mach, s.lusgs_colours()[0], lu["iterations"])
+    s.close()
+    print(json.dumps(out))
+
+
+def lusgs_cpu_baseline(m, mach, colour, iterations, budget_s=10.0):
+    """The oracle's LU-SGS iteration (OpenMP residual, sequential colour sweeps) on the same mesh and
+    colours: a bounded sample of iterations, scaled to the GPU run's iteration count."""
+    sys.path.insert(0, ROOT)
+    from mstcfd import ramp as RP
+    from oracle import pyoracle as O
+
+    threads = min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    om = O.OracleMesh(m.raw())
+    c = RP.config(m, mach)
+    co = O.make_config(scheme=O.ROE, order=1, q_inlet=[c.q_inlet[i] for i in range(4)])
+    Q = RP.freestream(m, mach)
+    n, t = 0, 0.0
+    while n < 1 or (t < budget_s and n < 100):
+        t0 = time.perf_counter()
+        Q, _ = om.lusgs_step(co, Q, colour, 1e4, 1.0)
+        t += time.perf_counter() - t0
+        n += 1
+    return dict(value=t / n * iterations, unit="s", cores=threads, kind="port",
+                sample=f"{n} LU-SGS iterations of the same 300,000-cell case ({t:.2f} s, oracle/mst_oracle.c, "
+                       f"OpenMP residual, sequential sweeps), scaled to the GPU run's {iterations} iterations")
 
 
 def main():
@@ -253,6 +346,8 @@ def main():
         return bench_batch(M, args)
     if args.workload == "ramp":
         return bench_ramp(M, args)
+    if args.workload == "steady":
+        return bench_steady(M, args)
     copy = None
     if args.calibrate_copy:
         M.copy_probe(args.calibrate_copy)  # first launch pays the code-object load

@@ -235,7 +235,8 @@ int mst_lusgs_step(mst_ctx *ctx, int n_iter);
 int mst_lusgs_colours(mst_ctx *ctx, int32_t *colour, int64_t n_cells, int *n_colours);
 /* NEW: dt = cfl * min_c V_c / sum_d (|u_d| + a) Shalf_c,d on the Old state */
 int mst_compute_dt(mst_ctx *ctx, double cfl, double *dt);
-/* NEW: L2 norm over cells of R(Q_old)/V per field (nv values) from the first stage of the last step */
+/* NEW: L2 norm over cells of R(Q_old)/V per field (nv values) from the first stage of the last step of
+ * the last mst_step call (or the last iteration of mst_lusgs_step): reduced once per call */
 int mst_residual_norms(mst_ctx *ctx, double *out);
 /* Time::goNextTimeStep monitor (time/Time.cpp:69-76): max_i |New-Old|/Old per field */
 int mst_monitor_maxrel(mst_ctx *ctx, double *out);
@@ -341,7 +342,7 @@ int mst_ctx_warm_start(mst_ctx *ctx, mst_mlp *m, const double *cond, int n_cond)
 #define MST_K_CELL 4 /* fused flux + gather + update (MST_STRATEGY_CELL) */
 #define MST_K_PATCH 5 /* patch fluxes in LDS + gather + update (MST_STRATEGY_PATCH) */
 #define MST_K_FUSED 6 /* gradients + face states + fluxes in LDS + gather + update (MST_STRATEGY_FUSED) */
-#define MST_K_LUSGS 7 /* LU-SGS face spectral radii and colour sweeps (mst_lusgs_step) */
+#define MST_K_LUSGS 7 /* LU-SGS colour sweeps (mst_lusgs_step); algorithmic bytes: per launch, averaged */
 #define MST_K_COUNT 8
 int mst_timing_enable(mst_ctx *ctx, int on);
 /* per kernel class: launches timed and total milliseconds since the last reset */

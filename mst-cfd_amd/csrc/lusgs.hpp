@@ -14,26 +14,30 @@
 // Sequential Gauss-Seidel order is replaced by colour order: no two cells of one colour are
 // neighbours, so one launch per colour updates all of its cells at once, and the result is the same
 // as sweeping the cells one by one in colour order (the order inside a colour does not matter).
-//   k_lu_radius  one lane per face: r_f
-//   k_lu_sweep   one lane per cell of one colour; forward: dQ* = (-R - sum_L ...) / D;
-//                backward: dQ = dQ* - sum_U ... / D, Q_new = Q + dQ (and the admissibility check)
+//   k_lu_sweep   one lane per cell of one colour; forward: r_f of the cell's faces, D, and
+//                dQ* = (-R - sum_L ...) / D; backward: dQ = dQ* - sum_U ... / D, Q_new = Q + dQ
+//                (and the admissibility check)
 // Both sweeps are memory-bound gathers over the neighbours of a cell, like the residual's gather.
 #pragma once
 #include "device_common.hpp"
 
 namespace mstk {
 
+// Cells are addressed by their position t in the colour-sorted list (colour k: positions
+// [start_k, start_{k+1})); per (slot, position), slot-major, the sweeps read precomputed records so a
+// neighbour costs one round of dependent loads (its state and update) instead of walking
+// cell -> face -> neighbour -> colour.
 struct LuArgs {
     int64_t nc;
-    const int32_t *__restrict__ ell;     // slot-major: (face << 1) | is_c1, -1 past the last face
-    const int32_t *__restrict__ fc0;
-    const int32_t *__restrict__ fc1;     // neighbour (interior / periodic partner cell), -1 boundary
-    const double *__restrict__ fn;       // outward normal of c0 (area-weighted)
-    const int32_t *__restrict__ colour;  // per cell
-    const double *__restrict__ rf;       // per face spectral radius
-    const double *__restrict__ Q;        // state at the start of the iteration
-    const double *__restrict__ R;        // its residual
-    double *__restrict__ dQ;
+    const int32_t *__restrict__ cells;   // position -> cell
+    const int32_t *__restrict__ nb;      // (neighbour << 2) | 1 (lower colour) | 2 (higher colour);
+                                         // -1 boundary face, -2 past the last face
+    const double *__restrict__ n;        // outward normal of the cell at the slot (area-weighted)
+    double *__restrict__ r;              // spectral radius of the slot's face (per iteration)
+    double *__restrict__ D;              // diagonal (per iteration)
+    const double *__restrict__ Q;        // state at the start of the iteration (cell order)
+    const double *__restrict__ R;        // its residual (cell order)
+    double *__restrict__ dQ;             // update (cell order)
     double *__restrict__ Qout;
     int *__restrict__ bad;
     double gamma, dscale, omega;         // dscale = 1/CFL + omega/2
@@ -74,83 +78,83 @@ __device__ __forceinline__ void lu_flux(const double *q, const double *n, double
     F[NV - 1] = (q[NV - 1] + p) * un;
 }
 
-template <int DIM>
-__global__ __launch_bounds__(BLOCK) void k_lu_radius(const LuArgs a, int64_t nf, double *__restrict__ rf) {
+// One colour's positions [t0, t0 + n).
+// FWD: the spectral radius of each of the cell's faces (max over the face's two cells: the same value
+// from either side, exactly, since it depends on the normal only through |u.n| and |n|) and D are
+// computed and stored for the backward sweep, then dQ* = (-R - sum_L ...) / D.  The last colour has
+// no higher neighbours, so its backward sweep is the identity: LAST also writes Qout.
+// Backward (!FWD): dQ = dQ* - sum_U ... / D, Qout = Q + dQ.
+template <int DIM, bool FWD, bool LAST, int MAXF>
+__global__ __launch_bounds__(BLOCK) void k_lu_sweep(const LuArgs a, int64_t t0, int64_t n) {
     constexpr int NV = DIM + 2;
-    const int64_t f = block_id() * BLOCK + threadIdx.x;
-    if (f >= nf) return;
-    double n[DIM], q[NV];
-#pragma unroll
-    for (int d = 0; d < DIM; d++) n[d] = a.fn[f * DIM + d];
-    ldq<NV>(a.Q + (int64_t)a.fc0[f] * NV, q);
-    double r = lu_radius<DIM>(q, n, a.gamma);
-    const int32_t c1 = a.fc1[f];
-    if (c1 >= 0) {
-        ldq<NV>(a.Q + (int64_t)c1 * NV, q);
-        r = fmax(r, lu_radius<DIM>(q, n, a.gamma));
-    }
-    rf[f] = r;
-}
-
-// one colour's cells (cells[0..n)); FWD: forward sweep into dQ, else backward sweep, dQ and Qout
-template <int DIM, bool FWD, int MAXF>
-__global__ __launch_bounds__(BLOCK) void k_lu_sweep(const LuArgs a, const int32_t *__restrict__ cells, int64_t n,
-                                                    int col) {
-    constexpr int NV = DIM + 2;
-    const int64_t t = block_id() * BLOCK + threadIdx.x;
-    if (t >= n) return;
-    const int64_t c = cells[t];
-    double s[NV];
+    const int64_t l = block_id() * BLOCK + threadIdx.x;
+    if (l >= n) return;
+    const int64_t t = t0 + l;
+    const int64_t c = a.cells[t];
+    double s[NV], qc[NV];
     if constexpr (FWD) {
         ldq<NV>(a.R + c * NV, s);
+        ldq<NV>(a.Q + c * NV, qc);
 #pragma unroll
         for (int k = 0; k < NV; k++) s[k] = -s[k];
     } else {
 #pragma unroll
         for (int k = 0; k < NV; k++) s[k] = 0;
     }
+    constexpr int32_t SIDE = FWD ? 1 : 2;
     double rsum = 0;
 #pragma unroll
     for (int slot = 0; slot < MAXF; slot++) {
-        const int32_t e = a.ell[(int64_t)slot * a.nc + c];
-        if (e < 0) break;
-        const int64_t f = e >> 1;
-        const bool c1side = e & 1;
-        const double r = a.rf[f];
-        rsum += r;
-        const int32_t o = c1side ? a.fc0[f] : a.fc1[f];
-        if (o < 0 || o == c) continue;
-        const int32_t oc = a.colour[o];
-        if (FWD ? oc < col : oc > col) {
-            double nrm[DIM], q[NV], dq[NV], qp[NV], F1[NV], F0[NV];
+        const int64_t i = (int64_t)slot * a.nc + t;
+        const int32_t code = a.nb[i];
+        if (code == -2) break;
+        if (!FWD && (code < 0 || !(code & SIDE))) continue;
+        double nrm[DIM], q[NV];
 #pragma unroll
-            for (int d = 0; d < DIM; d++) nrm[d] = c1side ? -a.fn[f * DIM + d] : a.fn[f * DIM + d];
-            ldq<NV>(a.Q + (int64_t)o * NV, q);
-            ldq<NV>(a.dQ + (int64_t)o * NV, dq);
-#pragma unroll
-            for (int k = 0; k < NV; k++) qp[k] = q[k] + dq[k];
-            lu_flux<DIM>(qp, nrm, a.gamma, F1);
-            lu_flux<DIM>(q, nrm, a.gamma, F0);
-#pragma unroll
-            for (int k = 0; k < NV; k++) s[k] -= 0.5 * ((F1[k] - F0[k]) - a.omega * r * dq[k]);
+        for (int d = 0; d < DIM; d++) nrm[d] = a.n[i * DIM + d];
+        const int64_t o = code >> 2;
+        if (code >= 0) ldq<NV>(a.Q + o * NV, q);
+        double r;
+        if constexpr (FWD) {
+            r = lu_radius<DIM>(qc, nrm, a.gamma);
+            if (code >= 0) r = fmax(r, lu_radius<DIM>(q, nrm, a.gamma));
+            a.r[i] = r;
+            rsum += r;
+            if (code < 0 || !(code & SIDE)) continue;
+        } else {
+            r = a.r[i];
         }
+        double dq[NV], qp[NV], F1[NV], F0[NV];
+        ldq<NV>(a.dQ + o * NV, dq);
+#pragma unroll
+        for (int k = 0; k < NV; k++) qp[k] = q[k] + dq[k];
+        lu_flux<DIM>(qp, nrm, a.gamma, F1);
+        lu_flux<DIM>(q, nrm, a.gamma, F0);
+#pragma unroll
+        for (int k = 0; k < NV; k++) s[k] -= 0.5 * ((F1[k] - F0[k]) - a.omega * r * dq[k]);
     }
-    const double D = a.dscale * rsum;
-    double out[NV];
+    double D;
+    if constexpr (FWD) {
+        D = a.dscale * rsum;
+        a.D[t] = D;
+    } else {
+        D = a.D[t];
+    }
+    double dq[NV], out[NV];
     if constexpr (FWD) {
 #pragma unroll
-        for (int k = 0; k < NV; k++) out[k] = s[k] / D;
-        stq<NV>(a.dQ + c * NV, out);
-    } else {
-        double dq[NV], q[NV];
-        ldq<NV>(a.dQ + c * NV, dq);
-        ldq<NV>(a.Q + c * NV, q);
-#pragma unroll
-        for (int k = 0; k < NV; k++) {
-            dq[k] += s[k] / D;
-            out[k] = q[k] + dq[k];
-        }
+        for (int k = 0; k < NV; k++) dq[k] = s[k] / D;
         stq<NV>(a.dQ + c * NV, dq);
+    } else {
+        ldq<NV>(a.dQ + c * NV, dq);
+        ldq<NV>(a.Q + c * NV, qc);
+#pragma unroll
+        for (int k = 0; k < NV; k++) dq[k] += s[k] / D;
+        stq<NV>(a.dQ + c * NV, dq);
+    }
+    if constexpr (!FWD || LAST) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) out[k] = qc[k] + dq[k];
         stq<NV>(a.Qout + c * NV, out);
         const double p = lu_pressure<DIM>(out, a.gamma);
         if (!(out[0] > 0) || !(p > 0)) a.bad[0] = 1;

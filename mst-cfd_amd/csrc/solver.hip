@@ -94,9 +94,10 @@ struct mst_ctx {
     int *bad = nullptr;
     // LU-SGS steady driver (built on first use): cell colours, the cells of each colour in ascending
     // order (colour k: lu_cells[lu_start[k] .. lu_start[k + 1])), face spectral radii, the update
-    int32_t *lu_colour = nullptr, *lu_cells = nullptr;
+    int32_t *lu_cells = nullptr, *lu_nb = nullptr;
     std::vector<int64_t> lu_start;
-    double *lu_rf = nullptr, *lu_dQ = nullptr;
+    std::vector<int32_t> lu_colour;  // per cell (host copy)
+    double *lu_n = nullptr, *lu_r = nullptr, *lu_D = nullptr, *lu_dQ = nullptr;
     int steady = MST_STEADY_EXPLICIT;
     double lu_cfl = 1e4, lu_omega = 1.0;
     bool timing = false;
@@ -681,7 +682,7 @@ void release(mst_ctx *x) {
                     x->scal, x->part, x->fin_mid, x->bad, x->Q32[0], x->Q32[1], x->Q32[2], x->Q32[3], x->rec32,
                     x->frec0, x->frec1, x->inc_code, x->inc_w, x->inc_n, x->inc_dx, x->inc_w32, x->inc_n32,
                     x->inc_dx32, x->pstart, x->pface, x->ploc, x->hstart, x->hcell, x->hloc,
-                    x->lu_colour, x->lu_cells, x->lu_rf, x->lu_dQ};
+                    x->lu_cells, x->lu_nb, x->lu_n, x->lu_r, x->lu_D, x->lu_dQ};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (int k = 0; k < MST_K_COUNT; k++)
@@ -725,12 +726,15 @@ int check_bad(mst_ctx *x) {
 }
 
 // LU-SGS: greedy colouring of the cell graph in device order (each cell takes the smallest colour
-// none of its lower-numbered neighbours has; at most maxf + 1 colours) and each colour's cells in
-// ascending order.  Built once per context, on the first implicit iteration.
+// none of its lower-numbered neighbours has; at most maxf + 1 colours), the cells sorted by colour
+// (ascending inside a colour), and per (slot, position) the neighbour record and the cell's outward
+// normal.  Built once per context, on the first implicit iteration.
 int build_colours(mst_ctx *x) {
-    if (x->lu_colour) return MST_OK;
+    if (x->lu_cells) return MST_OK;
+    if (x->nc >= ((int64_t)1 << 29)) return mst::fail(MST_E_UNSUPPORTED, "LU-SGS records hold cell indices below 2^29");
     const auto &P = x->prep.P;
     const int64_t nc = x->nc;
+    const int dim = x->dim;
     std::vector<int32_t> col(nc, -1);
     int ncol = 0;
     for (int64_t c = 0; c < nc; c++) {
@@ -753,45 +757,68 @@ int build_colours(mst_ctx *x) {
     std::vector<int32_t> cells(nc);
     std::vector<int64_t> fill(start.begin(), start.end() - 1);
     for (int64_t c = 0; c < nc; c++) cells[fill[col[c]]++] = (int32_t)c;
-    int rc = dupload(&x->lu_colour, col);
-    if (rc == MST_OK) rc = dupload(&x->lu_cells, cells);
-    if (rc == MST_OK) rc = dalloc(&x->lu_rf, (size_t)std::max<int64_t>(x->nf, 1));
+    const int64_t slots = x->slots;
+    std::vector<int32_t> nb(slots * nc, -2);
+    std::vector<double> nrm(slots * nc * dim, 0.0);
+    for (int64_t t = 0; t < nc; t++) {
+        const int64_t c = cells[t];
+        for (int s = 0; s < P.maxf; s++) {
+            const int32_t e = P.ell[(int64_t)s * nc + c];
+            if (e < 0) break;
+            const int32_t f = e >> 1;
+            const bool c1side = e & 1;
+            const int32_t o = c1side ? P.fc0[f] : P.fc1[f];
+            const int64_t i = (int64_t)s * nc + t;
+            for (int d = 0; d < dim; d++) nrm[i * dim + d] = c1side ? -P.fn[(int64_t)f * dim + d] : P.fn[(int64_t)f * dim + d];
+            nb[i] = o < 0 ? -1 : ((o << 2) | (col[o] < col[c] ? 1 : 0) | (col[o] > col[c] ? 2 : 0));
+        }
+    }
+    int rc = dupload(&x->lu_cells, cells);
+    if (rc == MST_OK) rc = dupload(&x->lu_nb, nb);
+    if (rc == MST_OK) rc = dupload(&x->lu_n, nrm);
+    if (rc == MST_OK) rc = dalloc(&x->lu_r, (size_t)(slots * nc));
+    if (rc == MST_OK) rc = dalloc(&x->lu_D, (size_t)nc);
     if (rc == MST_OK) rc = dalloc(&x->lu_dQ, (size_t)(nc * x->nv));
     if (rc == MST_OK && !x->R) rc = dalloc(&x->R, (size_t)(nc * x->nv));
     if (rc) return rc;
     x->lu_start = start;
+    x->lu_colour = std::move(col);
     return MST_OK;
 }
 
 template <int DIM, int MAXF>
 int lu_sweeps(mst_ctx *x, const LuArgs &a) {
-    int rc = launch(x, MST_K_LUSGS, k_lu_radius<DIM>, x->nf, a, x->nf, x->lu_rf);
     const int ncol = (int)x->lu_start.size() - 1;
-    for (int k = 0; k < ncol && rc == MST_OK; k++)
-        rc = launch(x, MST_K_LUSGS, k_lu_sweep<DIM, true, MAXF>, x->lu_start[k + 1] - x->lu_start[k], a,
-                    (const int32_t *)(x->lu_cells + x->lu_start[k]), x->lu_start[k + 1] - x->lu_start[k], k);
-    for (int k = ncol - 1; k >= 0 && rc == MST_OK; k--)
-        rc = launch(x, MST_K_LUSGS, k_lu_sweep<DIM, false, MAXF>, x->lu_start[k + 1] - x->lu_start[k], a,
-                    (const int32_t *)(x->lu_cells + x->lu_start[k]), x->lu_start[k + 1] - x->lu_start[k], k);
+    int rc = MST_OK;
+    for (int k = 0; k < ncol && rc == MST_OK; k++) {
+        const int64_t t0 = x->lu_start[k], n = x->lu_start[k + 1] - t0;
+        rc = k == ncol - 1 ? launch(x, MST_K_LUSGS, k_lu_sweep<DIM, true, true, MAXF>, n, a, t0, n)
+                           : launch(x, MST_K_LUSGS, k_lu_sweep<DIM, true, false, MAXF>, n, a, t0, n);
+    }
+    for (int k = ncol - 2; k >= 0 && rc == MST_OK; k--) {
+        const int64_t t0 = x->lu_start[k], n = x->lu_start[k + 1] - t0;
+        rc = launch(x, MST_K_LUSGS, k_lu_sweep<DIM, false, false, MAXF>, n, a, t0, n);
+    }
     return rc;
 }
 
-// one LU-SGS iteration: residual of Old (+ its L2 norms), spectral radii, the two sweeps into New
-int lu_iteration(mst_ctx *x) {
+// one LU-SGS iteration: residual of Old (+ its L2 norms when l2 is set), the two sweeps into New
+int lu_iteration(mst_ctx *x, bool l2) {
     int rc = run_face_flux(x, x->iold);
     if (rc) return rc;
-    rc = run_update(x, UPD_L2, x->iold, -1, 0, 0, 1, -1);
+    rc = run_update(x, l2 ? UPD_L2 : UPD_RESIDUAL, x->iold, -1, 0, 0, 1, -1);
     if (rc) return rc;
-    rc = run_finish(x, x->nv, 0);
-    if (rc) return rc;
+    if (l2) {
+        rc = run_finish(x, x->nv, 0);
+        if (rc) return rc;
+    }
     LuArgs a;
     a.nc = x->nc;
-    a.ell = x->ell;
-    a.fc0 = x->fc0;
-    a.fc1 = x->fc1;
-    a.fn = x->fn;
-    a.colour = x->lu_colour;
-    a.rf = x->lu_rf;
+    a.cells = x->lu_cells;
+    a.nb = x->lu_nb;
+    a.n = x->lu_n;
+    a.r = x->lu_r;
+    a.D = x->lu_D;
     a.Q = x->Q[x->iold];
     a.R = x->R;
     a.dQ = x->lu_dQ;
@@ -800,7 +827,7 @@ int lu_iteration(mst_ctx *x) {
     a.gamma = x->cfg.gamma;
     a.omega = x->lu_omega;
     a.dscale = 1.0 / x->lu_cfl + 0.5 * x->lu_omega;
-    // the sweeps unroll over the padded slot count of the device table (3, 4 or 8)
+    // the records are padded to the slot count of the device table (3, 4 or 8)
     if (x->dim == 2) {
         if (x->slots == 3) return lu_sweeps<2, 3>(x, a);
         if (x->slots == 4) return lu_sweeps<2, 4>(x, a);
@@ -1065,21 +1092,25 @@ int mst_step(mst_ctx *ctx, int nsteps) {
             if (rc) return rc;
         }
         // stage buffers: the two spare buffers alternate, the last stage lands in "new"; the first
-        // stage reduces |R/V|, the last one the CFL time step of the new state (next step's dt)
+        // stage of the call's last step reduces |R/V| (the norms are a monitor: one reduction per
+        // call), the last stage of every step the CFL time step of the new state (next step's dt)
         int spare[2], w = 0;
         for (int i = 0; i < 4; i++)
             if (i != ctx->iold && i != ctx->inew) spare[w++] = i;
+        const bool l2 = it == nsteps - 1;
         int src = ctx->iold;
         for (int s = 0; s < ns; s++) {
             const int dst = (s == ns - 1) ? ctx->inew : spare[s & 1];
-            const int mode = UPD_STAGE | (s == 0 ? UPD_L2 : 0) | ((cfl && s == ns - 1) ? UPD_DT : 0) |
+            const int mode = UPD_STAGE | ((l2 && s == 0) ? UPD_L2 : 0) | ((cfl && s == ns - 1) ? UPD_DT : 0) |
                              (s == ns - 1 ? UPD_CHECK : 0);
             rc = run_stage(ctx, mode, src, ctx->iold, A[s], B[s], s == 0 ? 1 : 0, dst);
             if (rc) return rc;
             src = dst;
         }
-        rc = run_finish(ctx, ctx->nv, cfl ? 1 : 0);
-        if (rc) return rc;
+        if (l2 || cfl) {
+            rc = run_finish(ctx, l2 ? ctx->nv : 0, cfl ? 1 : 0);
+            if (rc) return rc;
+        }
         ctx->dt_valid = cfl;
         std::swap(ctx->iold, ctx->inew);
     }
@@ -1140,7 +1171,7 @@ int mst_lusgs_step(mst_ctx *ctx, int n_iter) {
     int rc = build_colours(ctx);
     if (rc) return rc;
     for (int it = 0; it < n_iter; it++) {
-        rc = lu_iteration(ctx);
+        rc = lu_iteration(ctx, it == n_iter - 1);  // norms of the call's last iteration (a monitor)
         if (rc) return rc;
         ctx->shadow_ok[ctx->inew] = false;
         std::swap(ctx->iold, ctx->inew);
@@ -1154,10 +1185,8 @@ int mst_lusgs_colours(mst_ctx *ctx, int32_t *colour, int64_t n_cells, int *n_col
     if (!colour || !n_colours || n_cells != ctx->nc) return mst::fail(MST_E_ARG, "size does not match the mesh");
     int rc = build_colours(ctx);
     if (rc) return rc;
-    std::vector<int32_t> buf(ctx->nc);
-    HIPCHK(hipMemcpy(buf.data(), ctx->lu_colour, sizeof(int32_t) * buf.size(), hipMemcpyDeviceToHost));
     const auto &new_of_old = ctx->prep.P.cell_new_of_old;
-    for (int64_t c = 0; c < ctx->nc; c++) colour[c] = buf[new_of_old[c]];
+    for (int64_t c = 0; c < ctx->nc; c++) colour[c] = ctx->lu_colour[new_of_old[c]];
     *n_colours = (int)ctx->lu_start.size() - 1;
     return MST_OK;
 }
@@ -1459,6 +1488,17 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
         case MST_K_DT:
             *bytes = nc * (8 * NV + 8 + 8 * D);
             break;
+        case MST_K_LUSGS: {  // one iteration's sweeps, each array once per sweep, over their launches
+            if (ctx->lu_start.size() < 2) return mst::fail(MST_E_ARG, "no LU-SGS iteration has run");
+            const double S = ctx->slots, launches = 2.0 * (double)(ctx->lu_start.size() - 1) - 1;
+            // forward: positions, residual, states, neighbour updates, update out, D out; per slot the
+            // record (neighbour code, normal) and r out.  backward: positions, D, states, updates in
+            // and out, new state; per slot the record and r.
+            const double fwd = nc * (4 + 8 * NV * 4 + 8) + S * nc * (4 + 8 * D + 8);
+            const double bwd = nc * (4 + 8 + 8 * NV * 4) + S * nc * (4 + 8 * D + 8);
+            *bytes = (fwd + bwd) / launches;
+            break;
+        }
         default:
             return mst::fail(MST_E_ARG, "bad kernel id");
     }

@@ -15,8 +15,8 @@ import time
 
 import numpy as np
 
-from . import (BC_INLET, BC_OUTLET, BC_WALL, INIT_FREESTREAM, LIM_NONE, MLP, ROE, Batch, MstError, Mesh, Solver,
-               make_config)
+from . import (BC_INLET, BC_OUTLET, BC_WALL, INIT_FREESTREAM, LIM_NONE, MLP, ROE, STEADY_EXPLICIT, STEADY_LUSGS,
+               Batch, MstError, Mesh, Solver, make_config)
 
 GAMMA = 1.4
 X0, X1, HEIGHT = -0.5, 1.5, 1.2
@@ -138,7 +138,11 @@ def time_to_steady(s, m, mach, theta, net=None, ref=None, tol=1e-8, max_steps=60
     return dict(steps=int(steps) + lost, seconds=wall, ratio=float(ratio), ref=r, start=used, converged=ratio <= tol)
 
 
-def evaluate(net, cases, nx, ny, tol=1e-8, max_steps=60000, check_every=200):
+def evaluate(net, cases, nx, ny, tol=1e-8, max_steps=60000, check_every=200, lusgs=None, lu_max=20000,
+             lu_check=50):
+    """Per held-out case: the explicit driver from the freestream (cold) and from the prediction (warm);
+    with lusgs = (cfl, omega) also the LU-SGS driver from both starts.  Every run goes to the same
+    absolute level: tol x the explicit cold run's first-step norms."""
     rows = []
     for (M, th) in cases:
         m = mesh(th, nx, ny)
@@ -146,12 +150,24 @@ def evaluate(net, cases, nx, ny, tol=1e-8, max_steps=60000, check_every=200):
         cold = time_to_steady(s, m, M, th, None, None, tol, max_steps, check_every)
         e_fs, e_net = initial_error(s, m, M, th, net, s.get_state())
         warm = time_to_steady(s, m, M, th, net, cold["ref"], tol, max_steps, check_every)
+        row = dict(mach=M, theta=th, cold_steps=cold["steps"], cold_s=cold["seconds"], cold_ratio=cold["ratio"],
+                   warm_steps=warm["steps"], warm_s=warm["seconds"], warm_ratio=warm["ratio"],
+                   warm_start=warm["start"], converged=bool(cold["converged"] and warm["converged"]),
+                   speedup=cold["seconds"] / warm["seconds"],
+                   freestream_error=e_fs.tolist(), network_error=e_net.tolist())
+        if lusgs is not None:
+            s.set_steady_driver(STEADY_LUSGS, *lusgs)
+            s.lusgs_colours()  # the colouring is built once per mesh, outside the timed runs
+            lc = time_to_steady(s, m, M, th, None, cold["ref"], tol, lu_max, lu_check)
+            lw = time_to_steady(s, m, M, th, net, cold["ref"], tol, lu_max, lu_check)
+            s.set_steady_driver(STEADY_EXPLICIT)
+            row.update(lu_cold_steps=lc["steps"], lu_cold_s=lc["seconds"], lu_cold_ratio=lc["ratio"],
+                       lu_warm_steps=lw["steps"], lu_warm_s=lw["seconds"], lu_warm_ratio=lw["ratio"],
+                       lu_warm_start=lw["start"], lu_converged=bool(lc["converged"] and lw["converged"]),
+                       lu_speedup=lc["seconds"] / lw["seconds"], driver_speedup=cold["seconds"] / lc["seconds"],
+                       combined_speedup=cold["seconds"] / lw["seconds"])
         s.close()
-        rows.append(dict(mach=M, theta=th, cold_steps=cold["steps"], cold_s=cold["seconds"], cold_ratio=cold["ratio"],
-                         warm_steps=warm["steps"], warm_s=warm["seconds"], warm_ratio=warm["ratio"],
-                         warm_start=warm["start"], converged=bool(cold["converged"] and warm["converged"]),
-                         speedup=cold["seconds"] / warm["seconds"],
-                         freestream_error=e_fs.tolist(), network_error=e_net.tolist()))
+        rows.append(row)
     return rows
 
 
@@ -160,7 +176,18 @@ def summary(rows):
     sp = np.array([r["speedup"] for r in ok]) if ok else np.array([np.nan])
     st = np.array([r["cold_steps"] / r["warm_steps"] for r in ok]) if ok else np.array([np.nan])
     gain = np.array([np.mean(r["freestream_error"]) / np.mean(r["network_error"]) for r in rows])
-    return dict(cases=len(rows), converged=len(ok), geomean_speedup=float(np.exp(np.log(sp).mean())),
-                geomean_initial_error_reduction=float(np.exp(np.log(gain).mean())),
-                median_speedup=float(np.median(sp)), geomean_step_ratio=float(np.exp(np.log(st).mean())),
-                total_cold_s=float(sum(r["cold_s"] for r in ok)), total_warm_s=float(sum(r["warm_s"] for r in ok)))
+    out = dict(cases=len(rows), converged=len(ok), geomean_speedup=float(np.exp(np.log(sp).mean())),
+               geomean_initial_error_reduction=float(np.exp(np.log(gain).mean())),
+               median_speedup=float(np.median(sp)), geomean_step_ratio=float(np.exp(np.log(st).mean())),
+               total_cold_s=float(sum(r["cold_s"] for r in ok)), total_warm_s=float(sum(r["warm_s"] for r in ok)))
+    lu = [r for r in rows if r.get("lu_converged") and r["converged"]]
+    if lu:
+        gm = lambda k: float(np.exp(np.mean([np.log(r[k]) for r in lu])))
+        out.update(lusgs_converged=len(lu), lusgs_geomean_warm_speedup=gm("lu_speedup"),
+                   lusgs_geomean_driver_speedup=gm("driver_speedup"),
+                   lusgs_geomean_combined_speedup=gm("combined_speedup"),
+                   lusgs_geomean_iteration_ratio=float(np.exp(np.mean([np.log(r["lu_cold_steps"] / r["lu_warm_steps"])
+                                                                        for r in lu]))),
+                   lusgs_total_cold_s=float(sum(r["lu_cold_s"] for r in lu)),
+                   lusgs_total_warm_s=float(sum(r["lu_warm_s"] for r in lu)))
+    return out
