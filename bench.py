@@ -234,6 +234,11 @@ def bench_steady(M, args):
     iteration counts, the per-kernel times of LU-SGS iterations and the sweeps' roofline."""
     from mstcfd import ramp as RP
     mach, theta, tol = 2.51, 9.3, 1e-8
+    copy = None
+    if args.calibrate_copy:
+        M.copy_probe(args.calibrate_copy)  # first launch pays the code-object load
+        ms = min(M.copy_probe(args.calibrate_copy) for _ in range(3))
+        copy = dict(bytes=args.calibrate_copy, ms=ms, gbs=2 * args.calibrate_copy / (ms * 1e-3) / 1e9)
     m = RP.mesh(theta, 600, 250)
     s = M.Solver(m, RP.config(m, mach))
     q0 = RP.freestream(m, mach)
@@ -246,8 +251,8 @@ def bench_steady(M, args):
         s.set_state(q0)
         s.sync()
         t0 = time.perf_counter()
-        steps, drop, r = s.run_to_steady(tol, 60000 if drv == M.STEADY_EXPLICIT else 20000,
-                                         200 if drv == M.STEADY_EXPLICIT else 50, ref)
+        cap = args.max_steps or (60000 if drv == M.STEADY_EXPLICIT else 20000)
+        steps, drop, r = s.run_to_steady(tol, cap, min(cap, 200 if drv == M.STEADY_EXPLICIT else 50), ref)
         s.sync()
         runs[name] = dict(seconds=time.perf_counter() - t0, iterations=int(steps), drop=float(drop),
                           converged=bool(drop <= tol))
@@ -284,6 +289,10 @@ def bench_steady(M, args):
                roofline=dict(bound="hbm", kernel="k_lu_sweep", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
                              frac=ach / HBM_PEAK_GBS, traffic=None, alg_bytes=sw["alg_bytes"]),
                kernels=kern)
+    if copy:
+        out["copy_probe"] = copy
+    if args.max_steps:
+        out["note"] = f"to-steady runs capped at {args.max_steps} iterations (profiler pass): value is not a time to 1e-8"
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = lusgs_cpu_baseline(m, mach, s.lusgs_colours()[0], lu["iterations"])
     s.close()
@@ -329,6 +338,9 @@ def main():
     ap.add_argument("--order", choices=("hilbert", "rcm"), default="hilbert",
                     help="cell order of the device layout (Hilbert-curve patches or reverse Cuthill-McKee)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison")
+    ap.add_argument("--max-steps", type=int, default=0,
+                    help="steady workload: cap the to-steady runs at this many steps/iterations (profiler passes; "
+                         "the line then carries a note and no time to 1e-8); 0: no cap")
     ap.add_argument("--compare-steps", type=int, default=200,
                     help="steps of the FP64 vs mixed state comparison")
     ap.add_argument("--calibrate-copy", type=int, default=268435456, metavar="BYTES",

@@ -154,6 +154,10 @@ int mst_prep_geometry(const mst_prep *p, double *fcen, double *S, double *eta, i
 /* device-order integer tables: fc0, fc1 (n_faces), kind (n_faces), slot-major cell->face table
  * (maxf*n_cells, (face<<1)|is_c1, -1 padding); any pointer may be NULL */
 int mst_prep_tables(const mst_prep *p, int32_t *fc0, int32_t *fc1, int32_t *kind, int32_t *cell_faces);
+/* NEW: the cell colouring of the LU-SGS sweeps (no two cells sharing a face alike; two colours when
+ * the cell graph is bipartite, else greedy in device order + Kempe-chain reduction of the top colour),
+ * file order (colour[n_cells]), and the number of colours */
+int mst_prep_cell_colours(const mst_prep *p, int32_t *colour, int *n_colours);
 
 /* ---------------------------------------------------------------- solver context
  * RhoSolver::RhoSolver (rhoSolver/RhoSolver.cpp:3-26) + AllData::createAllData (data/AllData.cpp:3-28):
@@ -231,7 +235,7 @@ int mst_set_steady_driver(mst_ctx *ctx, int driver, double cfl, double omega);
 /* n_iter LU-SGS iterations (Old -> New, then swap, like mst_step); cfl/omega from
  * mst_set_steady_driver (defaults 1e4 and 1.0) */
 int mst_lusgs_step(mst_ctx *ctx, int n_iter);
-/* the cell colouring the sweeps use, file order (colour[n_cells]), and the number of colours */
+/* the cell colouring the sweeps use (as mst_prep_cell_colours), file order, and the number of colours */
 int mst_lusgs_colours(mst_ctx *ctx, int32_t *colour, int64_t n_cells, int *n_colours);
 /* NEW: dt = cfl * min_c V_c / sum_d (|u_d| + a) Shalf_c,d on the Old state */
 int mst_compute_dt(mst_ctx *ctx, double cfl, double *dt);

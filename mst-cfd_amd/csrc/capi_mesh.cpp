@@ -259,4 +259,13 @@ int mst_prep_tables(const mst_prep *p, int32_t *fc0, int32_t *fc1, int32_t *kind
     return MST_OK;
 }
 
+int mst_prep_cell_colours(const mst_prep *p, int32_t *colour, int *n_colours) {
+    if (!p || !colour || !n_colours) return mst::fail(MST_E_ARG, "null argument");
+    const mst::Prepared &P = p->P;
+    std::vector<int32_t> col;
+    *n_colours = mst::lusgs_colours(P, col);
+    for (int64_t c = 0; c < P.nc; c++) colour[c] = col[P.cell_new_of_old[c]];
+    return MST_OK;
+}
+
 }  // extern "C"

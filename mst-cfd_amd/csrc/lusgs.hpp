@@ -101,57 +101,102 @@ __global__ __launch_bounds__(BLOCK) void k_lu_sweep(const LuArgs a, int64_t t0, 
 #pragma unroll
         for (int k = 0; k < NV; k++) s[k] = 0;
     }
+    double dqc[NV], D = 0;  // backward: the cell's own update, state and diagonal, read up front
+    if constexpr (!FWD) {
+        ldq<NV>(a.dQ + c * NV, dqc);
+        ldq<NV>(a.Q + c * NV, qc);
+        D = a.D[t];
+    }
     constexpr int32_t SIDE = FWD ? 1 : 2;
     double rsum = 0;
+    if constexpr (MAXF <= 4) {
+        // every memory read of the cell's slots is issued before the first is used: two rounds of
+        // dependent loads per cell (records, then neighbour states and updates) instead of two per
+        // slot.  The arithmetic below runs in the same order as the slot loop further down.
+        int32_t code[MAXF];
+        double nrm[MAXF][DIM], q[MAXF][NV], dq[MAXF][NV], rr[MAXF];
 #pragma unroll
-    for (int slot = 0; slot < MAXF; slot++) {
-        const int64_t i = (int64_t)slot * a.nc + t;
-        const int32_t code = a.nb[i];
-        if (code == -2) break;
-        if (!FWD && (code < 0 || !(code & SIDE))) continue;
-        double nrm[DIM], q[NV];
+        for (int k = 0; k < MAXF; k++) code[k] = a.nb[(int64_t)k * a.nc + t];
 #pragma unroll
-        for (int d = 0; d < DIM; d++) nrm[d] = a.n[i * DIM + d];
-        const int64_t o = code >> 2;
-        if (code >= 0) ldq<NV>(a.Q + o * NV, q);
-        double r;
-        if constexpr (FWD) {
-            r = lu_radius<DIM>(qc, nrm, a.gamma);
-            if (code >= 0) r = fmax(r, lu_radius<DIM>(q, nrm, a.gamma));
-            a.r[i] = r;
-            rsum += r;
-            if (code < 0 || !(code & SIDE)) continue;
-        } else {
-            r = a.r[i];
+        for (int k = 0; k < MAXF; k++) {
+            const int64_t i = (int64_t)k * a.nc + t;
+            const bool nbr = code[k] >= 0, use = nbr && (code[k] & SIDE);
+            const int64_t o = code[k] >> 2;
+            if (FWD ? code[k] != -2 : use) {
+#pragma unroll
+                for (int d = 0; d < DIM; d++) nrm[k][d] = a.n[i * DIM + d];
+            }
+            if (FWD ? nbr : use) ldq<NV>(a.Q + o * NV, q[k]);
+            if (use) ldq<NV>(a.dQ + o * NV, dq[k]);
+            if (!FWD && use) rr[k] = a.r[i];
         }
-        double dq[NV], qp[NV], F1[NV], F0[NV];
-        ldq<NV>(a.dQ + o * NV, dq);
 #pragma unroll
-        for (int k = 0; k < NV; k++) qp[k] = q[k] + dq[k];
-        lu_flux<DIM>(qp, nrm, a.gamma, F1);
-        lu_flux<DIM>(q, nrm, a.gamma, F0);
+        for (int k = 0; k < MAXF; k++) {
+            if (code[k] == -2) break;  // the slots after it are padding too
+            const bool nbr = code[k] >= 0, use = nbr && (code[k] & SIDE);
+            double r;
+            if constexpr (FWD) {
+                r = lu_radius<DIM>(qc, nrm[k], a.gamma);
+                if (nbr) r = fmax(r, lu_radius<DIM>(q[k], nrm[k], a.gamma));
+                a.r[(int64_t)k * a.nc + t] = r;
+                rsum += r;
+            } else {
+                r = rr[k];
+            }
+            if (!use) continue;
+            double qp[NV], F1[NV], F0[NV];
 #pragma unroll
-        for (int k = 0; k < NV; k++) s[k] -= 0.5 * ((F1[k] - F0[k]) - a.omega * r * dq[k]);
+            for (int j = 0; j < NV; j++) qp[j] = q[k][j] + dq[k][j];
+            lu_flux<DIM>(qp, nrm[k], a.gamma, F1);
+            lu_flux<DIM>(q[k], nrm[k], a.gamma, F0);
+#pragma unroll
+            for (int j = 0; j < NV; j++) s[j] -= 0.5 * ((F1[j] - F0[j]) - a.omega * r * dq[k][j]);
+        }
+    } else {
+#pragma unroll
+        for (int slot = 0; slot < MAXF; slot++) {
+            const int64_t i = (int64_t)slot * a.nc + t;
+            const int32_t code = a.nb[i];
+            if (code == -2) break;
+            if (!FWD && (code < 0 || !(code & SIDE))) continue;
+            double nrm[DIM], q[NV];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) nrm[d] = a.n[i * DIM + d];
+            const int64_t o = code >> 2;
+            if (code >= 0) ldq<NV>(a.Q + o * NV, q);
+            double r;
+            if constexpr (FWD) {
+                r = lu_radius<DIM>(qc, nrm, a.gamma);
+                if (code >= 0) r = fmax(r, lu_radius<DIM>(q, nrm, a.gamma));
+                a.r[i] = r;
+                rsum += r;
+                if (code < 0 || !(code & SIDE)) continue;
+            } else {
+                r = a.r[i];
+            }
+            double dq[NV], qp[NV], F1[NV], F0[NV];
+            ldq<NV>(a.dQ + o * NV, dq);
+#pragma unroll
+            for (int k = 0; k < NV; k++) qp[k] = q[k] + dq[k];
+            lu_flux<DIM>(qp, nrm, a.gamma, F1);
+            lu_flux<DIM>(q, nrm, a.gamma, F0);
+#pragma unroll
+            for (int k = 0; k < NV; k++) s[k] -= 0.5 * ((F1[k] - F0[k]) - a.omega * r * dq[k]);
+        }
     }
-    double D;
     if constexpr (FWD) {
         D = a.dscale * rsum;
         a.D[t] = D;
-    } else {
-        D = a.D[t];
     }
     double dq[NV], out[NV];
     if constexpr (FWD) {
 #pragma unroll
         for (int k = 0; k < NV; k++) dq[k] = s[k] / D;
-        stq<NV>(a.dQ + c * NV, dq);
     } else {
-        ldq<NV>(a.dQ + c * NV, dq);
-        ldq<NV>(a.Q + c * NV, qc);
 #pragma unroll
-        for (int k = 0; k < NV; k++) dq[k] += s[k] / D;
-        stq<NV>(a.dQ + c * NV, dq);
+        for (int k = 0; k < NV; k++) dq[k] = dqc[k] + s[k] / D;
     }
+    stq<NV>(a.dQ + c * NV, dq);
     if constexpr (!FWD || LAST) {
 #pragma unroll
         for (int k = 0; k < NV; k++) out[k] = qc[k] + dq[k];

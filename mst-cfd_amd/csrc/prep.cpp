@@ -564,4 +564,136 @@ void lr_masks(const Prepared &P, int compat, std::vector<int32_t> &mask) {
     }
 }
 
+// LU-SGS colouring of the cell graph (cells sharing a face get different colours).  Fewer colours
+// mean fewer sweep launches per iteration (2 ncol - 1), each of which costs a fixed few microseconds:
+//  1. two colours when the graph is bipartite (breadth-first search from each uncoloured cell in
+//     device order; the split-quad meshes are);
+//  2. otherwise greedy in device order (each cell takes the smallest colour none of its
+//     lower-numbered neighbours has; at most maxf + 1 colours), then Kempe-chain swaps that empty
+//     the top colour where they can: a top-colour cell whose neighbours use colours a and b swaps a
+//     and b along the {a, b}-chain through its a-neighbour; if that chain misses its b-neighbours,
+//     colour a becomes free for it.  Chains are searched up to a fixed size.
+// Deterministic.  Returns the number of colours (every colour 0..n-1 is used).
+static void cell_neighbours(const Prepared &P, int64_t c, int32_t (&nb)[8], int &n) {
+    n = 0;
+    for (int s = 0; s < P.maxf; s++) {
+        const int32_t e = P.ell[(int64_t)s * P.nc + c];
+        if (e < 0) break;
+        const int32_t f = e >> 1;
+        const int32_t o = (e & 1) ? P.fc0[f] : P.fc1[f];
+        if (o >= 0 && o != c) nb[n++] = o;
+    }
+}
+
+static bool two_colours(const Prepared &P, std::vector<int32_t> &col) {
+    const int64_t nc = P.nc;
+    col.assign(nc, -1);
+    std::vector<int32_t> queue;
+    queue.reserve(nc);
+    for (int64_t s0 = 0; s0 < nc; s0++) {
+        if (col[s0] >= 0) continue;
+        col[s0] = 0;
+        queue.clear();
+        queue.push_back((int32_t)s0);
+        for (size_t h = 0; h < queue.size(); h++) {
+            const int32_t c = queue[h];
+            int32_t nb[8];
+            int n;
+            cell_neighbours(P, c, nb, n);
+            for (int j = 0; j < n; j++) {
+                if (col[nb[j]] < 0) {
+                    col[nb[j]] = 1 - col[c];
+                    queue.push_back(nb[j]);
+                } else if (col[nb[j]] == col[c]) {
+                    return false;
+                }
+            }
+        }
+    }
+    return true;
+}
+
+static void greedy_kempe(const Prepared &P, std::vector<int32_t> &col) {
+    const int64_t nc = P.nc;
+    col.assign(nc, -1);
+    int ncol = 0;
+    for (int64_t c = 0; c < nc; c++) {
+        int32_t nb[8];
+        int n;
+        cell_neighbours(P, c, nb, n);
+        uint32_t used = 0;
+        for (int j = 0; j < n; j++)
+            if (col[nb[j]] >= 0) used |= 1u << col[nb[j]];
+        int k = 0;
+        while ((used >> k) & 1u) k++;
+        col[c] = k;
+        ncol = std::max(ncol, k + 1);
+    }
+    constexpr size_t CHAIN_MAX = 1 << 14;
+    std::vector<int32_t> stamp(nc, -1), chain;
+    int32_t pass = 0;
+    while (ncol > 2) {
+        const int top = ncol - 1;
+        bool emptied = true;
+        for (int64_t v = 0; v < nc; v++) {
+            if (col[v] != top) continue;
+            int32_t nb[8];
+            int n;
+            cell_neighbours(P, v, nb, n);
+            bool fixed = false;
+            for (int a = 0; a < top && !fixed; a++) {
+                uint32_t used = 0;
+                for (int j = 0; j < n; j++) used |= 1u << col[nb[j]];
+                if (!((used >> a) & 1u)) {  // already free
+                    col[v] = a;
+                    fixed = true;
+                    break;
+                }
+                for (int b = 0; b < top && !fixed; b++) {
+                    if (b == a) continue;
+                    int ua = -1;
+                    for (int j = 0; j < n; j++)
+                        if (col[nb[j]] == a) ua = nb[j];
+                    // chain of colours {a, b} through one a-neighbour (every a-neighbour must be in it)
+                    ++pass;
+                    chain.clear();
+                    chain.push_back(ua);
+                    stamp[ua] = pass;
+                    bool ok = true;
+                    for (size_t h = 0; h < chain.size() && ok; h++) {
+                        int32_t nb2[8];
+                        int n2;
+                        cell_neighbours(P, chain[h], nb2, n2);
+                        for (int j = 0; j < n2; j++) {
+                            const int32_t y = nb2[j];
+                            if (stamp[y] != pass && (col[y] == a || col[y] == b)) {
+                                stamp[y] = pass;
+                                chain.push_back(y);
+                            }
+                        }
+                        if (chain.size() > CHAIN_MAX) ok = false;
+                    }
+                    if (!ok) continue;
+                    bool blocked = false;
+                    for (int j = 0; j < n; j++)
+                        if (col[nb[j]] == a && stamp[nb[j]] != pass) blocked = true;  // an a-neighbour outside
+                        else if (col[nb[j]] == b && stamp[nb[j]] == pass) blocked = true;  // a b-neighbour inside
+                    if (blocked) continue;
+                    for (int32_t x : chain) col[x] = col[x] == a ? b : a;
+                    col[v] = a;
+                    fixed = true;
+                }
+            }
+            if (!fixed) emptied = false;
+        }
+        if (!emptied) break;
+        ncol--;
+    }
+}
+
+int lusgs_colours(const Prepared &P, std::vector<int32_t> &col) {
+    if (!two_colours(P, col)) greedy_kempe(P, col);
+    return P.nc > 0 ? *std::max_element(col.begin(), col.end()) + 1 : 0;
+}
+
 }  // namespace mst

@@ -65,6 +65,11 @@ void lr_masks(const Prepared &P, int compat, std::vector<int32_t> &mask);
 // partner face's centre on the c1 side of a periodic pair).  Returns the face (or -1).
 int64_t incidence(const Prepared &P, int s, int64_t c, int32_t &code, double *n, double *dx);
 
+// colouring of the cell graph for the LU-SGS sweeps (new cell order): no two cells sharing a face
+// alike; two colours when the graph is bipartite, else greedy + Kempe-chain reduction.  Returns the
+// number of colours.
+int lusgs_colours(const Prepared &P, std::vector<int32_t> &col);
+
 }  // namespace mst
 
 // the opaque handle of the C ABI

@@ -725,32 +725,17 @@ int check_bad(mst_ctx *x) {
     return MST_OK;
 }
 
-// LU-SGS: greedy colouring of the cell graph in device order (each cell takes the smallest colour
-// none of its lower-numbered neighbours has; at most maxf + 1 colours), the cells sorted by colour
-// (ascending inside a colour), and per (slot, position) the neighbour record and the cell's outward
-// normal.  Built once per context, on the first implicit iteration.
+// LU-SGS: the cell colouring (prep.cpp, lusgs_colours), the cells sorted by colour (ascending inside
+// a colour), and per (slot, position) the neighbour record and the cell's outward normal.  Built once
+// per context, on the first implicit iteration.
 int build_colours(mst_ctx *x) {
     if (x->lu_cells) return MST_OK;
     if (x->nc >= ((int64_t)1 << 29)) return mst::fail(MST_E_UNSUPPORTED, "LU-SGS records hold cell indices below 2^29");
     const auto &P = x->prep.P;
     const int64_t nc = x->nc;
     const int dim = x->dim;
-    std::vector<int32_t> col(nc, -1);
-    int ncol = 0;
-    for (int64_t c = 0; c < nc; c++) {
-        uint32_t used = 0;
-        for (int s = 0; s < P.maxf; s++) {
-            const int32_t e = P.ell[(int64_t)s * nc + c];
-            if (e < 0) break;
-            const int32_t f = e >> 1;
-            const int32_t o = (e & 1) ? P.fc0[f] : P.fc1[f];
-            if (o >= 0 && o != c && col[o] >= 0) used |= 1u << col[o];
-        }
-        int k = 0;
-        while ((used >> k) & 1u) k++;
-        col[c] = k;
-        ncol = std::max(ncol, k + 1);
-    }
+    std::vector<int32_t> col;
+    const int ncol = lusgs_colours(P, col);
     std::vector<int64_t> start(ncol + 1, 0);
     for (int64_t c = 0; c < nc; c++) start[col[c] + 1]++;
     for (int k = 0; k < ncol; k++) start[k + 1] += start[k];

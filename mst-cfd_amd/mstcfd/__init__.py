@@ -51,7 +51,7 @@ EXPORTS = [
     "mst_mlp_set_scaling", "mst_mlp_set_data", "mst_mlp_reset_optimizer", "mst_mlp_train", "mst_mlp_forward",
     "mst_mlp_gradient", "mst_ctx_warm_start", "mst_run_to_steady", "mst_batch_run_to_steady",
     "mst_mesh_cell_centres", "mst_batch_set_strategy", "mst_batch_get_strategy", "mst_set_cell_order",
-    "mst_get_cell_order", "mst_set_steady_driver", "mst_lusgs_step", "mst_lusgs_colours",
+    "mst_get_cell_order", "mst_set_steady_driver", "mst_lusgs_step", "mst_lusgs_colours", "mst_prep_cell_colours",
 ]
 
 
@@ -194,6 +194,7 @@ def lib():
         "mst_set_steady_driver": (C.c_int, [vp, C.c_int, C.c_double, C.c_double]),
         "mst_lusgs_step": (C.c_int, [vp, C.c_int]),
         "mst_lusgs_colours": (C.c_int, [vp, I32P, C.c_int64, C.POINTER(C.c_int)]),
+        "mst_prep_cell_colours": (C.c_int, [vp, I32P, C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -326,6 +327,13 @@ class Prep:
         col = np.zeros(self.n_faces, np.int32)
         _check(lib().mst_prep_permutation(self.h, cn, fnw, col), "mst_prep_permutation")
         return cn, fnw, col
+
+    def cell_colours(self):
+        """(colour per file-order cell, number of colours) of the LU-SGS sweeps."""
+        col = np.zeros(self.n_cells, np.int32)
+        n = C.c_int()
+        _check(lib().mst_prep_cell_colours(self.h, col, C.byref(n)), "mst_prep_cell_colours")
+        return col, n.value
 
     def geometry(self):
         d, nf, nc = self.dim, self.n_faces, self.n_cells

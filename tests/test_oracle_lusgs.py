@@ -191,3 +191,62 @@ def test_ramp_steady_state_matches_explicit_and_takes_fewer_iterations():
     assert it_rk is not None
     assert it_rk >= 3 * it_lu
     assert np.abs(Q_lu - Q).max() <= 1e-7 * np.abs(Q).max()
+
+
+def _neighbour_sets(raw, geo):
+    return [sorted({o for (_, o, _) in cell if o >= 0 and o != c}) for c, cell in enumerate(incidences(raw, geo))]
+
+
+def _bipartite(nb):
+    col = -np.ones(len(nb), int)
+    for s in range(len(nb)):
+        if col[s] >= 0:
+            continue
+        col[s] = 0
+        stack = [s]
+        while stack:
+            c = stack.pop()
+            for o in nb[c]:
+                if col[o] < 0:
+                    col[o] = 1 - col[c]
+                    stack.append(o)
+                elif col[o] == col[c]:
+                    return False
+    return True
+
+
+COLOUR_MESHES = {
+    "tri": lambda: M.Mesh.gen_tri(30, 20, 0, 3, 0, 2, 0.2, 5, (M.BC_WALL,) * 4),
+    "periodic": lambda: M.Mesh.gen_tri(12, 10, -1, 1, -1, 1, 0.1, 3, (M.BC_PERIODIC,) * 4),
+    "tet": lambda: M.Mesh.gen_tet(5, 4, 3, (0, 0, 0), (1, 0.8, 0.7), 0.15, 11, (M.BC_WALL,) * 6),
+    "stair-un-tri": lambda: M.Mesh.from_raw(_fixture("2d-stair-un-5-tri")),
+    "stairW": lambda: M.Mesh.from_raw(_fixture("2d-stairW-1")),
+}
+
+
+def _fixture(name):
+    import os
+    d = dict(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", f"msh_{name}.npz")))
+    for k in ("dim", "n_cells", "n_int"):
+        d[k] = int(d[k])
+    return d
+
+
+@pytest.mark.parametrize("name", sorted(COLOUR_MESHES))
+def test_sweep_colouring(name):
+    """The LU-SGS colouring: proper; two colours exactly when the cell graph is bipartite; otherwise
+    no more colours than greedy in device order (the Kempe pass only removes colours)."""
+    m = COLOUR_MESHES[name]()
+    raw = m.raw()
+    p = M.Prep(m)
+    col, ncol = p.cell_colours()
+    nb = _neighbour_sets(raw, O.OracleMesh(raw).geometry())
+    assert col.min() == 0 and col.max() == ncol - 1 and len(np.unique(col)) == ncol
+    assert all(col[o] != col[c] for c in range(len(nb)) for o in nb[c])
+    order_dev = np.argsort(p.permutation()[0])
+    greedy = greedy_colours(incidences(raw, O.OracleMesh(raw).geometry()), order_dev)
+    if _bipartite(nb):
+        assert ncol == 2
+    else:
+        assert 3 <= ncol <= greedy.max() + 1
+    assert np.array_equal(col, p.cell_colours()[0])  # deterministic
