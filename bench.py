@@ -53,6 +53,7 @@ TRAFFIC_PROFILES = {("vortex2d", st, k): ("profiles/hbm_traffic_vortex2d.md", na
                                         ("patch", "k_grad", "k_grad<2, 3, 2, double>"),
                                         ("patch", "k_patch", "k_patch<2, 0, true, double, 3>"),
                                         ("fused", "k_fused", "k_fused<2, 0, 2, double, 3>"))}
+TRAFFIC_PROFILES[("steady", "lusgs", "k_lu_sweep")] = ("profiles/hbm_traffic_lusgs_ramp300k.md", "k_lu_sweep")
 
 
 def committed_traffic(workload, strategy, kernel):
@@ -276,6 +277,7 @@ def bench_steady(M, args):
                 kern[name].update(alg_bytes=b, gbs=b / (1e-3 * ms / n) / 1e9)
     sw = kern["k_lu_sweep"]
     ach = sw["alg_bytes"] / (sw["avg_us"] * 1e-6) / 1e9
+    traffic, tsrc = committed_traffic("steady", "lusgs", "k_lu_sweep")
     lu, ex = runs["lusgs"], runs["explicit"]
     out = dict(metric="time-to-1e-8 residual drop from the freestream (steady ramp, LU-SGS driver)",
                value=lu["seconds"], unit="s", n_gpus=1, steps=args.steps, warmup=args.warmup,
@@ -287,7 +289,7 @@ def bench_steady(M, args):
                            cells=m.n_cells, faces=m.n_faces),
                drivers=runs, speedup_vs_explicit=ex["seconds"] / lu["seconds"],
                roofline=dict(bound="hbm", kernel="k_lu_sweep", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
-                             frac=ach / HBM_PEAK_GBS, traffic=None, alg_bytes=sw["alg_bytes"]),
+                             frac=ach / HBM_PEAK_GBS, traffic=traffic, traffic_source=tsrc, alg_bytes=sw["alg_bytes"]),
                kernels=kern)
     if copy:
         out["copy_probe"] = copy
