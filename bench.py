@@ -126,6 +126,23 @@ def cpu_baseline(mesh, cfg_kw, Q, threads, budget_s=10.0):
                        f"{mesh.n_cells}-cell workload, oracle/mst_oracle.c with OpenMP, {t:.2f} s")
 
 
+def host_cpu():
+    """The host the CPU baseline ran on: processor model, logical CPUs and those this process may use."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = None
+    return dict(cpu_model=model, nproc=os.cpu_count(), usable_cpus=usable)
+
+
 def batch_cases(M, n=512, seed=777):
     """512 Sod/Riemann variants on [0,1] x [0,0.01] strips of 2 x nx quads (N_c = 4 nx uniform in
     [500, 50000]), rho_R = 0.125, p_R = 0.1, rho_L/rho_R ~ U[1, 10], Roe or AUSM+ with probability
@@ -297,6 +314,7 @@ def bench_steady(M, args):
         out["note"] = f"to-steady runs capped at {args.max_steps} iterations (profiler pass): value is not a time to 1e-8"
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = lusgs_cpu_baseline(m, mach, s.lusgs_colours()[0], lu["iterations"])
+        out["cpu_baseline"].update(host_cpu())
     s.close()
     print(json.dumps(out))
 
@@ -439,9 +457,19 @@ def main():
                             strategies={st: 1e3 * runs_m[st][0] / args.steps for st in runs_m},
                             compare_steps=nsteps, rel_l2_per_field=l2, rel_linf_per_field=linf)
         solver.set_config(M.make_config(**cfg_kw))
+    # the survey's fused-minimum lower bound: per stage each face's two cell indices and normal and
+    # each cell's state in, volume and state out, once (2D: 24 N_f + 72 N_c bytes)
+    dim = mesh.dim
+    lb = mesh.n_faces * (8 + 8 * dim) + mesh.n_cells * (2 * 8 * (dim + 2) + 8)
+    out["fused_minimum"] = dict(bytes_per_stage=lb, achieved_gbs=lb * stages * args.steps / wall / 1e9,
+                                frac=lb * stages * args.steps / wall / 1e9 / HBM_PEAK_GBS)
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(mesh, cfg_kw, Q0, threads)
+        out["cpu_baseline"].update(host_cpu())
+        if threads != 8:  # the reference's own OpenMP team size (NUM_CPU_THREADS, include/CONST.h:7)
+            t8 = cpu_baseline(mesh, cfg_kw, Q0, 8, budget_s=5.0)
+            out["cpu_baseline"]["threads8"] = dict(value=t8["value"], sample=t8["sample"])
     Qf = solver.get_state()
     out["state_finite"] = bool(np.isfinite(Qf).all())
     print(json.dumps(out))

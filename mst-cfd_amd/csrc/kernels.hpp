@@ -673,7 +673,8 @@ __device__ __forceinline__ void patch_sum(const double *g, int npf, const int (&
     }
 }
 
-template <int DIM, int SCHEME, bool SECOND, typename T, int MAXF>
+// STAGE false: the residual only (R and, with a.l2, its norms), for the implicit steady driver
+template <int DIM, int SCHEME, bool SECOND, typename T, int MAXF, bool STAGE = true>
 __global__ __launch_bounds__(BLOCK) void k_patch(const FluxArgs<T> fa, const UpdArgs a, const PatchArgs p) {
     constexpr int NV = DIM + 2;
     constexpr bool F32 = std::is_same<T, float>::value;
@@ -697,7 +698,7 @@ __global__ __launch_bounds__(BLOCK) void k_patch(const FluxArgs<T> fa, const Upd
     for (int s = 0; s < MAXF; s++) cl[s] = valid ? p.loc[(int64_t)s * a.nc + c] : -1;  // padded to MAXF rows
     double r[NV];
     patch_sum<NV, MAXF, F32>(gsh, npf, cl, r);
-    stage_tail<NV, true, F32>(a, c, valid, r, blk, a.l2, a.dtred);
+    stage_tail<NV, STAGE, F32>(a, c, valid, r, blk, a.l2, a.dtred);
 }
 
 // Order 2 in one kernel: the patch also computes the limited gradients of its own cells and of its

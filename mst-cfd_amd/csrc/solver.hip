@@ -349,6 +349,29 @@ int run_patch_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int 
 #undef MST_PATCH
 }
 
+// the patch kernel's residual-only form (FP64): R of state is, and its L2 partials when mode has UPD_L2
+template <int DIM>
+int run_patch_residual_t(mst_ctx *x, int mode, int is) {
+    const FluxArgs<double> fa = flux_args<double>(x, is);
+    const UpdArgs a = upd_args(x, mode, is, -1, 0, 0, 1, -1);
+    PatchArgs pa;
+    pa.start = x->pstart;
+    pa.face = x->pface;
+    pa.loc = x->ploc;
+    const size_t shm = (size_t)x->max_pf * x->nv * sizeof(double);
+    const bool second = x->cfg.order >= 2;
+    const bool roe = x->cfg.scheme == MST_SCHEME_ROE;
+#define MST_PATCH_R(MF)                                                                                          \
+    if (roe)                                                                                                     \
+        return second ? launch_shm(x, MST_K_PATCH, k_patch<DIM, MST_SCHEME_ROE, true, double, MF, false>, x->nc, shm, fa, a, pa) \
+                      : launch_shm(x, MST_K_PATCH, k_patch<DIM, MST_SCHEME_ROE, false, double, MF, false>, x->nc, shm, fa, a, pa); \
+    return second ? launch_shm(x, MST_K_PATCH, k_patch<DIM, MST_SCHEME_AUSM, true, double, MF, false>, x->nc, shm, fa, a, pa)    \
+                  : launch_shm(x, MST_K_PATCH, k_patch<DIM, MST_SCHEME_AUSM, false, double, MF, false>, x->nc, shm, fa, a, pa);
+    if (x->slots == 3) { MST_PATCH_R(3) }
+    MST_PATCH_R(4)
+#undef MST_PATCH_R
+}
+
 // fused strategy (order 2): gradients, face states, fluxes, sums and update of a patch in one kernel
 template <int DIM, typename T>
 int run_fused_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
@@ -789,9 +812,19 @@ int lu_sweeps(mst_ctx *x, const LuArgs &a) {
 
 // one LU-SGS iteration: residual of Old (+ its L2 norms when l2 is set), the two sweeps into New
 int lu_iteration(mst_ctx *x, bool l2) {
-    int rc = run_face_flux(x, x->iold);
-    if (rc) return rc;
-    rc = run_update(x, l2 ? UPD_L2 : UPD_RESIDUAL, x->iold, -1, 0, 0, 1, -1);
+    int rc;
+    const int mode = l2 ? UPD_L2 : UPD_RESIDUAL;
+    if (x->patch_ok && !x->mixed) {  // one launch: the patch kernel in residual-only form (same R bit for bit)
+        if (x->cfg.order >= 2) {
+            rc = run_grad(x, x->iold, false);
+            if (rc) return rc;
+        }
+        rc = x->dim == 2 ? run_patch_residual_t<2>(x, mode, x->iold) : run_patch_residual_t<3>(x, mode, x->iold);
+    } else {
+        rc = run_face_flux(x, x->iold);
+        if (rc) return rc;
+        rc = run_update(x, mode, x->iold, -1, 0, 0, 1, -1);
+    }
     if (rc) return rc;
     if (l2) {
         rc = run_finish(x, x->nv, 0);
