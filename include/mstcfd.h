@@ -159,6 +159,25 @@ int mst_prep_tables(const mst_prep *p, int32_t *fc0, int32_t *fc1, int32_t *kind
  * file order (colour[n_cells]), and the number of colours */
 int mst_prep_cell_colours(const mst_prep *p, int32_t *colour, int *n_colours);
 
+/* ---------------------------------------------------------------- on-disk formats (file order)
+ * Tecplot ASCII result file: Work::writedataRhoBasedMshNodePlt (work/Work.cpp:204-304) -- cell
+ * states carried to faces (interior eta0-weighted, boundary the cell's value, walls with zero
+ * velocity when wall_noslip, as FLAG_WALLSMOOTH = 0), then to nodes (1/area-weighted over the node's
+ * faces), written as X Y [Z] rho u v [w] T p Ma per node (T = e/cv, include/CONST.h:39) plus the
+ * cells' nodes.  Fixed against the reference: zone type from the mesh (triangles, quadrilaterals,
+ * tetrahedra; :231-235 uses the FELNUM macro), each node's own face weights (:292-293 indexes faces
+ * by node number), values on symmetry and periodic faces, a well-formed TITLE and a 3D w column. */
+int mst_write_tecplot(const mst_mesh *m, const double *Q, int64_t n_cells, double gamma, double cv, int wall_noslip,
+                      const char *title, const char *path);
+/* Binary checkpoint replacing the text .cellPUVT dump (Work::writedataCellContinue /
+ * readPbasedCellPlt, work/Work.cpp:432-514): the state (n_cells x nv doubles, file order) with a step
+ * counter and a time, a mesh fingerprint and a state checksum.  Reading restores the state bit for
+ * bit; a file for another mesh is MST_E_ARG, a damaged or truncated one MST_E_IO.  The write goes
+ * through a temporary file renamed into place. */
+int mst_checkpoint_write(const char *path, const mst_mesh *m, const double *Q, int64_t n_cells, int64_t step,
+                         double time);
+int mst_checkpoint_read(const char *path, const mst_mesh *m, double *Q, int64_t n_cells, int64_t *step, double *time);
+
 /* ---------------------------------------------------------------- solver context
  * RhoSolver::RhoSolver (rhoSolver/RhoSolver.cpp:3-26) + AllData::createAllData (data/AllData.cpp:3-28):
  * preprocesses the mesh (as mst_prep_create), uploads the SoA arrays and allocates the state

@@ -50,6 +50,27 @@ int mst_mesh_write_msh(const mst_mesh *m, const char *path) {
     MST_GUARD(return mst::write_msh(*m, path));
 }
 
+int mst_write_tecplot(const mst_mesh *m, const double *Q, int64_t n_cells, double gamma, double cv, int wall_noslip,
+                      const char *title, const char *path) {
+    if (!m || !Q || !path) return mst::fail(MST_E_ARG, "null argument");
+    if (n_cells != m->n_cells) return mst::fail(MST_E_ARG, "size does not match the mesh");
+    if (!(gamma > 1) || !(cv > 0)) return mst::fail(MST_E_ARG, "gamma must exceed 1 and cv be positive");
+    MST_GUARD(return mst::write_tecplot(*m, Q, gamma, cv, wall_noslip, title ? title : "mst-cfd", path));
+}
+
+int mst_checkpoint_write(const char *path, const mst_mesh *m, const double *Q, int64_t n_cells, int64_t step,
+                         double time) {
+    if (!m || !Q || !path) return mst::fail(MST_E_ARG, "null argument");
+    if (n_cells != m->n_cells) return mst::fail(MST_E_ARG, "size does not match the mesh");
+    MST_GUARD(return mst::write_checkpoint(path, *m, Q, step, time));
+}
+
+int mst_checkpoint_read(const char *path, const mst_mesh *m, double *Q, int64_t n_cells, int64_t *step, double *time) {
+    if (!m || !Q || !path) return mst::fail(MST_E_ARG, "null argument");
+    if (n_cells != m->n_cells) return mst::fail(MST_E_ARG, "size does not match the mesh");
+    MST_GUARD(return mst::read_checkpoint(path, *m, Q, step, time));
+}
+
 int mst_mesh_from_arrays(int dim, int64_t n_nodes, const double *xyz, int64_t n_cells, int64_t n_faces, int fnn,
                          const int32_t *fnodes, const int32_t *c0, const int32_t *c1, const int32_t *ftype,
                          const int32_t *partner, mst_mesh **out) {

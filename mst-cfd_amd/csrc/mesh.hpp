@@ -56,6 +56,12 @@ int gen_ramp(int nx, int ny, double x0, double x1, double height, double theta_d
 // cell centres (mean of face centres, mesh/Cell.cpp:6-14) in file order
 int cell_centres(const mst_mesh &m, std::vector<double> &ccen);
 
+// on-disk formats (output.cpp): Tecplot ASCII result file, binary checkpoint
+int write_tecplot(const mst_mesh &m, const double *Q, double gamma, double cv, int wall_noslip, const std::string &title,
+                  const std::string &path);
+int write_checkpoint(const std::string &path, const mst_mesh &m, const double *Q, int64_t step, double time);
+int read_checkpoint(const std::string &path, const mst_mesh &m, double *Q, int64_t *step, double *time);
+
 // std::mt19937_64 (its output sequence is fixed by the C++ standard) with a portable 53-bit
 // uniform in [0,1), so generated meshes are identical on every host
 struct Rng {

@@ -52,6 +52,7 @@ EXPORTS = [
     "mst_mlp_gradient", "mst_ctx_warm_start", "mst_run_to_steady", "mst_batch_run_to_steady",
     "mst_mesh_cell_centres", "mst_batch_set_strategy", "mst_batch_get_strategy", "mst_set_cell_order",
     "mst_get_cell_order", "mst_set_steady_driver", "mst_lusgs_step", "mst_lusgs_colours", "mst_prep_cell_colours",
+    "mst_write_tecplot", "mst_checkpoint_write", "mst_checkpoint_read",
 ]
 
 
@@ -195,6 +196,11 @@ def lib():
         "mst_lusgs_step": (C.c_int, [vp, C.c_int]),
         "mst_lusgs_colours": (C.c_int, [vp, I32P, C.c_int64, C.POINTER(C.c_int)]),
         "mst_prep_cell_colours": (C.c_int, [vp, I32P, C.POINTER(C.c_int)]),
+        "mst_write_tecplot": (C.c_int, [vp, F64P, C.c_int64, C.c_double, C.c_double, C.c_int, C.c_char_p,
+                                        C.c_char_p]),
+        "mst_checkpoint_write": (C.c_int, [C.c_char_p, vp, F64P, C.c_int64, C.c_int64, C.c_double]),
+        "mst_checkpoint_read": (C.c_int, [C.c_char_p, vp, F64P, C.c_int64, C.POINTER(C.c_int64),
+                                          C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -279,6 +285,26 @@ class Mesh:
                         int(fn.shape[0]), int(fn.shape[1]), fn.reshape(-1), np.ascontiguousarray(raw["c0"], np.int32),
                         np.ascontiguousarray(raw["c1"], np.int32), np.ascontiguousarray(raw["ftype"], np.int32),
                         np.ascontiguousarray(raw["partner"], np.int32))
+
+    def write_tecplot(self, Q, path, gamma=1.4, cv=715.8, wall_noslip=False, title="mst-cfd"):
+        """Tecplot ASCII file of state Q (file order): nodal X Y [Z] rho u v [w] T p Ma + the cells' nodes."""
+        Q = np.ascontiguousarray(Q, np.float64).reshape(self.n_cells, self.nv)
+        _check(lib().mst_write_tecplot(self.h, Q.reshape(-1), self.n_cells, float(gamma), float(cv),
+                                       1 if wall_noslip else 0, title.encode(), str(path).encode()),
+               "mst_write_tecplot")
+
+    def write_checkpoint(self, path, Q, step=0, time=0.0):
+        Q = np.ascontiguousarray(Q, np.float64).reshape(self.n_cells, self.nv)
+        _check(lib().mst_checkpoint_write(str(path).encode(), self.h, Q.reshape(-1), self.n_cells, int(step),
+                                          float(time)), "mst_checkpoint_write")
+
+    def read_checkpoint(self, path):
+        """(state, step, time) of a checkpoint written for this mesh."""
+        Q = np.zeros((self.n_cells, self.nv))
+        st, t = C.c_int64(), C.c_double()
+        _check(lib().mst_checkpoint_read(str(path).encode(), self.h, Q.reshape(-1), self.n_cells, C.byref(st),
+                                         C.byref(t)), "mst_checkpoint_read")
+        return Q, st.value, t.value
 
     def write_msh(self, path):
         _check(lib().mst_mesh_write_msh(self.h, str(path).encode()), "mst_mesh_write_msh")

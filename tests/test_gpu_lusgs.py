@@ -110,3 +110,22 @@ def test_lusgs_rejects_bad_settings():
     with pytest.raises(M.MstError):
         s.lusgs_step(-1)
     s.close()
+
+
+def test_lusgs_mixed_precision_follows_fp64():
+    """Mixed form: the residual comes from the FP32 flux kernels (k_flux + k_update), the sweeps stay
+    FP64.  After 20 iterations the state stays within the mixed form's rounding of the FP64 one."""
+    m = M.Mesh.gen_tri(60, 40, 0, 3, 0, 2, 0.1, 9, (M.BC_INLET, M.BC_OUTLET, M.BC_WALL, M.BC_SYMMETRY))
+    nc = m.n_cells
+    Q0 = mild_state(nc, 2, seed=8)
+    out = {}
+    for prec in (M.PREC_FP64, M.PREC_MIXED):
+        s = M.Solver(m, M.make_config(scheme=M.ROE, order=2, limiter=M.LIM_MINMOD, q_inlet=inlet(2), precision=prec))
+        s.set_state(Q0)
+        s.set_steady_driver(M.STEADY_LUSGS, 50.0, 1.0)
+        s.lusgs_step(20)
+        out[prec] = (s.get_state(), s.residual_norms())
+        s.close()
+    (Qd, rd), (Qm, rm) = out[M.PREC_FP64], out[M.PREC_MIXED]
+    assert np.abs(Qm - Qd).max() <= 1e-5 * np.abs(Qd).max()
+    assert np.allclose(rm, rd, rtol=1e-4)
