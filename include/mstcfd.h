@@ -71,6 +71,13 @@ typedef struct {
     double entropy_delta;  /* 0.125 (rhoSolver/SolverRoe.cpp:115) */
     double cfl;            /* > 0: mst_step computes dt = cfl * min V / sum (|u_d|+a) S_d; 0: use mst_set_dt */
     double q_inlet[5];     /* inlet ghost state (dim+2 used) */
+    /* NEW viscous terms (the intent of RhoSolver::updateViscid, rhoSolver/RhoSolver.cpp:371-429; the
+     * reference runs inviscid, FLAGVISCID 0): mu = kappa = 0 is inviscid.  Viscous runs use the face
+     * strategy and FP64 or mixed single contexts (not batches). */
+    double mu;             /* dynamic viscosity (include/CONST.h:46: 1.7894e-5) */
+    double lam;            /* bulk coefficient (CONST.h:47: -0.666667 mu) */
+    double kappa;          /* thermal conductivity (CONST.h:48 TEMPK: 0.0242) */
+    double cv;             /* T = e / cv (CONST.h:39: 715.8); > 0 when viscous */
 } mst_config;
 
 typedef struct mst_mesh mst_mesh; /* host mesh in file order (MshBlock, mesh/MshBlock.h:19-76) */
@@ -366,7 +373,8 @@ int mst_ctx_warm_start(mst_ctx *ctx, mst_mlp *m, const double *cond, int n_cond)
 #define MST_K_PATCH 5 /* patch fluxes in LDS + gather + update (MST_STRATEGY_PATCH) */
 #define MST_K_FUSED 6 /* gradients + face states + fluxes in LDS + gather + update (MST_STRATEGY_FUSED) */
 #define MST_K_LUSGS 7 /* LU-SGS colour sweeps (mst_lusgs_step); algorithmic bytes: per launch, averaged */
-#define MST_K_COUNT 8
+#define MST_K_VISC 8 /* viscous gradient + face kernels (mu or kappa > 0); algorithmic bytes: per launch, averaged */
+#define MST_K_COUNT 9
 int mst_timing_enable(mst_ctx *ctx, int on);
 /* per kernel class: launches timed and total milliseconds since the last reset */
 int mst_timing_get(mst_ctx *ctx, int kernel, int64_t *count, double *total_ms);

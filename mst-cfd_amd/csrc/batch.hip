@@ -1114,6 +1114,8 @@ int mst_batch_create(int n_cases, const mst_mesh *const *meshes, const mst_confi
         if (!meshes[i]) return mst::fail(MST_E_ARG, "null mesh");
         int rc = check_cfg(&cfgs[i]);
         if (rc) return rc;
+        if (cfgs[i].mu != 0 || cfgs[i].kappa != 0)
+            return mst::fail(MST_E_UNSUPPORTED, "batches run inviscid cases only (mu = kappa = 0)");
         const mst_config &a = cfgs[0], &c = cfgs[i];
         if (c.order != a.order || c.limiter != a.limiter || c.rk != a.rk || c.precision != a.precision ||
             meshes[i]->dim != meshes[0]->dim)

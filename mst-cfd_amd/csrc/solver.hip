@@ -12,6 +12,7 @@
 
 #include "kernels.hpp"
 #include "lusgs.hpp"
+#include "viscous.hpp"
 #include "mlp.hpp"
 #include "mstcfd.h"
 #include "prep.hpp"
@@ -41,6 +42,10 @@ int check_config(const mst_config *c) {
     if (!(c->gamma > 1) || !(c->entropy_delta > 0) || c->eor < 0 || c->cfl < 0)
         return mst::fail(MST_E_ARG, "bad gamma / entropy_delta / eor / cfl");
     if (!(c->q_inlet[0] > 0)) return mst::fail(MST_E_ARG, "inlet density must be positive");
+    if (!(c->mu >= 0) || !(c->kappa >= 0) || !std::isfinite(c->mu) || !std::isfinite(c->kappa) ||
+        !std::isfinite(c->lam))
+        return mst::fail(MST_E_ARG, "viscous coefficients must be finite, mu and kappa >= 0");
+    if ((c->mu > 0 || c->kappa > 0) && !(c->cv > 0)) return mst::fail(MST_E_ARG, "viscous runs need cv > 0");
     return MST_OK;
 }
 
@@ -92,6 +97,8 @@ struct mst_ctx {
     double *fin_mid = nullptr;  // first-level results of the two-level finish
     bool dt_valid = false;   // scal[0] holds the CFL time step of the current Old state
     int *bad = nullptr;
+    bool viscous = false;     // mu or kappa > 0: the face strategy with the viscous kernels
+    double *gw = nullptr;     // viscous: per cell Green-Gauss gradients of (u, T), [nc][dim + 1][dim]
     // LU-SGS steady driver (built on first use): cell colours, the cells of each colour in ascending
     // order (colour k: lu_cells[lu_start[k] .. lu_start[k + 1])), face spectral radii, the update
     int32_t *lu_cells = nullptr, *lu_nb = nullptr;
@@ -234,7 +241,40 @@ int run_flux_t(mst_ctx *x, int isrc) {
                   : launch(x, MST_K_FLUX, k_flux<DIM, MST_SCHEME_AUSM, false, T>, x->nf, a);
 }
 
-// residual evaluation up to the face fluxes: gradient + limiter (order 2) then the flux kernel
+// viscous terms of state isrc subtracted from the face fluxes (after the flux kernel)
+int run_viscous(mst_ctx *x, int isrc) {
+    ViscArgs a;
+    a.nc = x->nc;
+    a.nf = x->nf;
+    a.inc_code = x->inc_code;
+    a.inc_w = x->inc_w;
+    a.inc_n = x->inc_n;
+    a.vol = x->cvol;
+    a.fc0 = x->fc0;
+    a.fc1 = x->fc1;
+    a.feta = x->feta;
+    a.fn = x->fn;
+    a.Q = x->Q[isrc];
+    a.gw = x->gw;
+    a.G = x->G;
+    a.mu = x->cfg.mu;
+    a.lam = x->cfg.lam;
+    a.kappa = x->cfg.kappa;
+    a.cv = x->cfg.cv;
+    int rc;
+    if (x->dim == 2)
+        rc = x->slots == 3 ? launch(x, MST_K_VISC, k_visc_grad<2, 3>, x->nc, a)
+                           : (x->slots == 4 ? launch(x, MST_K_VISC, k_visc_grad<2, 4>, x->nc, a)
+                                            : launch(x, MST_K_VISC, k_visc_grad<2, 8>, x->nc, a));
+    else
+        rc = x->slots == 4 ? launch(x, MST_K_VISC, k_visc_grad<3, 4>, x->nc, a)
+                           : launch(x, MST_K_VISC, k_visc_grad<3, 8>, x->nc, a);
+    if (rc) return rc;
+    return x->dim == 2 ? launch(x, MST_K_VISC, k_visc_flux<2>, x->nf, a) : launch(x, MST_K_VISC, k_visc_flux<3>, x->nf, a);
+}
+
+// residual evaluation up to the face fluxes: gradient + limiter (order 2), the flux kernel, and the
+// viscous terms when the configuration has them
 int run_face_flux(mst_ctx *x, int isrc) {
     int rc = ensure_shadow(x, isrc);
     if (rc) return rc;
@@ -242,8 +282,12 @@ int run_face_flux(mst_ctx *x, int isrc) {
         rc = run_grad(x, isrc, false);
         if (rc) return rc;
     }
-    if (x->mixed) return x->dim == 2 ? run_flux_t<2, float>(x, isrc) : run_flux_t<3, float>(x, isrc);
-    return x->dim == 2 ? run_flux_t<2, double>(x, isrc) : run_flux_t<3, double>(x, isrc);
+    if (x->mixed)
+        rc = x->dim == 2 ? run_flux_t<2, float>(x, isrc) : run_flux_t<3, float>(x, isrc);
+    else
+        rc = x->dim == 2 ? run_flux_t<2, double>(x, isrc) : run_flux_t<3, double>(x, isrc);
+    if (rc || !x->viscous) return rc;
+    return run_viscous(x, isrc);
 }
 
 UpdArgs upd_args(const mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
@@ -415,7 +459,7 @@ int run_stage(mst_ctx *x, int mode, int is, int in, double ca, double cb, int st
         int rc = resolve_strategy(x);
         if (rc) return rc;
     }
-    if (x->resolved == MST_STRATEGY_FACE) {
+    if (x->resolved == MST_STRATEGY_FACE || x->viscous) {  // the viscous kernels follow the flux kernel
         int rc = run_face_flux(x, is);
         if (rc) return rc;
         return run_update(x, mode, is, in, ca, cb, stage0, iout);
@@ -690,6 +734,11 @@ int upload_config(mst_ctx *x) {
     x->p64 = make_params<double>(x->cfg);
     x->p32 = make_params<float>(x->cfg);
     x->mixed = x->cfg.precision == MST_PREC_MIXED;
+    x->viscous = x->cfg.mu > 0 || x->cfg.kappa > 0;
+    if (x->viscous && !x->gw) {
+        int rc = dalloc(&x->gw, (size_t)(x->nc * (x->dim + 1) * x->dim));
+        if (rc) return rc;
+    }
     if (x->mixed) {
         int rc = alloc_mixed(x);
         if (rc) return rc;
@@ -705,7 +754,7 @@ void release(mst_ctx *x) {
                     x->scal, x->part, x->fin_mid, x->bad, x->Q32[0], x->Q32[1], x->Q32[2], x->Q32[3], x->rec32,
                     x->frec0, x->frec1, x->inc_code, x->inc_w, x->inc_n, x->inc_dx, x->inc_w32, x->inc_n32,
                     x->inc_dx32, x->pstart, x->pface, x->ploc, x->hstart, x->hcell, x->hloc,
-                    x->lu_cells, x->lu_nb, x->lu_n, x->lu_r, x->lu_D, x->lu_dQ};
+                    x->lu_cells, x->lu_nb, x->lu_n, x->lu_r, x->lu_D, x->lu_dQ, x->gw};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (int k = 0; k < MST_K_COUNT; k++)
@@ -814,7 +863,7 @@ int lu_sweeps(mst_ctx *x, const LuArgs &a) {
 int lu_iteration(mst_ctx *x, bool l2) {
     int rc;
     const int mode = l2 ? UPD_L2 : UPD_RESIDUAL;
-    if (x->patch_ok && !x->mixed) {  // one launch: the patch kernel in residual-only form (same R bit for bit)
+    if (x->patch_ok && !x->mixed && !x->viscous) {  // one launch: the patch kernel's residual-only form (same R)
         if (x->cfg.order >= 2) {
             rc = run_grad(x, x->iold, false);
             if (rc) return rc;
@@ -1506,6 +1555,13 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
         case MST_K_DT:
             *bytes = nc * (8 * NV + 8 + 8 * D);
             break;
+        case MST_K_VISC: {  // the gradient kernel and the face kernel, averaged over the two launches
+            const double nw = D + 1;
+            const double grad = 4.0 * ctx->slots * nc + ninc * (8 + 8 * D) + nc * (8 * NV + 8 + 8 * nw * D);
+            const double face = nf * (8 + 8 + 8 * D + 2 * 8 * NV) + nc * (8 * NV + 8 * nw * D);
+            *bytes = 0.5 * (grad + face);
+            break;
+        }
         case MST_K_LUSGS: {  // one iteration's sweeps, each array once per sweep, over their launches
             if (ctx->lu_start.size() < 2) return mst::fail(MST_E_ARG, "no LU-SGS iteration has run");
             const double S = ctx->slots, launches = 2.0 * (double)(ctx->lu_start.size() - 1) - 1;

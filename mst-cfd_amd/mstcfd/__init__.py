@@ -24,9 +24,9 @@ PREC_FP64, PREC_MIXED = 0, 1
 COMPAT_BRANCH2_LR, COMPAT_GRAD_LASTINT, COMPAT_AUSM_LOCAL_A, COMPAT_AUSM_P_UNSCALED = 1, 2, 4, 8
 COMPAT_AUSM_ONESIDED = 16
 COMPAT_REFERENCE = COMPAT_GRAD_LASTINT | COMPAT_AUSM_LOCAL_A | COMPAT_AUSM_P_UNSCALED | COMPAT_AUSM_ONESIDED
-K_GRAD, K_FLUX, K_UPDATE, K_DT, K_CELL, K_PATCH, K_FUSED, K_LUSGS = 0, 1, 2, 3, 4, 5, 6, 7
+K_GRAD, K_FLUX, K_UPDATE, K_DT, K_CELL, K_PATCH, K_FUSED, K_LUSGS, K_VISC = 0, 1, 2, 3, 4, 5, 6, 7, 8
 KERNEL_NAMES = {K_GRAD: "k_grad", K_FLUX: "k_flux", K_UPDATE: "k_update", K_DT: "k_cfl", K_CELL: "k_cell",
-                K_PATCH: "k_patch", K_FUSED: "k_fused", K_LUSGS: "k_lu_sweep"}
+                K_PATCH: "k_patch", K_FUSED: "k_fused", K_LUSGS: "k_lu_sweep", K_VISC: "k_visc"}
 STRATEGY_FACE, STRATEGY_CELL, STRATEGY_AUTO, STRATEGY_PATCH, STRATEGY_FUSED = 0, 1, 2, 3, 4
 INIT_REFERENCE, INIT_UNIFORM, INIT_RIEMANN, INIT_VORTEX, INIT_PULSE, INIT_FREESTREAM = 0, 1, 2, 3, 4, 5
 BATCH_BUCKETS = 4
@@ -75,6 +75,10 @@ class Config(C.Structure):
         ("entropy_delta", C.c_double),
         ("cfl", C.c_double),
         ("q_inlet", C.c_double * 5),
+        ("mu", C.c_double),
+        ("lam", C.c_double),
+        ("kappa", C.c_double),
+        ("cv", C.c_double),
     ]
 
 
@@ -85,10 +89,14 @@ def reference_inlet(dim=2):
 
 
 def make_config(scheme=ROE, order=1, limiter=LIM_NONE, rk=1, compat=COMPAT_REFERENCE, gamma=1.4, eor=1e-10,
-                entropy_delta=0.125, cfl=0.0, q_inlet=None, dim=2, precision=PREC_FP64):
+                entropy_delta=0.125, cfl=0.0, q_inlet=None, dim=2, precision=PREC_FP64, mu=0.0, lam=None, kappa=0.0,
+                cv=715.8):
+    """mu / lam / kappa / cv: viscous terms (mu = kappa = 0: inviscid, the reference's FLAGVISCID 0);
+    lam defaults to -0.666667 mu (include/CONST.h:47)."""
     c = Config()
     c.scheme, c.order, c.limiter, c.rk, c.compat, c.precision = scheme, order, limiter, rk, compat, precision
     c.gamma, c.eor, c.entropy_delta, c.cfl = gamma, eor, entropy_delta, cfl
+    c.mu, c.lam, c.kappa, c.cv = mu, (-0.666667 * mu if lam is None else lam), kappa, cv
     q = list(reference_inlet(dim) if q_inlet is None else q_inlet)
     q = q + [0.0] * (5 - len(q))
     for i in range(5):

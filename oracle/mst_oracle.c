@@ -756,10 +756,98 @@ static void gather(const orc_mesh *m, const double *F, double *R, double *Sabs) 
     }
 }
 
+/* ---------------------------------------------------------------- viscous flux (NEW)
+ * RhoSolver::updateViscid (rhoSolver/RhoSolver.cpp:371-429) states the algorithm but cannot run as
+ * written (cell gradients read the face array by cell index, :383-388; velocity components read the
+ * wrong state slots, :386; the stress indexes a third direction, :404; boundary faces read the face
+ * gradient array by cell id, :398; the heat flux takes one matrix element, :418), and the residual
+ * never uses it (FLAGVISCID 0, include/CONST.h:14).  Restated as intended:
+ *   Q_f      as the gradient's face values (eta0 interpolation; boundary: the cell; :373-378)
+ *   W_f      = (u_1..u_dim, T) of Q_f, T = e / cv with e the internal energy per mass
+ *   grad W_c = (1/V) sum_f W_f n_out                                              (:380-391)
+ *   grad W_f = eta0 grad W_c0 + (1 - eta0) grad W_c1; boundary: grad W_c0          (:392-399)
+ *   tau_id   = mu (du_i/dx_d + du_d/dx_i) + lam (div u) delta_id                   (:401-413)
+ *   Fv[d]    = (0, tau_1d .. tau_dim,d, sum_i u_i tau_id + kappa dT/dx_d)           (:415-419)
+ * and R_i = sum_f n_out . (F_conv - Fv). */
+static void prim_w(const double *q, int dim, double cv, double *w) {
+    double ke = 0;
+    for (int d = 0; d < dim; d++) {
+        w[d] = q[1 + d] / q[0];
+        ke += q[1 + d] * q[1 + d];
+    }
+    w[dim] = (q[dim + 1] - 0.5 * ke / q[0]) / q[0] / cv;
+}
+
+int orc_viscous_flux(const orc_mesh *m, const orc_config *cfg, const double *Q, double *Fv, double *gradW) {
+    const int nv = m->nv, dim = m->dim, nw = dim + 1;
+    double *Qf = (double *)malloc(sizeof(double) * m->nf * nv);
+    double *gw = (double *)calloc((size_t)m->nc * nw * dim, sizeof(double));
+    if (!Qf || !gw) {
+        free(Qf);
+        free(gw);
+        return -1;
+    }
+    face_values(m, cfg, Q, Qf);
+    for (int64_t c = 0; c < m->nc; c++) {
+        double *g = gw + c * nw * dim;
+        for (int64_t j = m->cptr[c]; j < m->cptr[c + 1]; j++) {
+            const int32_t f = m->cface[j];
+            double w[MAXD + 1];
+            prim_w(Qf + (int64_t)f * nv, dim, cfg->cv, w);
+            for (int k = 0; k < nw; k++)
+                for (int d = 0; d < dim; d++) g[k * dim + d] += w[k] * nout(m, c, f, d);
+        }
+        for (int k = 0; k < nw * dim; k++) g[k] /= m->vol[c];
+    }
+    const int quirk = (cfg->compat & ORC_COMPAT_GRAD_LASTINT) && m->n_int >= 1 && m->n_int <= m->nf;
+    for (int64_t f = 0; f < m->nf; f++) {
+        const double *g0 = gw + (int64_t)m->c0[f] * nw * dim;
+        int64_t o = -1;
+        if (m->c1[f] >= 0 && f < m->n_int && !(quirk && f == m->n_int - 1))
+            o = m->c1[f];
+        else if (m->ftype[f] == ORC_PERIODIC && m->partner[f] >= 0)
+            o = m->c0[m->partner[f]];
+        double gf[(MAXD + 1) * MAXD];
+        const double e = m->eta[f];
+        for (int k = 0; k < nw * dim; k++) gf[k] = o >= 0 ? e * g0[k] + (1 - e) * gw[o * nw * dim + k] : g0[k];
+        double w[MAXD + 1];
+        prim_w(Qf + f * nv, dim, cfg->cv, w);
+        double div = 0;
+        for (int d = 0; d < dim; d++) div += gf[d * dim + d];
+        for (int d = 0; d < dim; d++) {
+            double *F = Fv + (f * dim + d) * nv;
+            F[0] = 0;
+            double en = 0;
+            for (int i = 0; i < dim; i++) {
+                double tau = cfg->mu * (gf[i * dim + d] + gf[d * dim + i]);
+                if (i == d) tau += cfg->lam * div;
+                F[1 + i] = tau;
+                en += w[i] * tau;
+            }
+            F[nv - 1] = en + cfg->kappa * gf[dim * dim + d];
+        }
+    }
+    if (gradW) memcpy(gradW, gw, sizeof(double) * m->nc * nw * dim);
+    free(Qf);
+    free(gw);
+    return 0;
+}
+
 int orc_residual(const orc_mesh *m, const orc_config *cfg, const double *Q, double *R, double *Sabs) {
-    double *F = (double *)malloc(sizeof(double) * m->nf * m->dim * m->nv);
+    const size_t n = (size_t)m->nf * m->dim * m->nv;
+    double *F = (double *)malloc(sizeof(double) * n);
     if (!F) return -1;
     orc_face_flux(m, cfg, Q, F);
+    if (cfg->mu > 0 || cfg->kappa > 0) {
+        double *Fv = (double *)malloc(sizeof(double) * n);
+        if (!Fv || orc_viscous_flux(m, cfg, Q, Fv, NULL)) {
+            free(Fv);
+            free(F);
+            return -1;
+        }
+        for (size_t i = 0; i < n; i++) F[i] -= Fv[i];
+        free(Fv);
+    }
     gather(m, F, R, Sabs);
     free(F);
     return 0;

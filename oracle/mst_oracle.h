@@ -52,6 +52,10 @@ typedef struct {
     double entropy_delta;  /* 0.125: Harten fix */
     double cfl;
     double q_inlet[5];     /* inlet ghost state */
+    double mu;             /* NEW viscous terms: dynamic viscosity (0: inviscid), include/CONST.h:46 */
+    double lam;            /* bulk coefficient lambda (CONST.h:47: -0.666667 mu) */
+    double kappa;          /* thermal conductivity (CONST.h:48 TEMPK) */
+    double cv;             /* T = e / cv (CONST.h:39) */
 } orc_config;
 
 typedef struct orc_mesh orc_mesh;
@@ -86,6 +90,11 @@ int orc_euler(const orc_mesh *m, const orc_config *cfg, const double *Qold, doub
 double orc_cfl_dt(const orc_mesh *m, const orc_config *cfg, const double *Q);
 int orc_rk_step(const orc_mesh *m, const orc_config *cfg, double *Q, double dt, double *res_l2);
 int orc_monitor_maxrel(int64_t n_cells, int nv, const double *Qnew, const double *Qold, double *out);
+
+/* NEW viscous flux (the intent of RhoSolver::updateViscid, rhoSolver/RhoSolver.cpp:371-429):
+ * Fv[n_faces][dim][nv]; orc_residual subtracts it from the convective flux when mu or kappa > 0.
+ * gradW: optional [n_cells][dim+1][dim] Green-Gauss gradients of (u_1..u_dim, T). */
+int orc_viscous_flux(const orc_mesh *m, const orc_config *cfg, const double *Q, double *Fv, double *gradW);
 
 /* NEW steady driver: matrix-free LU-SGS over a cell colouring (colour[c] >= 0 per file-order cell,
  * no two neighbours alike).  sweep: dQ from a given residual R; step: R(Q), res_l2 (as orc_rk_step),

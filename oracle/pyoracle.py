@@ -38,6 +38,10 @@ class Config(C.Structure):
         ("entropy_delta", C.c_double),
         ("cfl", C.c_double),
         ("q_inlet", C.c_double * 5),
+        ("mu", C.c_double),
+        ("lam", C.c_double),
+        ("kappa", C.c_double),
+        ("cv", C.c_double),
     ]
 
 
@@ -49,10 +53,11 @@ def reference_inlet(dim=2):
 
 
 def make_config(scheme=ROE, order=1, limiter=LIM_NONE, rk=1, compat=COMPAT_REFERENCE, gamma=1.4, eor=1e-10,
-                entropy_delta=0.125, cfl=0.0, q_inlet=None, dim=2):
+                entropy_delta=0.125, cfl=0.0, q_inlet=None, dim=2, mu=0.0, lam=None, kappa=0.0, cv=715.8):
     c = Config()
     c.scheme, c.order, c.limiter, c.rk, c.compat = scheme, order, limiter, rk, compat
     c.gamma, c.eor, c.entropy_delta, c.cfl = gamma, eor, entropy_delta, cfl
+    c.mu, c.lam, c.kappa, c.cv = mu, (-0.666667 * mu if lam is None else lam), kappa, cv
     q = reference_inlet(dim) if q_inlet is None else list(q_inlet) + [0.0] * (5 - len(q_inlet))
     for i in range(5):
         c.q_inlet[i] = q[i]
@@ -92,6 +97,7 @@ def lib():
     L.orc_cfl_dt.restype = C.c_double
     L.orc_rk_step.argtypes = [C.c_void_p, C.POINTER(Config), F64P, C.c_double, F64P]
     L.orc_monitor_maxrel.argtypes = [C.c_int64, C.c_int, F64P, F64P, F64P]
+    L.orc_viscous_flux.argtypes = [C.c_void_p, C.POINTER(Config), F64P, F64P, F64P]
     L.orc_lusgs_sweep.argtypes = [C.c_void_p, C.POINTER(Config), F64P, F64P, I32P, C.c_double, C.c_double, F64P]
     L.orc_lusgs_step.argtypes = [C.c_void_p, C.POINTER(Config), F64P, I32P, C.c_double, C.c_double, F64P]
     _LIB = L
@@ -179,6 +185,15 @@ class OracleMesh:
         res = np.zeros(self.nv)
         lib().orc_rk_step(self.h, C.byref(cfg), Qc.reshape(-1), float(dt), res)
         return Qc, res
+
+    def viscous_flux(self, cfg, Q):
+        """(Fv [n_faces][dim][nv], Green-Gauss gradients of (u, T) [n_cells][dim+1][dim])"""
+        Fv = np.zeros((self.n_faces, self.dim, self.nv))
+        gw = np.zeros((self.n_cells, self.dim + 1, self.dim))
+        if lib().orc_viscous_flux(self.h, C.byref(cfg), np.ascontiguousarray(Q, np.float64).reshape(-1),
+                                  Fv.reshape(-1), gw.reshape(-1)):
+            raise MemoryError("orc_viscous_flux")
+        return Fv, gw
 
     def lusgs_sweep(self, cfg, Q, R, colour, cfl, omega):
         dQ = np.zeros((self.n_cells, self.nv))

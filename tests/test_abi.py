@@ -87,7 +87,7 @@ def test_ctx_requires_a_device():
 
 
 def test_config_layout_matches_oracle():
-    assert C.sizeof(M.Config) == C.sizeof(O.Config) == 4 * 6 + 8 * 4 + 8 * 5
+    assert C.sizeof(M.Config) == C.sizeof(O.Config) == 4 * 6 + 8 * 4 + 8 * 5 + 8 * 4
     for (a, ta), (b, tb) in zip(M.Config._fields_, O.Config._fields_):
         assert a == b and ta == tb
     assert np.allclose(M.reference_inlet(), O.reference_inlet()[:4])
