@@ -19,7 +19,7 @@
 //                (and the admissibility check)
 // Both sweeps are memory-bound gathers over the neighbours of a cell, like the residual's gather.
 #pragma once
-#include "device_common.hpp"
+#include "kernels.hpp"
 
 namespace mstk {
 
@@ -41,6 +41,10 @@ struct LuArgs {
     double *__restrict__ Qout;
     int *__restrict__ bad;
     double gamma, dscale, omega;         // dscale = 1/CFL + omega/2
+    // first-order residual computed by the forward sweep itself (RSCHEME >= 0)
+    const int32_t *__restrict__ face;    // (face << 1) | is_c1 per (slot, position)
+    const int32_t *__restrict__ finfo;   // per face: kind | (L/R mask << 4)
+    Params<double> P;
 };
 
 template <int DIM>
@@ -84,8 +88,12 @@ __device__ __forceinline__ void lu_flux(const double *q, const double *n, double
 // computed and stored for the backward sweep, then dQ* = (-R - sum_L ...) / D.  The last colour has
 // no higher neighbours, so its backward sweep is the identity: LAST also writes Qout.
 // Backward (!FWD): dQ = dQ* - sum_U ... / D, Qout = Q + dQ.
-template <int DIM, bool FWD, bool LAST, int MAXF>
+// RSCHEME >= 0 (forward, order 1, cells of <= 4 faces): R of the cell is not read but computed here,
+// from the states the sweep loads anyway: the same face solves (face_solve, the c0 outward normal,
+// the c0 side first) summed in slot order as the gather kernels sum them, so R is the same bit for bit.
+template <int DIM, bool FWD, bool LAST, int MAXF, int RSCHEME = -1>
 __global__ __launch_bounds__(BLOCK) void k_lu_sweep(const LuArgs a, int64_t t0, int64_t n) {
+    static_assert(RSCHEME < 0 || (FWD && MAXF <= 4), "inline residual: forward sweeps of small cells");
     constexpr int NV = DIM + 2;
     const int64_t l = block_id() * BLOCK + threadIdx.x;
     if (l >= n) return;
@@ -93,10 +101,12 @@ __global__ __launch_bounds__(BLOCK) void k_lu_sweep(const LuArgs a, int64_t t0, 
     const int64_t c = a.cells[t];
     double s[NV], qc[NV];
     if constexpr (FWD) {
-        ldq<NV>(a.R + c * NV, s);
-        ldq<NV>(a.Q + c * NV, qc);
+        if constexpr (RSCHEME < 0) {
+            ldq<NV>(a.R + c * NV, s);
 #pragma unroll
-        for (int k = 0; k < NV; k++) s[k] = -s[k];
+            for (int k = 0; k < NV; k++) s[k] = -s[k];
+        }
+        ldq<NV>(a.Q + c * NV, qc);
     } else {
 #pragma unroll
         for (int k = 0; k < NV; k++) s[k] = 0;
@@ -113,10 +123,17 @@ __global__ __launch_bounds__(BLOCK) void k_lu_sweep(const LuArgs a, int64_t t0, 
         // every memory read of the cell's slots is issued before the first is used: two rounds of
         // dependent loads per cell (records, then neighbour states and updates) instead of two per
         // slot.  The arithmetic below runs in the same order as the slot loop further down.
-        int32_t code[MAXF];
+        int32_t code[MAXF], fcode[MAXF], info[MAXF];
         double nrm[MAXF][DIM], q[MAXF][NV], dq[MAXF][NV], rr[MAXF];
 #pragma unroll
         for (int k = 0; k < MAXF; k++) code[k] = a.nb[(int64_t)k * a.nc + t];
+        if constexpr (RSCHEME >= 0) {
+#pragma unroll
+            for (int k = 0; k < MAXF; k++) {
+                fcode[k] = a.face[(int64_t)k * a.nc + t];
+                info[k] = fcode[k] >= 0 ? a.finfo[fcode[k] >> 1] : 0;
+            }
+        }
 #pragma unroll
         for (int k = 0; k < MAXF; k++) {
             const int64_t i = (int64_t)k * a.nc + t;
@@ -129,6 +146,28 @@ __global__ __launch_bounds__(BLOCK) void k_lu_sweep(const LuArgs a, int64_t t0, 
             if (FWD ? nbr : use) ldq<NV>(a.Q + o * NV, q[k]);
             if (use) ldq<NV>(a.dQ + o * NV, dq[k]);
             if (!FWD && use) rr[k] = a.r[i];
+        }
+        if constexpr (RSCHEME >= 0) {
+            double r[NV], cmp[NV];
+#pragma unroll
+            for (int j = 0; j < NV; j++) r[j] = cmp[j] = 0;
+#pragma unroll
+            for (int k = 0; k < MAXF; k++) {
+                if (code[k] == -2) break;
+                const bool c1side = fcode[k] & 1;
+                double n64[DIM], X[NV], Y[NV], g[NV];
+#pragma unroll
+                for (int d = 0; d < DIM; d++) n64[d] = c1side ? -nrm[k][d] : nrm[k][d];
+#pragma unroll
+                for (int j = 0; j < NV; j++) {
+                    X[j] = (code[k] >= 0 && c1side) ? q[k][j] : qc[j];
+                    Y[j] = code[k] >= 0 ? (c1side ? qc[j] : q[k][j]) : qc[j];
+                }
+                face_solve<DIM, RSCHEME, double>(a.P, RSCHEME == MST_SCHEME_ROE, info[k] & 15, info[k] >> 4, n64, X, Y, g);
+                accumulate<NV, false>(r, cmp, g, c1side);
+            }
+#pragma unroll
+            for (int j = 0; j < NV; j++) s[j] = -r[j];
         }
 #pragma unroll
         for (int k = 0; k < MAXF; k++) {

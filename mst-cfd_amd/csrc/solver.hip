@@ -101,12 +101,14 @@ struct mst_ctx {
     double *gw = nullptr;     // viscous: per cell Green-Gauss gradients of (u, T), [nc][dim + 1][dim]
     // LU-SGS steady driver (built on first use): cell colours, the cells of each colour in ascending
     // order (colour k: lu_cells[lu_start[k] .. lu_start[k + 1])), face spectral radii, the update
-    int32_t *lu_cells = nullptr, *lu_nb = nullptr;
+    int32_t *lu_cells = nullptr, *lu_nb = nullptr, *lu_face = nullptr;
     std::vector<int64_t> lu_start;
     std::vector<int32_t> lu_colour;  // per cell (host copy)
     double *lu_n = nullptr, *lu_r = nullptr, *lu_D = nullptr, *lu_dQ = nullptr;
     int steady = MST_STEADY_EXPLICIT;
     double lu_cfl = 1e4, lu_omega = 1.0;
+    bool lu_inline = false;  // the configuration lets the forward sweeps compute the residual (all but a
+                             // call's last iteration do)
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[MST_K_COUNT];
     std::vector<hipEvent_t> pool;
@@ -754,7 +756,7 @@ void release(mst_ctx *x) {
                     x->scal, x->part, x->fin_mid, x->bad, x->Q32[0], x->Q32[1], x->Q32[2], x->Q32[3], x->rec32,
                     x->frec0, x->frec1, x->inc_code, x->inc_w, x->inc_n, x->inc_dx, x->inc_w32, x->inc_n32,
                     x->inc_dx32, x->pstart, x->pface, x->ploc, x->hstart, x->hcell, x->hloc,
-                    x->lu_cells, x->lu_nb, x->lu_n, x->lu_r, x->lu_D, x->lu_dQ, x->gw};
+                    x->lu_cells, x->lu_nb, x->lu_face, x->lu_n, x->lu_r, x->lu_D, x->lu_dQ, x->gw};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (int k = 0; k < MST_K_COUNT; k++)
@@ -815,7 +817,7 @@ int build_colours(mst_ctx *x) {
     std::vector<int64_t> fill(start.begin(), start.end() - 1);
     for (int64_t c = 0; c < nc; c++) cells[fill[col[c]]++] = (int32_t)c;
     const int64_t slots = x->slots;
-    std::vector<int32_t> nb(slots * nc, -2);
+    std::vector<int32_t> nb(slots * nc, -2), fcode(slots * nc, -1);
     std::vector<double> nrm(slots * nc * dim, 0.0);
     for (int64_t t = 0; t < nc; t++) {
         const int64_t c = cells[t];
@@ -828,10 +830,12 @@ int build_colours(mst_ctx *x) {
             const int64_t i = (int64_t)s * nc + t;
             for (int d = 0; d < dim; d++) nrm[i * dim + d] = c1side ? -P.fn[(int64_t)f * dim + d] : P.fn[(int64_t)f * dim + d];
             nb[i] = o < 0 ? -1 : ((o << 2) | (col[o] < col[c] ? 1 : 0) | (col[o] > col[c] ? 2 : 0));
+            fcode[i] = e;
         }
     }
     int rc = dupload(&x->lu_cells, cells);
     if (rc == MST_OK) rc = dupload(&x->lu_nb, nb);
+    if (rc == MST_OK) rc = dupload(&x->lu_face, fcode);
     if (rc == MST_OK) rc = dupload(&x->lu_n, nrm);
     if (rc == MST_OK) rc = dalloc(&x->lu_r, (size_t)(slots * nc));
     if (rc == MST_OK) rc = dalloc(&x->lu_D, (size_t)nc);
@@ -843,14 +847,31 @@ int build_colours(mst_ctx *x) {
     return MST_OK;
 }
 
-template <int DIM, int MAXF>
-int lu_sweeps(mst_ctx *x, const LuArgs &a) {
+// forward launches; RS >= 0: the first-order residual computed by the sweep (scheme RS)
+template <int DIM, int MAXF, int RS>
+int lu_forward(mst_ctx *x, const LuArgs &a) {
     const int ncol = (int)x->lu_start.size() - 1;
     int rc = MST_OK;
     for (int k = 0; k < ncol && rc == MST_OK; k++) {
         const int64_t t0 = x->lu_start[k], n = x->lu_start[k + 1] - t0;
-        rc = k == ncol - 1 ? launch(x, MST_K_LUSGS, k_lu_sweep<DIM, true, true, MAXF>, n, a, t0, n)
-                           : launch(x, MST_K_LUSGS, k_lu_sweep<DIM, true, false, MAXF>, n, a, t0, n);
+        rc = k == ncol - 1 ? launch(x, MST_K_LUSGS, k_lu_sweep<DIM, true, true, MAXF, RS>, n, a, t0, n)
+                           : launch(x, MST_K_LUSGS, k_lu_sweep<DIM, true, false, MAXF, RS>, n, a, t0, n);
+    }
+    return rc;
+}
+
+template <int DIM, int MAXF>
+int lu_sweeps(mst_ctx *x, const LuArgs &a, bool inline_res) {
+    const int ncol = (int)x->lu_start.size() - 1;
+    int rc;
+    if constexpr (MAXF <= 4) {
+        if (inline_res)
+            rc = x->cfg.scheme == MST_SCHEME_ROE ? lu_forward<DIM, MAXF, MST_SCHEME_ROE>(x, a)
+                                                 : lu_forward<DIM, MAXF, MST_SCHEME_AUSM>(x, a);
+        else
+            rc = lu_forward<DIM, MAXF, -1>(x, a);
+    } else {
+        rc = lu_forward<DIM, MAXF, -1>(x, a);
     }
     for (int k = ncol - 2; k >= 0 && rc == MST_OK; k--) {
         const int64_t t0 = x->lu_start[k], n = x->lu_start[k + 1] - t0;
@@ -861,9 +882,15 @@ int lu_sweeps(mst_ctx *x, const LuArgs &a) {
 
 // one LU-SGS iteration: residual of Old (+ its L2 norms when l2 is set), the two sweeps into New
 int lu_iteration(mst_ctx *x, bool l2) {
-    int rc;
+    int rc = MST_OK;
     const int mode = l2 ? UPD_L2 : UPD_RESIDUAL;
-    if (x->patch_ok && !x->mixed && !x->viscous) {  // one launch: the patch kernel's residual-only form (same R)
+    // order 1, FP64, inviscid, cells of <= 4 faces: the forward sweeps compute the residual themselves
+    // (the norms of a call's last iteration still come from the residual kernel)
+    x->lu_inline = x->cfg.order == 1 && !x->mixed && !x->viscous && x->slots <= 4;
+    const bool inline_res = !l2 && x->lu_inline;
+    if (inline_res) {
+        // nothing to launch before the sweeps
+    } else if (x->patch_ok && !x->mixed && !x->viscous) {  // one launch: the patch kernel's residual-only form (same R)
         if (x->cfg.order >= 2) {
             rc = run_grad(x, x->iold, false);
             if (rc) return rc;
@@ -894,15 +921,18 @@ int lu_iteration(mst_ctx *x, bool l2) {
     a.gamma = x->cfg.gamma;
     a.omega = x->lu_omega;
     a.dscale = 1.0 / x->lu_cfl + 0.5 * x->lu_omega;
+    a.face = x->lu_face;
+    a.finfo = x->finfo;
+    a.P = x->p64;
     // the records are padded to the slot count of the device table (3, 4 or 8)
     if (x->dim == 2) {
-        if (x->slots == 3) return lu_sweeps<2, 3>(x, a);
-        if (x->slots == 4) return lu_sweeps<2, 4>(x, a);
-        return lu_sweeps<2, 8>(x, a);
+        if (x->slots == 3) return lu_sweeps<2, 3>(x, a, inline_res);
+        if (x->slots == 4) return lu_sweeps<2, 4>(x, a, inline_res);
+        return lu_sweeps<2, 8>(x, a, inline_res);
     }
-    if (x->slots == 3) return lu_sweeps<3, 3>(x, a);
-    if (x->slots == 4) return lu_sweeps<3, 4>(x, a);
-    return lu_sweeps<3, 8>(x, a);
+    if (x->slots == 3) return lu_sweeps<3, 3>(x, a, inline_res);
+    if (x->slots == 4) return lu_sweeps<3, 4>(x, a, inline_res);
+    return lu_sweeps<3, 8>(x, a, inline_res);
 }
 
 // AUTO: time one stage of each strategy on this mesh and configuration (outputs go to a spare state
@@ -1568,7 +1598,10 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
             // forward: positions, residual, states, neighbour updates, update out, D out; per slot the
             // record (neighbour code, normal) and r out.  backward: positions, D, states, updates in
             // and out, new state; per slot the record and r.
-            const double fwd = nc * (4 + 8 * NV * 4 + 8) + S * nc * (4 + 8 * D + 8);
+            double fwd = nc * (4 + 8 * NV * 4 + 8) + S * nc * (4 + 8 * D + 8);
+            // forward sweeps that compute the residual read each slot's face code and face info
+            // instead of the residual array
+            if (ctx->lu_inline) fwd += S * nc * 8 - nc * 8 * NV;
             const double bwd = nc * (4 + 8 + 8 * NV * 4) + S * nc * (4 + 8 * D + 8);
             *bytes = (fwd + bwd) / launches;
             break;

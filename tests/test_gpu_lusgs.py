@@ -78,6 +78,13 @@ def test_lusgs_iterations_match_oracle(name):
             assert err <= TOL, (name, cfl, it, err)
             rg = s.residual_norms()
             assert np.allclose(rg, res, rtol=1e-12, atol=0), (name, it, rg, res)
+        # one call of three iterations: only its last one reduces norms, and at order 1 the others
+        # compute R inside the forward sweep; the state must equal the one-by-one run bit for bit
+        Qg = s.get_state()
+        s.set_state(Q0)
+        s.lusgs_step(3)
+        assert np.array_equal(s.get_state(), Qg), name
+        assert np.allclose(s.residual_norms(), res, rtol=1e-12, atol=0)
     s.close()
 
 
