@@ -4,11 +4,13 @@ The device colouring is checked as integers (a proper colouring, the preprocessi
 runs the same colours, so the states after each iteration must agree to 1e-12 relative
 (the order of the cells inside a colour does not change the result, only the rounding of sums).
 """
+import os
+
 import numpy as np
 import pytest
 
 import mstcfd as M
-from helpers import cons, rel_state_error
+from helpers import cons, quad_grid, rel_state_error
 from mstcfd import ramp
 from oracle import pyoracle as O
 from test_oracle_lusgs import incidences
@@ -42,7 +44,20 @@ CASES = {
     "tet_roe2": (lambda: M.Mesh.gen_tet(5, 4, 4, (0, 0, 0), (1, 0.8, 0.7), 0.1, 5,
                                         (M.BC_INLET, M.BC_OUTLET, M.BC_SYMMETRY, M.BC_WALL, M.BC_SYMMETRY,
                                          M.BC_SYMMETRY)), M.ROE, 2, M.LIM_MINMOD),
+    # four-slot 2D cells (two colours)
+    "quad_roe1": (lambda: M.Mesh.from_raw(quad_grid(24, bc=M.BC_WALL, h=0.05, jitter=0.2, seed=4)), M.ROE, 1,
+                  M.LIM_NONE),
+    # an unstructured Fluent mesh of the reference (not bipartite: three colours after the Kempe pass)
+    "stair_un_ausm1": (lambda: M.Mesh.from_raw(fixture("2d-stair-un-5-tri")), M.AUSM, 1, M.LIM_NONE),
+    "stair_un_roe2": (lambda: M.Mesh.from_raw(fixture("2d-stair-un-5-tri")), M.ROE, 2, M.LIM_VANLEER),
 }
+
+
+def fixture(name):
+    d = dict(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", f"msh_{name}.npz")))
+    for k in ("dim", "n_cells", "n_int"):
+        d[k] = int(d[k])
+    return d
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
@@ -61,6 +76,8 @@ def test_lusgs_iterations_match_oracle(name):
 
     # colours: proper, and the preprocessing's colouring (checked on the CPU in test_oracle_lusgs)
     col, ncol = s.lusgs_colours()
+    if name.startswith("stair_un"):
+        assert ncol == 3
     inc = incidences(raw, om.geometry())
     assert col.min() == 0 and col.max() == ncol - 1
     assert all(col[o] != col[c] for c in range(nc) for (_, o, _) in inc[c] if o >= 0 and o != c)
