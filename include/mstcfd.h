@@ -311,6 +311,12 @@ int mst_batch_residual_norms(mst_batch *b, double *out);
 int mst_batch_dt(mst_batch *b, double *out);
 /* active[n_cases]: inactive cases keep their state and are left out of the work list */
 int mst_batch_set_active(mst_batch *b, const uint8_t *active);
+/* NEW: the LU-SGS steady driver over a whole batch (as mst_set_steady_driver / mst_lusgs_step for one
+ * context): every active case's colour-k cells in one launch per colour and sweep direction; each
+ * case's result equals the same case in its own context bit for bit.  FP64 batches of cells with at
+ * most 4 faces.  mst_batch_run_to_steady then runs LU-SGS iterations. */
+int mst_batch_set_steady_driver(mst_batch *b, int driver, double cfl, double omega);
+int mst_batch_lusgs_step(mst_batch *b, int n_iter);
 /* steady driver for every active case: a case that reaches max_f L2(R/V)_f / ref_f <= tol is frozen;
  * ref [n_cases*nv] in/out (a case with ref <= 0 takes its first step's norms), steps [n_cases] and
  * drop [n_cases] out.  The activity flags are restored at the end. */

@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "lusgs.hpp"
 #include "mstcfd.h"
 #include "prep.hpp"
 
@@ -389,6 +390,74 @@ __global__ __launch_bounds__(BLOCK) void kb_shadow(int64_t n, const double *__re
     if (i < n) Q32[i] = (float)Q[i];
 }
 
+// LU-SGS (implicit steady driver) over the whole batch: every active case's faces in one launch (the
+// case's scheme and parameters per face), then the colour sweeps of lusgs.hpp over the cases' merged
+// colour lists, gathering each cell's residual from these fluxes.
+template <int DIM, bool SECOND>
+__global__ __launch_bounds__(BLOCK) void kb_lu_flux(const FluxArgs<double> f0, const Params<double> *__restrict__ P,
+                                                    const int32_t *__restrict__ scheme,
+                                                    const int32_t *__restrict__ fcase,
+                                                    const uint8_t *__restrict__ active) {
+    constexpr int NV = DIM + 2;
+    const int64_t f = block_id() * BLOCK + threadIdx.x;
+    if (f >= f0.nf) return;
+    const int32_t cs = fcase[f];
+    if (!active[cs]) return;
+    FluxArgs<double> fa = f0;
+    fa.P = P[cs];
+    const int info = fa.finfo[f];
+    const int kind = info & 15, mask = info >> 4;
+    double g[NV];
+    if (kind >= mst::KIND_NONE) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) g[k] = 0;
+    } else if (scheme[cs] == MST_SCHEME_ROE) {
+        face_flux<DIM, MST_SCHEME_ROE, SECOND, double>(fa, f, kind, mask, g);
+    } else {
+        face_flux<DIM, MST_SCHEME_AUSM, SECOND, double>(fa, f, kind, mask, g);
+    }
+    stq<NV>(fa.G + f * NV, g);
+}
+
+// per active case (one block each): L2 over its cells of R/V, R gathered from G in slot order
+template <int NV, int MAXF>
+__global__ __launch_bounds__(BLOCK) void kb_lu_norms(int64_t NC, const int32_t *__restrict__ ell,
+                                                     const double *__restrict__ G, const double *__restrict__ vol,
+                                                     const int64_t *__restrict__ cstart,
+                                                     const uint8_t *__restrict__ active, double *__restrict__ l2) {
+    const int cs = blockIdx.x;
+    if (!active[cs]) return;
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) acc[k] = 0;
+    for (int64_t c = cstart[cs] + threadIdx.x; c < cstart[cs + 1]; c += BLOCK) {
+        double r[NV], cmp[NV];
+#pragma unroll
+        for (int k = 0; k < NV; k++) r[k] = cmp[k] = 0;
+#pragma unroll
+        for (int s = 0; s < MAXF; s++) {
+            const int32_t e = ell[(int64_t)s * NC + c];
+            if (e < 0) break;
+            double gf[NV];
+            ldq<NV>(G + (int64_t)(e >> 1) * NV, gf);
+            accumulate<NV, false>(r, cmp, gf, e & 1);
+        }
+        const double iV = frcp(vol[c]);
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            const double x = r[k] * iV;
+            acc[k] += x * x;
+        }
+    }
+    __shared__ double sh[NV];
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = acc[k];
+    block_partial<NV, false>(v, sh, 1);
+    __syncthreads();
+    if ((int)threadIdx.x < NV) l2[cs * NV + threadIdx.x] = sqrt(sh[threadIdx.x] / (double)(cstart[cs + 1] - cstart[cs]));
+}
+
 int bucket_of(int64_t nc) {
     if (nc < 2048) return 0;
     if (nc < 8192) return 1;
@@ -448,6 +517,15 @@ struct mst_batch {
     bool use_graph = true;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     bool graphs_valid = false;
+    // LU-SGS steady driver (built on first use): positions sorted by colour across the cases, per
+    // (slot, position) records, per position its case, per face its case, device active flags
+    int steady = MST_STEADY_EXPLICIT;
+    double lu_cfl = 1e4, lu_omega = 1.0;
+    int32_t *lu_cells = nullptr, *lu_nb = nullptr, *lu_face = nullptr, *lu_pcase = nullptr, *lu_fcase = nullptr;
+    double *lu_n = nullptr, *lu_r = nullptr, *lu_D = nullptr, *lu_dQ = nullptr;
+    int64_t *lu_cstart = nullptr;
+    uint8_t *lu_active = nullptr;
+    std::vector<int64_t> lu_start;
 };
 
 namespace {
@@ -483,7 +561,9 @@ void release(mst_batch *b) {
                     b->frec0, b->frec1, b->rec32, b->P64, b->P32, b->scheme, b->dts, b->gam,
                     b->cfl, b->l2, b->ncells, b->citems, b->fitems, b->ci0, b->ci1, b->part, b->cnt, b->bad,
                     b->inc_code, b->inc_w, b->inc_n, b->inc_dx, b->inc_w32, b->inc_n32, b->inc_dx32,
-                    b->pstart, b->pface, b->ploc, b->hstart, b->hcell, b->hloc};
+                    b->pstart, b->pface, b->ploc, b->hstart, b->hcell, b->hloc, b->lu_cells, b->lu_nb,
+                    b->lu_face, b->lu_pcase, b->lu_fcase, b->lu_n, b->lu_r, b->lu_D, b->lu_dQ, b->lu_cstart,
+                    b->lu_active};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -962,6 +1042,150 @@ int build_graphs(mst_batch *b) {
     return MST_OK;
 }
 
+// LU-SGS tables of the batch: each case's colouring (prep.cpp, lusgs_colours), the positions of
+// colour k = every case's colour-k cells (case by case, ascending); records as the single context's
+int build_lu(mst_batch *b) {
+    if (b->lu_cells) return MST_OK;
+    const int64_t NC = b->NC, S = b->slots;
+    const int dim = b->dim;
+    if (NC >= ((int64_t)1 << 29)) return mst::fail(MST_E_UNSUPPORTED, "LU-SGS records hold cell indices below 2^29");
+    std::vector<int32_t> col(NC);
+    int ncol = 0;
+    for (int i = 0; i < b->n; i++) {
+        std::vector<int32_t> ci;
+        ncol = std::max(ncol, mst::lusgs_colours(b->preps[i], ci));
+        for (size_t c = 0; c < ci.size(); c++) col[b->cstart[i] + c] = ci[c];
+    }
+    std::vector<int64_t> start(ncol + 1, 0);
+    for (int64_t c = 0; c < NC; c++) start[col[c] + 1]++;
+    for (int k = 0; k < ncol; k++) start[k + 1] += start[k];
+    std::vector<int32_t> cells(NC), pcase(NC), fcase(b->NF);
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    for (int i = 0; i < b->n; i++)
+        for (int64_t c = b->cstart[i]; c < b->cstart[i + 1]; c++) {
+            const int64_t t = fill[col[c]]++;
+            cells[t] = (int32_t)c;
+            pcase[t] = i;
+        }
+    for (int i = 0; i < b->n; i++)
+        for (int64_t f = b->fstart[i]; f < b->fstart[i + 1]; f++) fcase[f] = i;
+    std::vector<int32_t> nb(S * NC, -2), fcode(S * NC, -1);
+    std::vector<double> nrm(S * NC * dim, 0.0);
+    for (int64_t t = 0; t < NC; t++) {
+        const int i = pcase[t];
+        const mst::Prepared &P = b->preps[i];
+        const int64_t c0 = b->cstart[i], f0 = b->fstart[i];
+        const int64_t lc = cells[t] - c0;
+        for (int s = 0; s < P.maxf; s++) {
+            const int32_t e = P.ell[(int64_t)s * P.nc + lc];
+            if (e < 0) break;
+            const int32_t f = e >> 1;
+            const bool c1side = e & 1;
+            const int32_t lo = c1side ? P.fc0[f] : P.fc1[f];
+            const int64_t idx = (int64_t)s * NC + t;
+            for (int d = 0; d < dim; d++)
+                nrm[idx * dim + d] = c1side ? -P.fn[(int64_t)f * dim + d] : P.fn[(int64_t)f * dim + d];
+            if (lo < 0) {
+                nb[idx] = -1;
+            } else {
+                const int64_t o = c0 + lo;
+                nb[idx] = (int32_t)((o << 2) | (col[o] < col[cells[t]] ? 1 : 0) | (col[o] > col[cells[t]] ? 2 : 0));
+            }
+            fcode[idx] = (int32_t)(((f0 + f) << 1) | (c1side ? 1 : 0));
+        }
+    }
+    std::vector<int64_t> cst(b->cstart.begin(), b->cstart.end());
+    int rc = dupload(&b->lu_cells, cells);
+    if (rc == MST_OK) rc = dupload(&b->lu_nb, nb);
+    if (rc == MST_OK) rc = dupload(&b->lu_face, fcode);
+    if (rc == MST_OK) rc = dupload(&b->lu_pcase, pcase);
+    if (rc == MST_OK) rc = dupload(&b->lu_fcase, fcase);
+    if (rc == MST_OK) rc = dupload(&b->lu_n, nrm);
+    if (rc == MST_OK) rc = dupload(&b->lu_cstart, cst);
+    if (rc == MST_OK) rc = dalloc(&b->lu_r, (size_t)(S * NC));
+    if (rc == MST_OK) rc = dalloc(&b->lu_D, (size_t)NC);
+    if (rc == MST_OK) rc = dalloc(&b->lu_dQ, (size_t)(NC * b->nv));
+    if (rc == MST_OK) rc = dalloc(&b->lu_active, (size_t)b->n);
+    if (rc) return rc;
+    b->lu_start = start;
+    return MST_OK;
+}
+
+template <int DIM, int MAXF>
+int lu_sweeps_b(mst_batch *b, const LuArgs &a) {
+    const int ncol = (int)b->lu_start.size() - 1;
+    for (int k = 0; k < ncol; k++) {
+        const int64_t t0 = b->lu_start[k], n = b->lu_start[k + 1] - t0;
+        if (n <= 0) continue;
+        const uint32_t grid = (uint32_t)((n + BLOCK - 1) / BLOCK);
+        if (k == ncol - 1)
+            hipLaunchKernelGGL((k_lu_sweep<DIM, true, true, MAXF, LU_R_GATHER>), dim3(grid), dim3(BLOCK), 0, b->stream, a, t0, n);
+        else
+            hipLaunchKernelGGL((k_lu_sweep<DIM, true, false, MAXF, LU_R_GATHER>), dim3(grid), dim3(BLOCK), 0, b->stream, a, t0, n);
+        HIPCHK(hipGetLastError());
+    }
+    for (int k = ncol - 2; k >= 0; k--) {
+        const int64_t t0 = b->lu_start[k], n = b->lu_start[k + 1] - t0;
+        if (n <= 0) continue;
+        const uint32_t grid = (uint32_t)((n + BLOCK - 1) / BLOCK);
+        hipLaunchKernelGGL((k_lu_sweep<DIM, false, false, MAXF>), dim3(grid), dim3(BLOCK), 0, b->stream, a, t0, n);
+        HIPCHK(hipGetLastError());
+    }
+    return MST_OK;
+}
+
+// one LU-SGS iteration of every active case: gradients (order 2), fluxes, sweeps into New, swap
+template <int DIM>
+int lu_iteration_b(mst_batch *b, bool l2) {
+    constexpr int NV = DIM + 2;
+    const int iold = b->iold, inew = b->inew;
+    if (b->order >= 2) {
+        HIPCHK(hipMemsetAsync(b->cnt, 0, sizeof(int) * NCNT, b->stream));  // direct launches, no graph
+        int rc = run_grad<DIM, double, BLOCK>(b, iold);
+        if (rc) return rc;
+    }
+    BArgs<double> ba = base_args<double>(b, iold);
+    const uint32_t fgrid = (uint32_t)((b->NF + BLOCK - 1) / BLOCK);
+    if (b->order >= 2)
+        hipLaunchKernelGGL((kb_lu_flux<DIM, true>), dim3(fgrid), dim3(BLOCK), 0, b->stream, ba.f, (const Params<double> *)b->P64,
+                           (const int32_t *)b->scheme, (const int32_t *)b->lu_fcase, (const uint8_t *)b->lu_active);
+    else
+        hipLaunchKernelGGL((kb_lu_flux<DIM, false>), dim3(fgrid), dim3(BLOCK), 0, b->stream, ba.f, (const Params<double> *)b->P64,
+                           (const int32_t *)b->scheme, (const int32_t *)b->lu_fcase, (const uint8_t *)b->lu_active);
+    HIPCHK(hipGetLastError());
+    if (l2) {
+        if (b->slots == 3)
+            hipLaunchKernelGGL((kb_lu_norms<NV, 3>), dim3(b->n), dim3(BLOCK), 0, b->stream, b->NC, (const int32_t *)b->ell,
+                               (const double *)b->G, (const double *)b->cvol, (const int64_t *)b->lu_cstart,
+                               (const uint8_t *)b->lu_active, b->l2);
+        else
+            hipLaunchKernelGGL((kb_lu_norms<NV, 4>), dim3(b->n), dim3(BLOCK), 0, b->stream, b->NC, (const int32_t *)b->ell,
+                               (const double *)b->G, (const double *)b->cvol, (const int64_t *)b->lu_cstart,
+                               (const uint8_t *)b->lu_active, b->l2);
+        HIPCHK(hipGetLastError());
+    }
+    LuArgs a;
+    memset(&a, 0, sizeof(a));
+    a.nc = b->NC;
+    a.cells = b->lu_cells;
+    a.nb = b->lu_nb;
+    a.n = b->lu_n;
+    a.r = b->lu_r;
+    a.D = b->lu_D;
+    a.Q = b->Q[iold];
+    a.dQ = b->lu_dQ;
+    a.Qout = b->Q[inew];
+    a.bad = b->bad;
+    a.dscale = 1.0 / b->lu_cfl + 0.5 * b->lu_omega;
+    a.omega = b->lu_omega;
+    a.face = b->lu_face;
+    a.pcase = b->lu_pcase;
+    a.gam = b->gam;
+    a.active = b->lu_active;
+    a.G = b->G;
+    return b->slots == 3 ? lu_sweeps_b<DIM, 3>(b, a) : lu_sweeps_b<DIM, 4>(b, a);
+}
+
 int check_bad(mst_batch *b) {
     std::vector<int> h(b->n, 0);
     HIPCHK(hipStreamSynchronize(b->stream));
@@ -1415,6 +1639,40 @@ int mst_batch_set_active(mst_batch *b, const uint8_t *active) {
     return build_items(b);
 }
 
+int mst_batch_set_steady_driver(mst_batch *b, int driver, double cfl, double omega) {
+    B_GUARD(b);
+    if (driver != MST_STEADY_EXPLICIT && driver != MST_STEADY_LUSGS)
+        return mst::fail(MST_E_ARG, "driver must be MST_STEADY_EXPLICIT or MST_STEADY_LUSGS");
+    if (driver == MST_STEADY_LUSGS) {
+        if (!(cfl > 0) || !(omega > 0) || !std::isfinite(cfl))
+            return mst::fail(MST_E_ARG, "LU-SGS needs a finite cfl > 0 and omega > 0");
+        if (b->mixed) return mst::fail(MST_E_UNSUPPORTED, "batched LU-SGS runs FP64 batches");
+        if (b->slots > 4) return mst::fail(MST_E_UNSUPPORTED, "batched LU-SGS covers cells of at most 4 faces");
+        b->lu_cfl = cfl;
+        b->lu_omega = omega;
+    }
+    b->steady = driver;
+    return MST_OK;
+}
+
+int mst_batch_lusgs_step(mst_batch *b, int n_iter) {
+    B_GUARD(b);
+    if (n_iter < 0) return mst::fail(MST_E_ARG, "n_iter must be >= 0");
+    if (b->steady != MST_STEADY_LUSGS) return mst::fail(MST_E_ARG, "select the LU-SGS driver first (mst_batch_set_steady_driver)");
+    int rc = build_lu(b);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(b->lu_active, b->active.data(), b->n, hipMemcpyHostToDevice));
+    for (int it = 0; it < n_iter; it++) {
+        rc = b->dim == 2 ? lu_iteration_b<2>(b, it == n_iter - 1) : lu_iteration_b<3>(b, it == n_iter - 1);
+        if (rc) return rc;
+        b->shadow_ok[b->inew] = false;
+        std::swap(b->iold, b->inew);
+    }
+    if (n_iter > 0) b->dt_valid = false;
+    return check_bad(b);
+}
+
 int mst_batch_run_to_steady(mst_batch *b, double tol, int max_steps, int check_every, double *ref, int32_t *steps,
                             double *drop) {
     B_GUARD(b);
@@ -1433,8 +1691,9 @@ int mst_batch_run_to_steady(mst_batch *b, double tol, int max_steps, int check_e
     // cases without a reference take the residual of their first step
     bool need_ref = false;
     for (int i = 0; i < n; i++) need_ref |= act[i] && !(ref[(size_t)i * nv] > 0);
+    const bool lu = b->steady == MST_STEADY_LUSGS;
     if (need_ref) {
-        rc = mst_batch_step(b, 1);
+        rc = lu ? mst_batch_lusgs_step(b, 1) : mst_batch_step(b, 1);
         if (!rc) rc = mst_batch_residual_norms(b, r.data());
         if (rc) return rc;
         for (int i = 0; i < n; i++) {
@@ -1450,7 +1709,7 @@ int mst_batch_run_to_steady(mst_batch *b, double tol, int max_steps, int check_e
         for (int i = 0; i < n; i++) nact += act[i];
         if (!nact) break;
         const int k = std::min(check_every, max_steps - done);
-        rc = mst_batch_step(b, k);
+        rc = lu ? mst_batch_lusgs_step(b, k) : mst_batch_step(b, k);
         if (rc) break;
         done += k;
         rc = mst_batch_residual_norms(b, r.data());

@@ -45,7 +45,17 @@ struct LuArgs {
     const int32_t *__restrict__ face;    // (face << 1) | is_c1 per (slot, position)
     const int32_t *__restrict__ finfo;   // per face: kind | (L/R mask << 4)
     Params<double> P;
+    // batches (pcase set): the case of each position, per case gamma and active flag (inactive cases
+    // are skipped; the bad flag is per case); the residual is gathered from the face fluxes G
+    const int32_t *__restrict__ pcase;
+    const double *__restrict__ gam;
+    const uint8_t *__restrict__ active;
+    const double *__restrict__ G;
 };
+
+// residual sources of the forward sweep
+constexpr int LU_R_ARRAY = -1;   // read R
+constexpr int LU_R_GATHER = -2;  // sum +-G_f of the cell's faces in slot order (the gather kernels' order)
 
 template <int DIM>
 __device__ __forceinline__ double lu_pressure(const double *q, double gamma) {
@@ -88,20 +98,27 @@ __device__ __forceinline__ void lu_flux(const double *q, const double *n, double
 // computed and stored for the backward sweep, then dQ* = (-R - sum_L ...) / D.  The last colour has
 // no higher neighbours, so its backward sweep is the identity: LAST also writes Qout.
 // Backward (!FWD): dQ = dQ* - sum_U ... / D, Qout = Q + dQ.
+// RSCHEME = LU_R_GATHER (forward, cells of <= 4 faces): R is gathered from the face fluxes G.
 // RSCHEME >= 0 (forward, order 1, cells of <= 4 faces): R of the cell is not read but computed here,
 // from the states the sweep loads anyway: the same face solves (face_solve, the c0 outward normal,
 // the c0 side first) summed in slot order as the gather kernels sum them, so R is the same bit for bit.
-template <int DIM, bool FWD, bool LAST, int MAXF, int RSCHEME = -1>
-__global__ __launch_bounds__(BLOCK) void k_lu_sweep(const LuArgs a, int64_t t0, int64_t n) {
-    static_assert(RSCHEME < 0 || (FWD && MAXF <= 4), "inline residual: forward sweeps of small cells");
+template <int DIM, bool FWD, bool LAST, int MAXF, int RSCHEME = LU_R_ARRAY>
+__global__ __launch_bounds__(BLOCK) void k_lu_sweep(LuArgs a, int64_t t0, int64_t n) {
+    static_assert(RSCHEME == LU_R_ARRAY || (FWD && MAXF <= 4), "computed residual: forward sweeps of small cells");
     constexpr int NV = DIM + 2;
     const int64_t l = block_id() * BLOCK + threadIdx.x;
     if (l >= n) return;
     const int64_t t = t0 + l;
+    if (a.pcase) {  // batch: this position's case
+        const int32_t cs = a.pcase[t];
+        if (!a.active[cs]) return;
+        a.gamma = a.gam[cs];
+        a.bad += cs;
+    }
     const int64_t c = a.cells[t];
     double s[NV], qc[NV];
     if constexpr (FWD) {
-        if constexpr (RSCHEME < 0) {
+        if constexpr (RSCHEME == LU_R_ARRAY) {
             ldq<NV>(a.R + c * NV, s);
 #pragma unroll
             for (int k = 0; k < NV; k++) s[k] = -s[k];
@@ -127,12 +144,18 @@ __global__ __launch_bounds__(BLOCK) void k_lu_sweep(const LuArgs a, int64_t t0, 
         double nrm[MAXF][DIM], q[MAXF][NV], dq[MAXF][NV], rr[MAXF];
 #pragma unroll
         for (int k = 0; k < MAXF; k++) code[k] = a.nb[(int64_t)k * a.nc + t];
-        if constexpr (RSCHEME >= 0) {
+        if constexpr (RSCHEME != LU_R_ARRAY) {
 #pragma unroll
             for (int k = 0; k < MAXF; k++) {
                 fcode[k] = a.face[(int64_t)k * a.nc + t];
-                info[k] = fcode[k] >= 0 ? a.finfo[fcode[k] >> 1] : 0;
+                if constexpr (RSCHEME >= 0) info[k] = fcode[k] >= 0 ? a.finfo[fcode[k] >> 1] : 0;
             }
+        }
+        double gsl[MAXF][NV];  // gathered face fluxes (RSCHEME == LU_R_GATHER)
+        if constexpr (RSCHEME == LU_R_GATHER) {
+#pragma unroll
+            for (int k = 0; k < MAXF; k++)
+                if (fcode[k] >= 0) ldq<NV>(a.G + (int64_t)(fcode[k] >> 1) * NV, gsl[k]);
         }
 #pragma unroll
         for (int k = 0; k < MAXF; k++) {
@@ -146,6 +169,18 @@ __global__ __launch_bounds__(BLOCK) void k_lu_sweep(const LuArgs a, int64_t t0, 
             if (FWD ? nbr : use) ldq<NV>(a.Q + o * NV, q[k]);
             if (use) ldq<NV>(a.dQ + o * NV, dq[k]);
             if (!FWD && use) rr[k] = a.r[i];
+        }
+        if constexpr (RSCHEME == LU_R_GATHER) {
+            double r[NV], cmp[NV];
+#pragma unroll
+            for (int j = 0; j < NV; j++) r[j] = cmp[j] = 0;
+#pragma unroll
+            for (int k = 0; k < MAXF; k++) {
+                if (code[k] == -2) break;
+                accumulate<NV, false>(r, cmp, gsl[k], fcode[k] & 1);
+            }
+#pragma unroll
+            for (int j = 0; j < NV; j++) s[j] = -r[j];
         }
         if constexpr (RSCHEME >= 0) {
             double r[NV], cmp[NV];

@@ -907,6 +907,7 @@ int lu_iteration(mst_ctx *x, bool l2) {
         if (rc) return rc;
     }
     LuArgs a;
+    memset(&a, 0, sizeof(a));  // the batch-only fields (pcase, gam, active, G) stay null: single context
     a.nc = x->nc;
     a.cells = x->lu_cells;
     a.nb = x->lu_nb;

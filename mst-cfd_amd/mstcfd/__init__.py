@@ -52,7 +52,8 @@ EXPORTS = [
     "mst_mlp_gradient", "mst_ctx_warm_start", "mst_run_to_steady", "mst_batch_run_to_steady",
     "mst_mesh_cell_centres", "mst_batch_set_strategy", "mst_batch_get_strategy", "mst_set_cell_order",
     "mst_get_cell_order", "mst_set_steady_driver", "mst_lusgs_step", "mst_lusgs_colours", "mst_prep_cell_colours",
-    "mst_write_tecplot", "mst_checkpoint_write", "mst_checkpoint_read",
+    "mst_write_tecplot", "mst_checkpoint_write", "mst_checkpoint_read", "mst_batch_set_steady_driver",
+    "mst_batch_lusgs_step",
 ]
 
 
@@ -207,6 +208,8 @@ def lib():
         "mst_write_tecplot": (C.c_int, [vp, F64P, C.c_int64, C.c_double, C.c_double, C.c_int, C.c_char_p,
                                         C.c_char_p]),
         "mst_checkpoint_write": (C.c_int, [C.c_char_p, vp, F64P, C.c_int64, C.c_int64, C.c_double]),
+        "mst_batch_set_steady_driver": (C.c_int, [vp, C.c_int, C.c_double, C.c_double]),
+        "mst_batch_lusgs_step": (C.c_int, [vp, C.c_int]),
         "mst_checkpoint_read": (C.c_int, [C.c_char_p, vp, F64P, C.c_int64, C.POINTER(C.c_int64),
                                           C.POINTER(C.c_double)]),
     }
@@ -616,6 +619,15 @@ class Batch:
     def set_active(self, active):
         a = np.ascontiguousarray(np.asarray(active, bool).astype(np.uint8))
         _check(lib().mst_batch_set_active(self.h, a), "mst_batch_set_active")
+
+    def set_steady_driver(self, driver, cfl=1e4, omega=1.0):
+        """STEADY_EXPLICIT or STEADY_LUSGS (FP64 batches) for run_to_steady."""
+        _check(lib().mst_batch_set_steady_driver(self.h, int(driver), float(cfl), float(omega)),
+               "mst_batch_set_steady_driver")
+
+    def lusgs_step(self, n=1):
+        """n LU-SGS iterations of every active case."""
+        _check(lib().mst_batch_lusgs_step(self.h, int(n)), "mst_batch_lusgs_step")
 
     def run_to_steady(self, tol=1e-8, max_steps=60000, check_every=500, ref=None):
         """Per case: steps taken, final ratio, reference norms (converged cases are frozen)."""

@@ -202,3 +202,71 @@ def test_batch_multi_item_case_with_stage_dip_matches_oracle():
     err = np.abs(qs - Qr).max() / np.abs(Qr).max()
     print(f"47k-cell tube: batch == single, vs oracle {err:.2e}", flush=True)
     assert err <= 1e-12
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_batch_lusgs_equals_single_contexts(order):
+    """LU-SGS over a batch: each case equals the same case in its own context, bit for bit (the same
+    colouring, the same residual, the same sweeps), including a frozen case that stays untouched."""
+    meshes = [M.Mesh.gen_tri(30, 4, 0, 1, 0, 0.1, 0.1, 3, (M.BC_WALL,) * 4),
+              M.Mesh.gen_tri(12, 9, 0, 2, 0, 1, 0.15, 5, (M.BC_INLET, M.BC_OUTLET, M.BC_WALL, M.BC_SYMMETRY)),
+              M.Mesh.gen_tri(20, 20, -1, 1, -1, 1, 0.1, 7, (M.BC_INLET, M.BC_OUTLET, M.BC_SYMMETRY, M.BC_WALL))]
+    lim = M.LIM_VANLEER if order == 2 else M.LIM_NONE
+    qin = [1.2, 0.7, 0.0, 2.8]
+    cfgs = [M.make_config(scheme=M.ROE if i != 1 else M.AUSM, order=order, limiter=lim, rk=2, cfl=0.5, q_inlet=qin)
+            for i in range(3)]
+    rng = np.random.default_rng(order)
+    states = []
+    for m in meshes:
+        n = m.n_cells
+        rho, p = rng.uniform(0.85, 1.2, n), rng.uniform(0.85, 1.2, n)
+        vel = rng.uniform(-0.3, 0.3, (n, 2))
+        states.append(np.column_stack([rho, rho[:, None] * vel, p / 0.4 + 0.5 * rho * (vel ** 2).sum(1)]))
+    b = M.Batch(meshes, cfgs)
+    for i, q in enumerate(states):
+        b.set_state(i, q)
+    b.set_steady_driver(M.STEADY_LUSGS, 20.0, 1.0)
+    b.set_active(np.array([1, 0, 1], np.uint8))
+    b.lusgs_step(4)
+    for i, (m, c, q) in enumerate(zip(meshes, cfgs, states)):
+        got = b.get_state(i)
+        if i == 1:
+            assert np.array_equal(got, q)  # frozen
+            continue
+        s = M.Solver(m, c)
+        s.set_state(q)
+        s.set_steady_driver(M.STEADY_LUSGS, 20.0, 1.0)
+        s.lusgs_step(4)
+        assert np.array_equal(got, s.get_state()), i
+        s.close()
+    b.close()
+
+
+def test_batch_lusgs_runs_ramps_to_steady():
+    from mstcfd import ramp
+    cases = [(2.5, 10.0), (2.9, 15.0), (2.2, 8.0)]
+    meshes = [ramp.mesh(th, 40, 24) for (_, th) in cases]
+    b = M.Batch(meshes, [ramp.config(m, mach) for m, (mach, _) in zip(meshes, cases)])
+    for i, (m, (mach, _)) in enumerate(zip(meshes, cases)):
+        b.set_state(i, ramp.freestream(m, mach))
+    b.set_steady_driver(M.STEADY_LUSGS, 1e4, 1.0)
+    steps, drop, _ = b.run_to_steady(1e-8, 5000, 50)
+    print("batch LU-SGS ramps:", steps, drop, flush=True)
+    assert np.all(drop <= 1e-8)
+    for i, (m, (mach, _)) in enumerate(zip(meshes, cases)):
+        s = M.Solver(m, ramp.config(m, mach))
+        s.set_state(ramp.freestream(m, mach))
+        s.set_steady_driver(M.STEADY_LUSGS, 1e4, 1.0)
+        n, d, _ = s.run_to_steady(1e-8, 5000, 50)
+        assert n == steps[i]
+        assert np.array_equal(s.get_state(), b.get_state(i))
+        s.close()
+    b.close()
+
+
+def test_batch_lusgs_refuses_mixed():
+    m = M.Mesh.gen_tri(8, 4, 0, 1, 0, 0.1, 0.1, 1, (M.BC_WALL,) * 4)
+    b = M.Batch([m], [M.make_config(cfl=0.5, precision=M.PREC_MIXED)])
+    with pytest.raises(M.MstError, match="FP64"):
+        b.set_steady_driver(M.STEADY_LUSGS, 10.0, 1.0)
+    b.close()
