@@ -62,25 +62,43 @@ def features(m, mach, theta):
     return X
 
 
-def generate(cases, nx, ny, tol=1e-8, max_steps=20000, check_every=500, autotune_cache=None):
-    """Steady solutions of the cases, solved together in one batch; returns (X, T, info)."""
+def generate(cases, nx, ny, tol=1e-8, max_steps=20000, check_every=500, autotune_cache=None, driver="lusgs",
+             lu_max=4000, lu_check=50):
+    """Steady solutions of the cases, solved together in one batch; returns (X, T, info).
+    driver "lusgs": the batched LU-SGS driver (CFL 1e4), falling back to the explicit one if a case
+    leaves the admissible states; "explicit": SSP-RK2 with the CFL time step."""
     t0 = time.perf_counter()
     meshes = [mesh(th, nx, ny) for (_, th) in cases]
     b = Batch(meshes, [config(m, M) for m, (M, _) in zip(meshes, cases)])
-    for i, (m, (M, _)) in enumerate(zip(meshes, cases)):
-        b.set_state(i, freestream(m, M))
+
+    def reset():
+        for i, (m, (M, _)) in enumerate(zip(meshes, cases)):
+            b.set_state(i, freestream(m, M))
+
+    reset()
     t_setup = time.perf_counter() - t0
     if autotune_cache:
         b.autotune(autotune_cache, steps=3)
     t1 = time.perf_counter()
-    steps, drop, _ = b.run_to_steady(tol, max_steps, check_every)
+    used = driver
+    if driver == "lusgs":
+        try:
+            b.set_steady_driver(STEADY_LUSGS, 1e4, 1.0)
+            steps, drop, _ = b.run_to_steady(tol, lu_max, lu_check)
+        except MstError:
+            used = "explicit (LU-SGS left the admissible states)"
+            b.set_steady_driver(STEADY_EXPLICIT)
+            reset()
+            steps, drop, _ = b.run_to_steady(tol, max_steps, check_every)
+    else:
+        steps, drop, _ = b.run_to_steady(tol, max_steps, check_every)
     t_solve = time.perf_counter() - t1
     X = np.concatenate([features(m, M, th) for m, (M, th) in zip(meshes, cases)])
     T = np.concatenate([encode(b.get_state(i), M) for i, (M, _) in enumerate(cases)])
     b.close()
     info = dict(cases=len(cases), cells_per_case=meshes[0].n_cells, samples=int(X.shape[0]), setup_s=t_setup,
-                solve_s=t_solve, converged=int((drop <= tol).sum()), max_steps=max_steps,
-                median_steps=float(np.median(steps)))
+                solve_s=t_solve, converged=int((drop <= tol).sum()), driver=used,
+                max_steps=lu_max if used == "lusgs" else max_steps, median_steps=float(np.median(steps)))
     return X, T, info
 
 
