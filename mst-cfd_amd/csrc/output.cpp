@@ -22,6 +22,7 @@
 //   header  "MSTCKPT1", version, dim, nv, n_cells, n_faces, n_nodes, mesh fingerprint, step, time,
 //           state checksum (FNV-1a 64 of the state's bytes)
 //   data    n_cells x nv doubles, the state in file order
+// in the host's byte order (little-endian on x86-64).
 // The fingerprint is FNV-1a 64 over the mesh's counts, node coordinates and face tables.
 #include <cmath>
 #include <cstdio>
@@ -128,7 +129,6 @@ int write_tecplot(const mst_mesh &m, const double *Q, double gamma, double cv, i
         if (m.c1[f] >= 0) cfaces[m.c1[f]].push_back((int32_t)f);
     }
     std::vector<int32_t> conn((size_t)m.n_cells * 4);
-    int maxn = 0;
     bool all_tri = true;
     for (int64_t c = 0; c < m.n_cells; c++) {
         int32_t nodes[4];
@@ -137,7 +137,6 @@ int write_tecplot(const mst_mesh &m, const double *Q, double gamma, double cv, i
             return fail(MST_E_UNSUPPORTED, "Tecplot output covers triangles, quadrilaterals and tetrahedra only (cell " +
                                                std::to_string(c) + ")");
         for (int k = 0; k < 4; k++) conn[(size_t)c * 4 + k] = nodes[k < n ? k : n - 1];
-        maxn = std::max(maxn, n);
         all_tri &= n == 3;
     }
     // face values (Work.cpp:246-285)
@@ -201,7 +200,6 @@ int write_tecplot(const mst_mesh &m, const double *Q, double gamma, double cv, i
     }
     const bool ok = !ferror(fp);
     if (fclose(fp) != 0 || !ok) return fail(MST_E_IO, "write failed: " + path);
-    (void)maxn;
     return MST_OK;
 }
 

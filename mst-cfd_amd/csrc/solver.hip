@@ -1544,7 +1544,6 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
     CTX_GUARD(ctx);
     if (!bytes) return mst::fail(MST_E_ARG, "null argument");
     const double nc = (double)ctx->nc, nf = (double)ctx->nf, D = ctx->dim, NV = ctx->nv;
-    const double np = (double)ctx->prep.P.pcount;
     const bool mixed = ctx->mixed, second = ctx->cfg.order >= 2;
     const double r = mixed ? 4 : 8;
     const double ell = 4.0 * ctx->maxf * nc;
