@@ -176,6 +176,11 @@ int mst_prep_cell_colours(const mst_prep *p, int32_t *colour, int *n_colours);
  * by node number), values on symmetry and periodic faces, a well-formed TITLE and a 3D w column. */
 int mst_write_tecplot(const mst_mesh *m, const double *Q, int64_t n_cells, double gamma, double cv, int wall_noslip,
                       const char *title, const char *path);
+/* NEW: binary mesh cache (MshBlock::readMsh's result in file order, zones included), so a large
+ * Fluent mesh is parsed once: written through a temporary file; read back bit for bit, checksummed
+ * and validated like the reader's output (MST_E_IO for a damaged or foreign file). */
+int mst_mesh_write_cache(const mst_mesh *m, const char *path);
+int mst_mesh_read_cache(const char *path, mst_mesh **out);
 /* Binary checkpoint replacing the text .cellPUVT dump (Work::writedataCellContinue /
  * readPbasedCellPlt, work/Work.cpp:432-514): the state (n_cells x nv doubles, file order) with a step
  * counter and a time, a mesh fingerprint and a state checksum.  Reading restores the state bit for

@@ -50,6 +50,26 @@ int mst_mesh_write_msh(const mst_mesh *m, const char *path) {
     MST_GUARD(return mst::write_msh(*m, path));
 }
 
+int mst_mesh_write_cache(const mst_mesh *m, const char *path) {
+    if (!m || !path) return mst::fail(MST_E_ARG, "null argument");
+    MST_GUARD(return mst::write_mesh_cache(*m, path));
+}
+
+int mst_mesh_read_cache(const char *path, mst_mesh **out) {
+    if (!path || !out) return mst::fail(MST_E_ARG, "null argument");
+    *out = nullptr;
+    MST_GUARD({
+        mst_mesh *m = new mst_mesh();
+        int rc = mst::read_mesh_cache(path, *m);
+        if (rc) {
+            delete m;
+            return rc;
+        }
+        *out = m;
+    });
+    return MST_OK;
+}
+
 int mst_write_tecplot(const mst_mesh *m, const double *Q, int64_t n_cells, double gamma, double cv, int wall_noslip,
                       const char *title, const char *path) {
     if (!m || !Q || !path) return mst::fail(MST_E_ARG, "null argument");

@@ -61,6 +61,8 @@ int write_tecplot(const mst_mesh &m, const double *Q, double gamma, double cv, i
                   const std::string &path);
 int write_checkpoint(const std::string &path, const mst_mesh &m, const double *Q, int64_t step, double time);
 int read_checkpoint(const std::string &path, const mst_mesh &m, double *Q, int64_t *step, double *time);
+int write_mesh_cache(const mst_mesh &m, const std::string &path);
+int read_mesh_cache(const std::string &path, mst_mesh &m);
 
 // std::mt19937_64 (its output sequence is fixed by the C++ standard) with a portable 53-bit
 // uniform in [0,1), so generated meshes are identical on every host

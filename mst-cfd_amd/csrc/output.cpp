@@ -24,6 +24,12 @@
 //   data    n_cells x nv doubles, the state in file order
 // in the host's byte order (little-endian on x86-64).
 // The fingerprint is FNV-1a 64 over the mesh's counts, node coordinates and face tables.
+//
+// Mesh cache: the reader's result (mst_mesh, file order) in binary, so a large mesh (C3's Fluent text
+// runs to gigabytes) is parsed once:
+//   "MSTMESH1", version, dim, fnn, counts (nodes, cells, faces, n_int, zones), the arrays
+//   (xyz, fnodes, c0, c1, ftype, partner), each zone (id, first, last, type, name), and an FNV-1a 64
+//   checksum of everything before it.  Reading validates the mesh as the reader does.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -258,6 +264,147 @@ int read_checkpoint(const std::string &path, const mst_mesh &m, double *Q, int64
     memcpy(Q, buf.data(), nbytes);
     if (step) *step = h.step;
     if (time) *time = h.time;
+    return MST_OK;
+}
+
+namespace {
+const char MESH_MAGIC[8] = {'M', 'S', 'T', 'M', 'E', 'S', 'H', '1'};
+
+struct Writer {
+    std::vector<unsigned char> buf;
+    void put(const void *p, size_t n) {
+        const unsigned char *b = (const unsigned char *)p;
+        buf.insert(buf.end(), b, b + n);
+    }
+    template <typename T>
+    void val(T v) { put(&v, sizeof v); }
+    template <typename T>
+    void vec(const std::vector<T> &v) { put(v.data(), v.size() * sizeof(T)); }
+};
+
+struct Reader {
+    const unsigned char *p, *end;
+    bool ok = true;
+    void get(void *dst, size_t n) {
+        if ((size_t)(end - p) < n) {
+            ok = false;
+            return;
+        }
+        memcpy(dst, p, n);
+        p += n;
+    }
+    template <typename T>
+    T val() {
+        T v{};
+        get(&v, sizeof v);
+        return v;
+    }
+    template <typename T>
+    void vec(std::vector<T> &v, int64_t n) {
+        if (n < 0 || (uint64_t)n > (uint64_t)(end - p) / sizeof(T)) {
+            ok = false;
+            return;
+        }
+        v.resize((size_t)n);
+        get(v.data(), (size_t)n * sizeof(T));
+    }
+};
+}  // namespace
+
+int write_mesh_cache(const mst_mesh &m, const std::string &path) {
+    Writer w;
+    w.put(MESH_MAGIC, 8);
+    w.val<uint32_t>(1);
+    w.val<int32_t>(m.dim);
+    w.val<int32_t>(m.fnn);
+    w.val<int64_t>(m.n_nodes);
+    w.val<int64_t>(m.n_cells);
+    w.val<int64_t>(m.n_faces);
+    w.val<int64_t>(m.n_int);
+    w.val<int64_t>((int64_t)m.zones.size());
+    w.vec(m.xyz);
+    w.vec(m.fnodes);
+    w.vec(m.c0);
+    w.vec(m.c1);
+    w.vec(m.ftype);
+    w.vec(m.partner);
+    for (const mst_zone &z : m.zones) {
+        w.val<int32_t>(z.id);
+        w.val<int32_t>(z.first);
+        w.val<int32_t>(z.last);
+        w.val<int32_t>(z.type);
+        w.val<int32_t>((int32_t)z.name.size());
+        w.put(z.name.data(), z.name.size());
+    }
+    Fnv h;
+    h.add(w.buf.data(), w.buf.size());
+    w.val<uint64_t>(h.h);
+    const std::string tmp = path + ".tmp";
+    FILE *fp = fopen(tmp.c_str(), "wb");
+    if (!fp) return fail(MST_E_IO, "cannot open " + tmp + " for writing");
+    bool ok = fwrite(w.buf.data(), 1, w.buf.size(), fp) == w.buf.size();
+    ok = (fclose(fp) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) {
+        std::remove(tmp.c_str());
+        return fail(MST_E_IO, "write failed: " + path);
+    }
+    return MST_OK;
+}
+
+int read_mesh_cache(const std::string &path, mst_mesh &m) {
+    FILE *fp = fopen(path.c_str(), "rb");
+    if (!fp) return fail(MST_E_IO, "cannot open " + path);
+    std::vector<unsigned char> buf;
+    unsigned char chunk[1 << 16];
+    size_t got;
+    while ((got = fread(chunk, 1, sizeof chunk, fp)) > 0) buf.insert(buf.end(), chunk, chunk + got);
+    const bool err = ferror(fp) != 0;
+    fclose(fp);
+    if (err) return fail(MST_E_IO, "read failed: " + path);
+    if (buf.size() < 8 + sizeof(uint64_t) || memcmp(buf.data(), MESH_MAGIC, 8) != 0)
+        return fail(MST_E_IO, path + ": not a mesh cache of this format");
+    Fnv h;
+    h.add(buf.data(), buf.size() - sizeof(uint64_t));
+    uint64_t stored;
+    memcpy(&stored, buf.data() + buf.size() - sizeof(uint64_t), sizeof stored);
+    if (stored != h.h) return fail(MST_E_IO, path + ": checksum mismatch (damaged file)");
+    Reader r{buf.data() + 8, buf.data() + buf.size() - sizeof(uint64_t)};
+    if (r.val<uint32_t>() != 1) return fail(MST_E_IO, path + ": unknown version");
+    mst_mesh t;
+    t.dim = r.val<int32_t>();
+    t.fnn = r.val<int32_t>();
+    t.n_nodes = r.val<int64_t>();
+    t.n_cells = r.val<int64_t>();
+    t.n_faces = r.val<int64_t>();
+    t.n_int = r.val<int64_t>();
+    const int64_t nz = r.val<int64_t>();
+    if (!r.ok || (t.dim != 2 && t.dim != 3) || t.fnn < 2 || t.fnn > 4 || t.n_nodes < 0 || t.n_faces < 0 || nz < 0)
+        return fail(MST_E_IO, path + ": bad header");
+    r.vec(t.xyz, t.n_nodes * t.dim);
+    r.vec(t.fnodes, t.n_faces * t.fnn);
+    r.vec(t.c0, t.n_faces);
+    r.vec(t.c1, t.n_faces);
+    r.vec(t.ftype, t.n_faces);
+    r.vec(t.partner, t.n_faces);
+    for (int64_t i = 0; i < nz && r.ok; i++) {
+        mst_zone z;
+        z.id = r.val<int32_t>();
+        z.first = r.val<int32_t>();
+        z.last = r.val<int32_t>();
+        z.type = r.val<int32_t>();
+        const int32_t len = r.val<int32_t>();
+        if (len < 0 || len > (int32_t)(r.end - r.p)) {
+            r.ok = false;
+            break;
+        }
+        z.name.assign((const char *)r.p, (size_t)len);
+        r.p += len;
+        t.zones.push_back(z);
+    }
+    if (!r.ok || r.p != r.end) return fail(MST_E_IO, path + ": truncated or inconsistent");
+    int rc = validate(t);
+    if (rc) return rc;
+    m = std::move(t);
     return MST_OK;
 }
 

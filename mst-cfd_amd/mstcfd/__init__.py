@@ -53,7 +53,7 @@ EXPORTS = [
     "mst_mesh_cell_centres", "mst_batch_set_strategy", "mst_batch_get_strategy", "mst_set_cell_order",
     "mst_get_cell_order", "mst_set_steady_driver", "mst_lusgs_step", "mst_lusgs_colours", "mst_prep_cell_colours",
     "mst_write_tecplot", "mst_checkpoint_write", "mst_checkpoint_read", "mst_batch_set_steady_driver",
-    "mst_batch_lusgs_step",
+    "mst_batch_lusgs_step", "mst_mesh_write_cache", "mst_mesh_read_cache",
 ]
 
 
@@ -120,6 +120,8 @@ def lib():
     sig = {
         "mst_last_error": (C.c_size_t, [C.c_char_p, C.c_size_t]),
         "mst_mesh_read_msh": (C.c_int, [C.c_char_p, pp]),
+        "mst_mesh_read_cache": (C.c_int, [C.c_char_p, pp]),
+        "mst_mesh_write_cache": (C.c_int, [vp, C.c_char_p]),
         "mst_mesh_write_msh": (C.c_int, [vp, C.c_char_p]),
         "mst_mesh_from_arrays": (C.c_int, [C.c_int, C.c_int64, F64P, C.c_int64, C.c_int64, C.c_int, I32P, I32P, I32P,
                                            I32P, I32P, pp]),
@@ -286,6 +288,14 @@ class Mesh:
     @classmethod
     def read_msh(cls, path):
         return cls._new(lib().mst_mesh_read_msh, str(path).encode())
+
+    @classmethod
+    def read_cache(cls, path):
+        """A mesh from its binary cache (mst_mesh_write_cache)."""
+        return cls._new(lib().mst_mesh_read_cache, str(path).encode())
+
+    def write_cache(self, path):
+        _check(lib().mst_mesh_write_cache(self.h, str(path).encode()), "mst_mesh_write_cache")
 
     @classmethod
     def from_raw(cls, raw):

@@ -164,3 +164,36 @@ def test_checkpoint_round_trip_and_refusals(tmp_path):
     (tmp_path / "text.ckpt").write_bytes(b"not a checkpoint at all" * 10)
     with pytest.raises(M.MstError, match="format"):
         m.read_checkpoint(tmp_path / "text.ckpt")
+
+
+@pytest.mark.parametrize("name", ["tri", "periodic", "quad", "tet", "fluent"])
+def test_mesh_cache_round_trip(name, tmp_path):
+    if name == "fluent":
+        import os
+        d = dict(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "msh_2d-stairW-1.npz")))
+        for k in ("dim", "n_cells", "n_int"):
+            d[k] = int(d[k])
+        m = M.Mesh.from_raw(d)
+    else:
+        m = MESHES[name][0]()
+    path = tmp_path / "mesh.bin"
+    m.write_cache(path)
+    m2 = M.Mesh.read_cache(path)
+    a, b = m.raw(), m2.raw()
+    for k in a:
+        assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
+    assert np.array_equal(m.zones(), m2.zones())
+    assert (m2.n_cells, m2.n_faces, m2.n_nodes, m2.n_int) == (m.n_cells, m.n_faces, m.n_nodes, m.n_int)
+    # a damaged copy, a truncated copy and another format are refused
+    raw = path.read_bytes()
+    bad = bytearray(raw)
+    bad[len(raw) // 2] ^= 0x01
+    (tmp_path / "bad.bin").write_bytes(bytes(bad))
+    with pytest.raises(M.MstError, match="checksum"):
+        M.Mesh.read_cache(tmp_path / "bad.bin")
+    (tmp_path / "short.bin").write_bytes(raw[:len(raw) // 3])
+    with pytest.raises(M.MstError):
+        M.Mesh.read_cache(tmp_path / "short.bin")
+    m.write_checkpoint(tmp_path / "state.ckpt", np.ones((m.n_cells, m.dim + 2)))
+    with pytest.raises(M.MstError, match="format"):
+        M.Mesh.read_cache(tmp_path / "state.ckpt")
