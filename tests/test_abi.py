@@ -91,3 +91,15 @@ def test_config_layout_matches_oracle():
     for (a, ta), (b, tb) in zip(M.Config._fields_, O.Config._fields_):
         assert a == b and ta == tb
     assert np.allclose(M.reference_inlet(), O.reference_inlet()[:4])
+
+
+def test_cpp_driver_fails_loudly_without_a_device():
+    """include/mstcfd.hpp's RhoSolverGPU throws when there is no HIP device; the driver reports it."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "mst-cfd_amd", "lib", "rhosolver_driver")
+    assert os.path.exists(exe), "built by __graft_entry__.build()"
+    r = subprocess.run([exe, os.devnull, "1", "1e-4"], capture_output=True, text=True, timeout=120)
+    if r.returncode == 0:
+        pytest.skip("a HIP device is present")
+    assert r.returncode == 3 and "no HIP device" in r.stdout
