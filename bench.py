@@ -51,7 +51,7 @@ TRAFFIC_PROFILES = {("vortex2d", st, k): ("profiles/hbm_traffic_vortex2d.md", na
                                         ("cell", "k_grad", "k_grad<2, 3, 2, double>"),
                                         ("cell", "k_cell", "k_cell<2, 0, true, double, 3>"),
                                         ("patch", "k_grad", "k_grad<2, 3, 2, double>"),
-                                        ("patch", "k_patch", "k_patch<2, 0, true, double, 3>"),
+                                        ("patch", "k_patch", "k_patch<2, 0, true, double, 3, true>"),
                                         ("fused", "k_fused", "k_fused<2, 0, 2, double, 3>"))}
 TRAFFIC_PROFILES[("steady", "lusgs", "k_lu_sweep")] = ("profiles/hbm_traffic_lusgs_ramp300k.md", "k_lu_sweep")
 
