@@ -513,6 +513,7 @@ struct mst_batch {
     int32_t *hstart = nullptr, *hcell = nullptr;
     int16_t *hloc = nullptr;
     bool fused_ok = false;
+    HaloGeo hg;  // the halo cells' incidence geometry in halo order
     std::vector<uint8_t> active;
     bool use_graph = true;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -566,6 +567,7 @@ void release(mst_batch *b) {
                     b->lu_active};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
+    b->hg.release();
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -644,6 +646,14 @@ int build_patches(mst_batch *b) {
     b->max_pf = (int)max_pf;
     b->patch_ok = true;
     b->fused_ok = max_pf * b->nv * 16 <= 65536;  // the fused kernel keeps states and fluxes in LDS
+    if (b->fused_ok && !hcell.empty()) {
+        const IncView<double> f64{b->inc_code, b->inc_w, b->inc_n, b->inc_dx, b->cvol, b->NC};
+        const IncView<float> f32{b->mixed ? b->inc_code : nullptr, b->inc_w32, b->inc_n32, b->inc_dx32, b->cvol, b->NC};
+        const int64_t nh = (int64_t)hcell.size();
+        HIPCHK(b->dim == 2 ? b->hg.fill<2>(b->slots, b->NC, nh, b->hcell, f64, f32, b->stream)
+                          : b->hg.fill<3>(b->slots, b->NC, nh, b->hcell, f64, f32, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+    }
     return MST_OK;
 }
 
@@ -910,6 +920,7 @@ int run_fused(mst_batch *b, int stage, int is, int in, double ca, double cb, int
     ha.start = b->hstart;
     ha.cell = b->hcell;
     ha.loc = b->hloc;
+    b->hg.args<T>(ha);
     const size_t shm = (size_t)FusedLds<T>::bytes_per_face_field * b->nv * b->max_pf;
 #define MST_BFUSED(LM, MF) return launch_persistent_shm(b, kb_fused<DIM, LM, T, MF>, BLOCK, shm, a.n_items, a, pa, ha);
 #define MST_BFUSED_L(MF)                                          \

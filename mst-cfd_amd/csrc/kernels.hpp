@@ -203,14 +203,25 @@ struct GradArgs {
 // sum written with differences, (1/V) sum_f w_f (Q_nb - Q_c) n_out (w = 1-eta on the c0 side, eta on
 // the c1 side), equal because the outward normals of a closed cell sum to zero, and free of the
 // cancellation an FP32 sum of whole states would suffer.
+// Where a cell's incidence geometry lives: slot s of the cell at geometry index g is entry s * stride + g
+// of code / w / n / dx (n and dx DIM-wide), its volume vol[g].  The cell-ordered arrays have g = c and
+// stride nc; a patch's halo reads a packed copy in halo order.
+template <typename T>
+struct IncView {
+    const int32_t *code;
+    const T *w, *n, *dx;
+    const double *vol;
+    int64_t stride;
+};
+
 template <int DIM, int MAXF, int LIM, typename T>
-__device__ __forceinline__ int grad_limited(const GradArgs<T> &a, const int64_t c, T (&qc)[DIM + 2],
-                                           T (&lgr)[(DIM + 2) * DIM], T (&dxs)[MAXF][DIM], T (&t)[DIM + 2][DIM],
-                                           T (&ph)[DIM + 2]) {
+__device__ __forceinline__ int grad_limited(const T *__restrict__ Q, const IncView<T> &a, const int64_t g,
+                                           const int64_t c, T (&qc)[DIM + 2], T (&lgr)[(DIM + 2) * DIM],
+                                           T (&dxs)[MAXF][DIM], T (&t)[DIM + 2][DIM], T (&ph)[DIM + 2]) {
     constexpr int NV = DIM + 2;
     constexpr bool F32 = std::is_same<T, float>::value;
     T qmax[NV], qmin[NV];
-    ldq<NV>(a.Q + c * NV, qc);
+    ldq<NV>(Q + c * NV, qc);
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         qmax[k] = qmin[k] = qc[k];
@@ -220,22 +231,22 @@ __device__ __forceinline__ int grad_limited(const GradArgs<T> &a, const int64_t 
     int nfc = 0;
 #pragma unroll
     for (int s = 0; s < MAXF; s++) {
-        const int64_t i = (int64_t)s * a.nc + c;
-        const int32_t code = a.inc_code[i];
+        const int64_t i = (int64_t)s * a.stride + g;
+        const int32_t code = a.code[i];
         if (code == -2) break;
         const int64_t nb = code >= 0 ? (int64_t)(code >> 1) : -1;
         const bool is1 = code >= 0 && (code & 1);
-        const T eta = a.inc_w[i];
+        const T eta = a.w[i];
         T n[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
-            n[d] = a.inc_n[i * DIM + d];
-            dxs[s][d] = a.inc_dx[i * DIM + d];
+            n[d] = a.n[i * DIM + d];
+            dxs[s][d] = a.dx[i * DIM + d];
         }
         T qf[NV];
         if (nb >= 0) {
             T qn[NV];
-            ldq<NV>(a.Q + nb * NV, qn);
+            ldq<NV>(Q + nb * NV, qn);
             if constexpr (F32) {
 #pragma unroll
                 for (int k = 0; k < NV; k++) qf[k] = eta * (qn[k] - qc[k]);
@@ -258,7 +269,7 @@ __device__ __forceinline__ int grad_limited(const GradArgs<T> &a, const int64_t 
             for (int d = 0; d < DIM; d++) t[k][d] += qf[k] * n[d];
         nfc = s + 1;
     }
-    const T iV = (T)frcp(a.vol[c]);
+    const T iV = (T)frcp(a.vol[g]);
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         ph[k] = 1;
@@ -306,12 +317,17 @@ __device__ __forceinline__ int grad_limited(const GradArgs<T> &a, const int64_t 
     return nfc;
 }
 
+template <typename T>
+__device__ __forceinline__ IncView<T> cell_view(const GradArgs<T> &a) {
+    return IncView<T>{a.inc_code, a.inc_w, a.inc_n, a.inc_dx, a.vol, a.nc};
+}
+
 // grad_limited, then the cell's face records and the optional raw gradient / limiter outputs
 template <int DIM, int MAXF, int LIM, typename T>
 __device__ __forceinline__ void grad_cell(const GradArgs<T> &a, const int64_t c) {
     constexpr int NV = DIM + 2;
     T qc[NV], lgr[NV * DIM], dxs[MAXF][DIM], t[NV][DIM], ph[NV];
-    const int nfc = grad_limited<DIM, MAXF, LIM, T>(a, c, qc, lgr, dxs, t, ph);
+    const int nfc = grad_limited<DIM, MAXF, LIM, T>(a.Q, cell_view(a), c, c, qc, lgr, dxs, t, ph);
     // reconstructed states at the cell's faces (RhoSolver.cpp:250,254 with the limiter), one record
     // per incidence, coalesced over the cells of a slot
 #pragma unroll
@@ -711,6 +727,91 @@ struct HaloArgs {
     const int32_t *__restrict__ start;  // nblk + 1: each block's range in cell
     const int32_t *__restrict__ cell;   // halo cells of each patch
     const int16_t *__restrict__ loc;    // per halo entry, MAXF slots: (local face << 1) | side, -1: not in the patch
+    // the halo cells' incidence geometry copied in halo order (slot-major, entry s * nh + j; k_halo_geo):
+    // a patch reads its halo's geometry as coalesced runs, where the cell-ordered arrays would cost a
+    // mostly unused line per slot and array for each scattered halo cell.  w, n, dx are of the
+    // kernel's real type.
+    const int32_t *__restrict__ code;
+    const void *w, *n, *dx;
+    const double *__restrict__ vol;
+    int64_t nh;
+};
+
+// the packed halo geometry: one lane per halo entry j, a copy of its cell's incidence entries
+template <int DIM, typename T>
+__global__ __launch_bounds__(BLOCK) void k_halo_geo(int64_t nh, int slots, int64_t nc, const int32_t *__restrict__ hcell,
+                                                    IncView<T> src, int32_t *__restrict__ code, T *__restrict__ w,
+                                                    T *__restrict__ n, T *__restrict__ dx, double *__restrict__ vol) {
+    const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= nh) return;
+    const int64_t c = hcell[j];
+    for (int s = 0; s < slots; s++) {
+        const int64_t i = (int64_t)s * nc + c, o = (int64_t)s * nh + j;
+        code[o] = src.code[i];
+        w[o] = src.w[i];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            n[o * DIM + d] = src.n[i * DIM + d];
+            dx[o * DIM + d] = src.dx[i * DIM + d];
+        }
+    }
+    vol[j] = src.vol[c];
+}
+
+// Host side of the packed halo geometry: the device copies, FP64 always and FP32 for the mixed form.
+struct HaloGeo {
+    int32_t *code = nullptr;
+    double *w = nullptr, *n = nullptr, *dx = nullptr, *vol = nullptr;
+    float *w32 = nullptr, *n32 = nullptr, *dx32 = nullptr;
+    int64_t nh = 0;
+    void release() {
+        for (void *p : {(void *)code, (void *)w, (void *)n, (void *)dx, (void *)vol, (void *)w32, (void *)n32, (void *)dx32})
+            if (p) (void)hipFree(p);
+        *this = HaloGeo();
+    }
+    // (re)fills the copies from the cell-ordered arrays (f32.code null: no FP32 form); allocates on
+    // first use.  Call after every change of the incidence weights.
+    template <int DIM>
+    hipError_t fill(int slots, int64_t nc, int64_t nh_, const int32_t *hcell, const IncView<double> &f64,
+                    const IncView<float> &f32, hipStream_t st) {
+        if (nh_ <= 0) return hipSuccess;
+        const size_t m = (size_t)slots * nh_;
+        hipError_t e = hipSuccess;
+        auto get = [&](auto **p, size_t count) {
+            if (!*p && e == hipSuccess) e = hipMalloc((void **)p, count * sizeof(**p));
+        };
+        get(&code, m);
+        get(&w, m);
+        get(&n, m * DIM);
+        get(&dx, m * DIM);
+        get(&vol, (size_t)nh_);
+        if (f32.code) {
+            get(&w32, m);
+            get(&n32, m * DIM);
+            get(&dx32, m * DIM);
+        }
+        if (e != hipSuccess) return e;
+        nh = nh_;
+        const dim3 grid((uint32_t)((nh + BLOCK - 1) / BLOCK));
+        hipLaunchKernelGGL((k_halo_geo<DIM, double>), grid, dim3(BLOCK), 0, st, nh, slots, nc, hcell, f64, code, w, n, dx, vol);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (f32.code) {
+            hipLaunchKernelGGL((k_halo_geo<DIM, float>), grid, dim3(BLOCK), 0, st, nh, slots, nc, hcell, f32, code, w32,
+                               n32, dx32, vol);
+            e = hipGetLastError();
+        }
+        return e;
+    }
+    template <typename T>
+    void args(HaloArgs &h) const {
+        constexpr bool F32 = std::is_same<T, float>::value;
+        h.code = code;
+        h.w = F32 ? (const void *)w32 : (const void *)w;
+        h.n = F32 ? (const void *)n32 : (const void *)n;
+        h.dx = F32 ? (const void *)dx32 : (const void *)dx;
+        h.vol = vol;
+        h.nh = nh;
+    }
 };
 
 // LDS of the fused kernels for npf faces: the two sides' face states and the fluxes, double[NV][npf].
@@ -787,22 +888,29 @@ __device__ __forceinline__ void fused_patch(const GradArgs<T> &ga, const FluxArg
     const int32_t h0 = h.start[patch];
     const int next = BLOCK + (h.start[patch + 1] - h0);
     const int32_t hc = BLOCK + (int)threadIdx.x < next ? h.cell[h0 + threadIdx.x] : 0;  // first halo pass
-    for (int i = threadIdx.x; i < next; i += BLOCK) {
-        int64_t cc;
+    const IncView<T> own = cell_view(ga);
+    const IncView<T> hv{h.code, static_cast<const T *>(h.w), static_cast<const T *>(h.n),
+                        static_cast<const T *>(h.dx), h.vol, h.nh};
+    for (int i0 = 0; i0 < next; i0 += BLOCK) {  // i0 == 0: the patch cells, then the halo
+        const int i = i0 + (int)threadIdx.x;
+        if (i >= next) break;
+        int64_t cc, g;
         const int16_t *lp;
         int64_t stride;
-        if (i < BLOCK) {
+        if (i0 == 0) {
             cc = cb + i;
             if (cc >= ce) continue;
+            g = cc;
             lp = p.loc + cc;
             stride = nc;
         } else {
-            cc = i < 2 * BLOCK ? hc : h.cell[h0 + i - BLOCK];
-            lp = h.loc + (int64_t)(h0 + i - BLOCK) * MAXF;
+            g = h0 + i - BLOCK;
+            cc = i < 2 * BLOCK ? hc : h.cell[g];
+            lp = h.loc + g * MAXF;
             stride = 1;
         }
         T qc[NV], lgr[NV * DIM], dxs[MAXF][DIM], t[NV][DIM], ph[NV];
-        const int nfc = grad_limited<DIM, MAXF, LIM, T>(ga, cc, qc, lgr, dxs, t, ph);
+        const int nfc = grad_limited<DIM, MAXF, LIM, T>(ga.Q, i0 == 0 ? own : hv, g, cc, qc, lgr, dxs, t, ph);
 #pragma unroll
         for (int s = 0; s < MAXF; s++) {
             if (s >= nfc) break;

@@ -82,6 +82,7 @@ struct mst_ctx {
     int32_t *hstart = nullptr, *hcell = nullptr;
     int16_t *hloc = nullptr;
     int64_t n_halo = 0;
+    HaloGeo hg;  // the halo cells' incidence geometry in halo order
     // mixed form: FP32 shadows of the states, FP32 geometry, offsets, face fluxes, limited gradients
     float *Q32[4] = {nullptr, nullptr, nullptr, nullptr};
     bool shadow_ok[4] = {false, false, false, false};
@@ -434,6 +435,7 @@ int run_fused_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int 
     ha.start = x->hstart;
     ha.cell = x->hcell;
     ha.loc = x->hloc;
+    x->hg.args<T>(ha);
     const size_t shm = (size_t)FusedLds<T>::bytes_per_face_field * x->nv * x->max_pf;
     const bool roe = x->cfg.scheme == MST_SCHEME_ROE;
     const int lim = x->cfg.limiter;
@@ -721,6 +723,13 @@ int upload_incidence_weights(mst_ctx *x, const std::vector<double> &eta) {
         }
         HIPCHK(hipMemcpy(x->inc_w32, w32.data(), sizeof(float) * n, hipMemcpyHostToDevice));
     }
+    if (x->patch_ok && x->n_halo > 0) {
+        const IncView<double> f64{x->inc_code, x->inc_w, x->inc_n, x->inc_dx, x->cvol, x->nc};
+        const IncView<float> f32{x->mixed ? x->inc_code : nullptr, x->inc_w32, x->inc_n32, x->inc_dx32, x->cvol, x->nc};
+        HIPCHK(x->dim == 2 ? x->hg.fill<2>(x->slots, x->nc, x->n_halo, x->hcell, f64, f32, x->stream)
+                           : x->hg.fill<3>(x->slots, x->nc, x->n_halo, x->hcell, f64, f32, x->stream));
+        HIPCHK(hipStreamSynchronize(x->stream));
+    }
     return MST_OK;
 }
 
@@ -759,6 +768,7 @@ void release(mst_ctx *x) {
                     x->lu_cells, x->lu_nb, x->lu_face, x->lu_n, x->lu_r, x->lu_D, x->lu_dQ, x->gw};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    x->hg.release();
     for (int k = 0; k < MST_K_COUNT; k++)
         for (auto &pr : x->ev[k]) {
             (void)hipEventDestroy(pr.first);
