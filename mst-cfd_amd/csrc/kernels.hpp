@@ -391,24 +391,30 @@ enum { UPD_RESIDUAL = 0, UPD_STAGE = 1, UPD_L2 = 2, UPD_DT = 4, UPD_CHECK = 8 };
 // Block-level reduction of W values (fixed tree order): sums, or a min when MIN is set; thread 0..W-1
 // of the block write the results.  The per-block partials are combined by k_finish in block order,
 // so every reduction is bit-reproducible (no float atomics, no cross-block fences).
+// sh: W * NT doubles of LDS.  The first barrier also orders the block's earlier readers of sh before
+// the writes, so a kernel may pass LDS it has finished with.
 template <int W, bool MIN, int NT = BLOCK>
-__device__ __forceinline__ void block_partial(double (&v)[W], double *__restrict__ out, int64_t stride) {
-    __shared__ double sh[W][NT];
+__device__ __forceinline__ void block_partial(double (&v)[W], double *__restrict__ out, int64_t stride, double *sh) {
     __syncthreads();  // a previous call's readers of sh[.][0] are done (persistent loops reuse it)
 #pragma unroll
-    for (int k = 0; k < W; k++) sh[k][threadIdx.x] = v[k];
+    for (int k = 0; k < W; k++) sh[k * NT + threadIdx.x] = v[k];
     __syncthreads();
     for (int w = NT / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
 #pragma unroll
             for (int k = 0; k < W; k++) {
-                const double y = sh[k][threadIdx.x + w];
-                sh[k][threadIdx.x] = MIN ? fmin(sh[k][threadIdx.x], y) : sh[k][threadIdx.x] + y;
+                const double y = sh[k * NT + threadIdx.x + w];
+                sh[k * NT + threadIdx.x] = MIN ? fmin(sh[k * NT + threadIdx.x], y) : sh[k * NT + threadIdx.x] + y;
             }
         }
         __syncthreads();
     }
-    if ((int)threadIdx.x < W) out[threadIdx.x * stride] = sh[threadIdx.x][0];
+    if ((int)threadIdx.x < W) out[threadIdx.x * stride] = sh[threadIdx.x * NT];
+}
+template <int W, bool MIN, int NT = BLOCK>
+__device__ __forceinline__ void block_partial(double (&v)[W], double *__restrict__ out, int64_t stride) {
+    __shared__ double sh[W * NT];
+    block_partial<W, MIN, NT>(v, out, stride, sh);
 }
 
 template <int DIM>
@@ -505,9 +511,12 @@ __device__ __forceinline__ void stage_cell(const UpdArgs &a, int64_t c, const do
 
 // stage_cell for the lanes of one block, then the block's partials of |R/V|^2 and the CFL time step.
 // Every lane of the block must call it (block-level reductions).
-template <int NV, bool STAGE, bool F32>
+// lds (DYN): NV * BLOCK doubles of the kernel's dynamic LDS for the reductions, in place of a static
+// array (which would count against every block's LDS for the whole kernel).
+template <int NV, bool STAGE, bool F32, bool DYN = false>
 __device__ __forceinline__ void stage_tail(const UpdArgs &a, int64_t c, bool valid, const double (&r)[NV],
-                                           int64_t blk, const bool L2, const bool DT, const StageIn<NV> *in = nullptr) {
+                                           int64_t blk, const bool L2, const bool DT, const StageIn<NV> *in = nullptr,
+                                           double *lds = nullptr) {
     if constexpr (!STAGE) {
         if (valid) stq<NV>(a.R + c * NV, r);
         if (L2) {  // residual-only evaluation with its norms (the implicit steady driver)
@@ -540,11 +549,17 @@ __device__ __forceinline__ void stage_tail(const UpdArgs &a, int64_t c, bool val
             double v[NV];
 #pragma unroll
             for (int k = 0; k < NV; k++) v[k] = red[k];
-            block_partial<NV, false>(v, a.part_l2 + blk, gridDim.x);
+            if constexpr (DYN)
+                block_partial<NV, false>(v, a.part_l2 + blk, gridDim.x, lds);
+            else
+                block_partial<NV, false>(v, a.part_l2 + blk, gridDim.x);
         }
         if (DT) {
             double v[1] = {red[NV]};
-            block_partial<1, true>(v, a.part_dt + blk, 1);
+            if constexpr (DYN)
+                block_partial<1, true>(v, a.part_dt + blk, 1, lds);
+            else
+                block_partial<1, true>(v, a.part_dt + blk, 1);
         }
     }
 }
@@ -839,13 +854,14 @@ struct FusedLds {
 
 // Resident waves per SIMD requested for the fused kernel, where the bound costs no more than a register
 // of spill: FP64 triangles with Roe or unlimited AUSM 4 (at most 128 VGPRs; the compiler would keep 3
-// and the kernel, bound by its dependent load rounds, runs about 15 % slower), FP32 triangles 6 (at
-// most 80 VGPRs, 3 % faster than 5), other 2D FP32 cells 4.  Elsewhere the bound would spill, so the
-// compiler chooses.
+// and the kernel, bound by its dependent load rounds, runs about 15 % slower), FP32 triangles 7 (at
+// most 72 VGPRs with one register spilled; 5 % faster than 6, which the kernel's LDS also allows
+// since its block reductions reuse the dynamic LDS), other 2D FP32 cells 4.  Elsewhere the bound
+// would spill, so the compiler chooses.
 template <int DIM, int SCHEME, int LIM, typename T, int MAXF>
 constexpr int fused_waves() {
     if constexpr (DIM != 2) return 1;
-    if constexpr (sizeof(T) == 4) return MAXF == 3 ? 6 : 4;
+    if constexpr (sizeof(T) == 4) return MAXF == 3 ? 7 : 4;
     return ((SCHEME == MST_SCHEME_ROE || LIM == MST_LIM_NONE) && MAXF == 3) ? 4 : 1;
 }
 
@@ -984,7 +1000,7 @@ __global__ __launch_bounds__(BLOCK, (fused_waves<DIM, SCHEME, LIM, T, MAXF>())) 
     });
     double r[NV];
     patch_sum<NV, MAXF, F32>(L.g, L.npf, cl, r);
-    stage_tail<NV, true, F32>(a, c, valid, r, blk, a.l2, a.dtred, &in);
+    stage_tail<NV, true, F32, true>(a, c, valid, r, blk, a.l2, a.dtred, &in, sh);  // (sh at least NV * BLOCK doubles)
 }
 
 // ------------------------------------------------------------------------------------ reductions

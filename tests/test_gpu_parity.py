@@ -252,6 +252,33 @@ def test_cell_strategy_matches_face_strategy(mname, scheme, lim, precision):
         assert np.array_equal(out[k][1], out[0][1])
 
 
+def test_fused_follows_reconfiguration():
+    """The fused kernel reads its halo cells' geometry from a copy packed in halo order.  A
+    configuration change that moves the incidence weights (the last-interior-face quirk) or switches
+    the precision refreshes that copy, so a reconfigured context steps exactly as a fresh one."""
+    m, om, geo = mesh("allbc")
+    Q = smooth_state(geo["ccen"], seed=6, mach=0.7)
+    base = dict(scheme=AUSM, order=2, limiter=M.LIM_MINMOD, rk=3, cfl=0.45, q_inlet=inlet(2))
+    cfgs = [M.make_config(compat=0, **base), M.make_config(**base), M.make_config(precision=M.PREC_MIXED, **base),
+            M.make_config(compat=0, precision=M.PREC_MIXED, **base), M.make_config(compat=0, **base)]
+    s = M.Solver(m, cfgs[0])
+    s.strategy = M.STRATEGY_FUSED
+    prev = None
+    for cg in cfgs[1:]:
+        s.set_config(cg)
+        s.set_state(Q)
+        s.step(5)
+        f = M.Solver(m, cg)
+        f.strategy = M.STRATEGY_FUSED
+        f.set_state(Q)
+        f.step(5)
+        got = s.get_state()
+        assert np.array_equal(got, f.get_state())
+        if prev is not None:
+            assert not np.array_equal(got, prev)
+        prev = got
+
+
 def test_auto_strategy_resolves_and_matches():
     """AUTO times both strategies on the first stage (outputs to a spare buffer) and keeps one; the
     run equals an explicitly chosen strategy bit for bit."""
