@@ -790,6 +790,7 @@ struct HaloGeo {
     hipError_t fill(int slots, int64_t nc, int64_t nh_, const int32_t *hcell, const IncView<double> &f64,
                     const IncView<float> &f32, hipStream_t st) {
         if (nh_ <= 0) return hipSuccess;
+        if (code && nh_ != nh) return hipErrorInvalidValue;  // the copies are sized for one halo list
         const size_t m = (size_t)slots * nh_;
         hipError_t e = hipSuccess;
         auto get = [&](auto **p, size_t count) {
