@@ -278,8 +278,16 @@ __global__ __launch_bounds__(BLOCK) void kb_patch(const BArgs<T> a, const PatchA
 
 // fused strategy (order 2): one item is one patch, whose cells' and halo's gradients, face states and
 // fluxes stay in the block (fused_patch), then sums in slot order and the stage update per cell
+// FP32 triangles: 5 resident waves per SIMD (at most 96 VGPRs, 4 registers spilled); elsewhere the
+// compiler chooses.
+template <int DIM, typename T, int MAXF>
+constexpr int kb_fused_waves() {
+    return (sizeof(T) == 4 && DIM == 2 && MAXF == 3) ? 5 : 1;
+}
+
 template <int DIM, int LIM, typename T, int MAXF>
-__global__ __launch_bounds__(BLOCK) void kb_fused(const BArgs<T> a, const PatchArgs p, const HaloArgs h) {
+__global__ __launch_bounds__(BLOCK, (kb_fused_waves<DIM, T, MAXF>())) void kb_fused(const BArgs<T> a, const PatchArgs p,
+                                                                                     const HaloArgs h) {
     constexpr int NV = DIM + 2;
     constexpr bool F32 = std::is_same<T, float>::value;
     extern __shared__ double sh[];
