@@ -36,6 +36,15 @@
 #ifndef MST_FLUX_WAVES
 #define MST_FLUX_WAVES 1
 #endif
+// Fused kernel: patch faces per lane whose ids, kinds and normals are fetched before the gradient
+// phase (at least 1), and resident waves per SIMD for 2D FP32 triangles (0: the default of
+// fused_waves).
+#ifndef MST_FUSED_PF
+#define MST_FUSED_PF 2
+#endif
+#ifndef MST_FUSED_WAVES32
+#define MST_FUSED_WAVES32 0
+#endif
 
 namespace mstk {
 
@@ -862,7 +871,7 @@ struct FusedLds {
 template <int DIM, int SCHEME, int LIM, typename T, int MAXF>
 constexpr int fused_waves() {
     if constexpr (DIM != 2) return 1;
-    if constexpr (sizeof(T) == 4) return MAXF == 3 ? 7 : 4;
+    if constexpr (sizeof(T) == 4) return MAXF == 3 ? (MST_FUSED_WAVES32 > 0 ? MST_FUSED_WAVES32 : 7) : 4;
     return ((SCHEME == MST_SCHEME_ROE || LIM == MST_LIM_NONE) && MAXF == 3) ? 4 : 1;
 }
 
@@ -882,7 +891,7 @@ __device__ __forceinline__ void fused_patch(const GradArgs<T> &ga, const FluxArg
     // Inputs of the later phases that do not depend on the gradients are fetched first, so their
     // latency overlaps the gradient phase: each lane's first PF patch faces (face, kind, normal), its
     // cell's slot entries and its first halo cell.
-    constexpr int PF = 2;
+    constexpr int PF = MST_FUSED_PF;
     int64_t pf_f[PF];
     int pf_info[PF];
     double pf_n[PF][DIM];
