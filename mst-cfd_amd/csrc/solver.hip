@@ -82,7 +82,8 @@ struct mst_ctx {
     int32_t *hstart = nullptr, *hcell = nullptr;
     int16_t *hloc = nullptr;
     int64_t n_halo = 0;
-    HaloGeo hg;  // the halo cells' incidence geometry in halo order
+    HaloGeo hg;  // the halo cells' incidence geometry in halo order, filled on the fused kernel's first use
+    bool hg_dirty = true;  // hg does not hold the current incidence weights
     // mixed form: FP32 shadows of the states, FP32 geometry, offsets, face fluxes, limited gradients
     float *Q32[4] = {nullptr, nullptr, nullptr, nullptr};
     bool shadow_ok[4] = {false, false, false, false};
@@ -419,11 +420,15 @@ int run_patch_residual_t(mst_ctx *x, int mode, int is) {
 #undef MST_PATCH_R
 }
 
+int ensure_halo_geo(mst_ctx *x);
+
 // fused strategy (order 2): gradients, face states, fluxes, sums and update of a patch in one kernel
 template <int DIM, typename T>
 int run_fused_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
     if (x->cfg.order < 2) return run_patch_t<DIM, T>(x, mode, is, in, ca, cb, stage0, iout);
     if (!x->patch_ok) return mst::fail(MST_E_UNSUPPORTED, "no patch tables for this mesh");
+    int rc = ensure_halo_geo(x);
+    if (rc) return rc;
     const GradArgs<T> ga = grad_args<T>(x, is, false);
     const FluxArgs<T> fa = flux_args<T>(x, is);
     const UpdArgs a = upd_args(x, mode, is, in, ca, cb, stage0, iout);
@@ -725,13 +730,18 @@ int upload_incidence_weights(mst_ctx *x, const std::vector<double> &eta) {
         }
         HIPCHK(hipMemcpy(x->inc_w32, w32.data(), sizeof(float) * n, hipMemcpyHostToDevice));
     }
-    if (x->patch_ok && x->n_halo > 0) {
-        const IncView<double> f64{x->inc_code, x->inc_w, x->inc_n, x->inc_dx, x->cvol, x->nc};
-        const IncView<float> f32{x->mixed ? x->inc_code : nullptr, x->inc_w32, x->inc_n32, x->inc_dx32, x->cvol, x->nc};
-        HIPCHK(x->dim == 2 ? x->hg.fill<2>(x->slots, x->nc, x->n_halo, x->hcell, f64, f32, x->stream)
-                           : x->hg.fill<3>(x->slots, x->nc, x->n_halo, x->hcell, f64, f32, x->stream));
-        HIPCHK(hipStreamSynchronize(x->stream));
-    }
+    x->hg_dirty = true;
+    return MST_OK;
+}
+
+// the fused kernel's packed halo geometry, gathered on the context's stream when the weights changed
+int ensure_halo_geo(mst_ctx *x) {
+    if (!x->hg_dirty || !x->patch_ok || x->n_halo == 0) return MST_OK;
+    const IncView<double> f64{x->inc_code, x->inc_w, x->inc_n, x->inc_dx, x->cvol, x->nc};
+    const IncView<float> f32{x->mixed ? x->inc_code : nullptr, x->inc_w32, x->inc_n32, x->inc_dx32, x->cvol, x->nc};
+    HIPCHK(x->dim == 2 ? x->hg.fill<2>(x->slots, x->nc, x->n_halo, x->hcell, f64, f32, x->stream)
+                       : x->hg.fill<3>(x->slots, x->nc, x->n_halo, x->hcell, f64, f32, x->stream));
+    x->hg_dirty = false;
     return MST_OK;
 }
 
@@ -985,6 +995,10 @@ int resolve_strategy(mst_ctx *x) {
     // the trial stages may have flagged their (unchecked) outputs, e.g. with no time step set yet
     HIPCHK(hipStreamSynchronize(x->stream));
     HIPCHK(hipMemset(x->bad, 0, sizeof(int)));
+    if (pick != MST_STRATEGY_FUSED) {  // the fused trial's halo copy (0.64 N_c entries in 3D) is not kept
+        x->hg.release();
+        x->hg_dirty = true;
+    }
     return rc;
 }
 

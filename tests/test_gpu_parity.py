@@ -277,6 +277,14 @@ def test_fused_follows_reconfiguration():
         if prev is not None:
             assert not np.array_equal(got, prev)
         prev = got
+    # AUTO frees the copy when it keeps another strategy; fused again gathers it anew
+    s.strategy = M.STRATEGY_AUTO
+    s.set_state(Q)
+    s.step(1)
+    s.strategy = M.STRATEGY_FUSED
+    s.set_state(Q)
+    s.step(5)
+    assert np.array_equal(s.get_state(), prev)
 
 
 def test_auto_strategy_resolves_and_matches():
