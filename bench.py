@@ -54,6 +54,9 @@ TRAFFIC_PROFILES = {("vortex2d", st, k): ("profiles/hbm_traffic_vortex2d.md", na
                                         ("patch", "k_patch", "k_patch<2, 0, true, double, 3, true>"),
                                         ("fused", "k_fused", "k_fused<2, 0, 2, double, 3>"))}
 TRAFFIC_PROFILES[("steady", "lusgs", "k_lu_sweep")] = ("profiles/hbm_traffic_lusgs_ramp300k.md", "k_lu_sweep")
+TRAFFIC_PROFILES.update({("pulse3d", "patch", k): ("profiles/hbm_traffic_pulse3d.md", name)
+                         for k, name in (("k_grad", "k_grad<3, 4, 1, double>"),
+                                         ("k_patch", "k_patch<3, 1, true, double, 4, true>"))})
 
 
 def committed_traffic(workload, strategy, kernel):
