@@ -815,7 +815,10 @@ struct HaloGeo {
             get(&n32, m * DIM);
             get(&dx32, m * DIM);
         }
-        if (e != hipSuccess) return e;
+        if (e != hipSuccess) {
+            release();
+            return e;
+        }
         nh = nh_;
         const dim3 grid((uint32_t)((nh + BLOCK - 1) / BLOCK));
         hipLaunchKernelGGL((k_halo_geo<DIM, double>), grid, dim3(BLOCK), 0, st, nh, slots, nc, hcell, f64, code, w, n, dx, vol);
