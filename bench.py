@@ -460,7 +460,7 @@ def main():
                             strategies={st: 1e3 * runs_m[st][0] / args.steps for st in runs_m},
                             compare_steps=nsteps, rel_l2_per_field=l2, rel_linf_per_field=linf)
         solver.set_config(M.make_config(**cfg_kw))
-    # the survey's fused-minimum lower bound: per stage each face's two cell indices and normal and
+    # the fused-minimum lower bound: per stage each face's two cell indices and normal and
     # each cell's state in, volume and state out, once (2D: 24 N_f + 72 N_c bytes)
     dim = mesh.dim
     lb = mesh.n_faces * (8 + 8 * dim) + mesh.n_cells * (2 * 8 * (dim + 2) + 8)

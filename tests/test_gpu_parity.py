@@ -1,7 +1,7 @@
 """GPU parity: the HIP residual path (through the C ABI) against the CPU oracle.
 
 Bar: per cell and field |R_gpu - R_ref| <= 1e-12 * S_ref with S = sum_f sum_d |n_d||F_d| (the
-survey's parity metric), and <= 1e-12 relative on updated states.
+flux scale of the cell's own terms), and <= 1e-12 relative on updated states.
 """
 import numpy as np
 import pytest
