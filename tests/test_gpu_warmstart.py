@@ -1,6 +1,6 @@
 """Learned warm start end to end at small size: the solver generates steady ramp solutions in one
 batch, the network is trained on them by the library's kernels, and held-out cases start from its
-prediction.  The freestream start is the judge: both runs must reach the same absolute residual."""
+prediction.  The freestream start sets the target: both runs must reach the same absolute residual."""
 import numpy as np
 import pytest
 
