@@ -6,7 +6,8 @@
 // Features are normalised as (f - in_shift) * in_scale and outputs de-normalised as
 // y * out_scale + out_shift inside the kernels.
 //
-// Kernels: an LDS-tiled FP32 GEMM (64x64 tile, 4x4 outputs per lane) with strided operands and fused
+// Kernels: an LDS-tiled FP32 GEMM on the matrix cores (64x64 block tile, one 32x32 v_mfma_f32_32x32x2_f32
+// accumulator per wave) with strided operands and fused
 // epilogues (bias, tanh, tanh-derivative), split along K with a fixed-order second pass for the
 // weight gradients (deterministic); column sums for the bias gradients; the squared-error gradient;
 // Adam.  A training step (batch gather by a counter-based hash, forward, backward, Adam, step
@@ -34,6 +35,7 @@ namespace {
     } while (0)
 
 constexpr int TM = 64, TN = 64, TK = 16, GT = 256;
+typedef float floatx16 __attribute__((ext_vector_type(16)));
 enum { EPI_NONE = 0, EPI_BIAS = 1, EPI_BIAS_TANH = 2, EPI_DTANH = 3 };
 
 struct GemmArgs {
@@ -50,15 +52,14 @@ template <int EPI>
 __global__ __launch_bounds__(GT) void k_gemm(const GemmArgs g) {
     __shared__ float As[TK][TM + 4];
     __shared__ float Bs[TK][TN + 4];
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;  // the wave's 32 x 32 quarter of the 64 x 64 tile
     const int64_t m0 = (int64_t)blockIdx.y * TM, n0 = (int64_t)blockIdx.x * TN;
     const int64_t k0 = (int64_t)blockIdx.z * g.kchunk;
     const int64_t k1 = k0 + g.kchunk < g.K ? k0 + g.kchunk : g.K;
-    float acc[4][4];
+    floatx16 acc;
 #pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = 0.f;
+    for (int r = 0; r < 16; r++) acc[r] = 0.f;
     for (int64_t kt = k0; kt < k1; kt += TK) {
         // 1024 elements per tile, 4 per lane, zero outside the matrices
 #pragma unroll
@@ -72,38 +73,30 @@ __global__ __launch_bounds__(GT) void k_gemm(const GemmArgs g) {
             Bs[bk][bn] = (gn < g.N && gk2 < k1) ? g.B[gk2 * g.sbk + gn * g.sbn] : 0.f;
         }
         __syncthreads();
+        // v_mfma_f32_32x32x2_f32: A(i, k) from lane i + 32k, B(k, j) from lane j + 32k
 #pragma unroll
-        for (int k = 0; k < TK; k++) {
-            float a[4], b[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) a[i] = As[k][ty * 4 + i];
-#pragma unroll
-            for (int j = 0; j < 4; j++) b[j] = Bs[k][tx * 4 + j];
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+        for (int k = 0; k < TK; k += 2) {
+            const float av = As[k + (lane >> 5)][wm + (lane & 31)];
+            const float bv = Bs[k + (lane >> 5)][wn + (lane & 31)];
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
         }
         __syncthreads();
     }
     float *C = g.C + (int64_t)blockIdx.z * g.cz;
+    // D(i, j) in lane j + 32 ((i / 4) % 2), register (i % 4) + 4 (i / 8)
+    const int64_t n = n0 + wn + (lane & 31);
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int64_t m = m0 + ty * 4 + i;
-        if (m >= g.M) continue;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int64_t n = n0 + tx * 4 + j;
-            if (n >= g.N) continue;
-            float v = acc[i][j];
-            if constexpr (EPI == EPI_BIAS || EPI == EPI_BIAS_TANH) v += g.bias[n];
-            if constexpr (EPI == EPI_BIAS_TANH) v = tanhf(v);
-            if constexpr (EPI == EPI_DTANH) {
-                const float p = g.P[m * g.ldc + n];
-                v *= 1.f - p * p;
-            }
-            C[m * g.ldc + n] = v;
+    for (int r = 0; r < 16; r++) {
+        const int64_t m = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (m >= g.M || n >= g.N) continue;
+        float v = acc[r];
+        if constexpr (EPI == EPI_BIAS || EPI == EPI_BIAS_TANH) v += g.bias[n];
+        if constexpr (EPI == EPI_BIAS_TANH) v = tanhf(v);
+        if constexpr (EPI == EPI_DTANH) {
+            const float p = g.P[m * g.ldc + n];
+            v *= 1.f - p * p;
         }
+        C[m * g.ldc + n] = v;
     }
 }
 
