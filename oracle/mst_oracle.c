@@ -10,14 +10,20 @@
  * Parity status (what pins each function; the tests in tests/test_oracle_*.py hold the checks):
  *   thermo            closed forms (p, H for hand-computed states)
  *   geometry          closed forms (Heron == 1/2|cross|, sum n_out = 0, sum V = domain area)
- *   Roe               consistency F(Q,Q)=F(Q), R*Rinv=I, mirror symmetry, supersonic upwinding,
- *                     Harten-fix continuity
+ *   Roe               consistency F(Q,Q)=F(Q), eigenvectors of the textbook Jacobian, mirror
+ *                     symmetry, supersonic upwinding, Harten-fix continuity
  *   AUSM              split-polynomial identities, F(Q,Q) closed form exposing the a/a~ quirk,
  *                     supersonic upwinding
- *   residual/update   freestream preservation, global conservation, 1D reduction vs a 1D Roe code
+ *   wall/symmetry     ghost from a rotation into the frame of a slanted wall; vanishing mass and
+ *                     energy fluxes on axis-aligned walls; uniform flow along a slanted wall is steady
+ *   residual/update   freestream preservation and global conservation (2D and 3D); the column sums
+ *                     of a y-uniform strip (and the slab sums of an x-only 3D box) equal a textbook
+ *                     1D Roe / AUSM+ finite-volume difference; 3D residual symmetric under axis
+ *                     permutations
  *   limiter (NEW)     textbook psi(r) values, psi=0 for r<=0, bounds of reconstructed values
  *   RK/CFL (NEW)      stage 1 == Euler bit-for-bit; SSP combination identities
  *   physics           Sod exact Riemann solution (L1 convergence), isentropic vortex
+ * Planted mistakes (ORC_PLANT, below) check that these pins catch plausible slips.
  */
 #include "mst_oracle.h"
 #include <math.h>
@@ -29,6 +35,13 @@
 
 #define MAXNV 5
 #define MAXD 3
+
+/* Planted mistakes: tests/test_oracle_planted.py compiles this file with -DORC_PLANT=k into a
+ * temporary library and checks that the pins of tests/test_oracle_*.py fail against it.  The
+ * default build (ORC_PLANT 0) contains none of them. */
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
 
 struct orc_mesh {
     int dim, nv, fnn;
@@ -59,7 +72,11 @@ int orc_set_threads(int n) {
  * (DIM==2).  For dim==3 the third momentum is added (NEW: no reference semantics in 3D). */
 static double momsq(const double *Q, int dim) {
     double s = Q[1] * Q[1] + Q[2] * Q[2];
+#if ORC_PLANT == 4 /* planted: the reference's 2D-only kinetic energy kept in 3D */
+    (void)dim;
+#else
     if (dim == 3) s += Q[3] * Q[3];
+#endif
     return s;
 }
 double orc_pressure(const double *Q, int dim, double gamma) {
@@ -147,7 +164,11 @@ static void roe_R(const roe_avg *ra, int d, int dim, double *Rm) {
     for (int k = 0; k < dim; k++) {
         if (k == d) continue;
         Rm[(1 + k) * nv + col] = 1;
+#if ORC_PLANT == 5 /* planted: shear wave's energy entry from the wrong velocity component */
+        Rm[(nv - 1) * nv + col] = ra->u[d];
+#else
         Rm[(nv - 1) * nv + col] = ra->u[k];
+#endif
         col++;
     }
     /* last column: u_d + a */
@@ -359,7 +380,11 @@ orc_mesh *orc_mesh_create(int dim, int64_t n_nodes, const double *xyz, int64_t n
             const double *p2 = xyz + (int64_t)fnodes[f * fnn + 2] * dim;
             double a[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
             double b[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+#if ORC_PLANT == 6 /* planted: a transposed operand in the cross product's second component */
+            double cr[3] = {a[1] * b[2] - a[2] * b[1], a[0] * b[2] - a[2] * b[0], a[0] * b[1] - a[1] * b[0]};
+#else
             double cr[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+#endif
             double s = fnn == 3 ? 0.5 : 1.0; /* Face.cpp:26 (tri) vs :33 (quad) */
             for (int k = 0; k < 3; k++) m->S[f * 3 + k] = s * cr[k];
         }
@@ -478,7 +503,11 @@ int orc_mesh_geometry(const orc_mesh *m, double *fcen, double *S, double *eta, i
  * (MshBlock.cpp:635-647), so the physical rule is used. */
 static int lr_flag(const orc_mesh *m, int compat, int64_t f, int d) {
     double s = (double)m->dac[f] * m->S[f * m->dim + d];
+#if ORC_PLANT == 8 /* planted: the downwind side as Riemann "left" */
+    int phys = (s < 0);
+#else
     int phys = (s >= 0);
+#endif
     if ((compat & ORC_COMPAT_BRANCH2_LR) && m->dim == 2 && m->dac[f] == 1) return !phys;
     return phys;
 }
@@ -522,8 +551,13 @@ static void face_values(const orc_mesh *m, const orc_config *cfg, const double *
 }
 
 static inline double nout(const orc_mesh *m, int64_t c, int32_t f, int d) {
+#if ORC_PLANT == 7 /* planted: the c1 side keeps c0's outward normal */
+    (void)c;
+    return (double)m->dac[f] * m->S[(int64_t)f * m->dim + d];
+#else
     return (c == m->c0[f]) ? (double)m->dac[f] * m->S[(int64_t)f * m->dim + d]
                            : (double)(-m->dac[f]) * m->S[(int64_t)f * m->dim + d];
+#endif
 }
 
 int orc_gradient(const orc_mesh *m, const orc_config *cfg, const double *Q, double *grad) {
@@ -633,10 +667,21 @@ static void reflect(const orc_mesh *m, int64_t f, const double *src, double *dst
     const double *S = m->S + f * dim;
     double nrm = norm(S, dim);
     double nh[MAXD];
+#if ORC_PLANT == 2 /* planted: the raw area vector instead of the unit normal */
+    (void)nrm;
+    for (int d = 0; d < dim; d++) nh[d] = S[d];
+#else
     for (int d = 0; d < dim; d++) nh[d] = S[d] / nrm;
+#endif
     double s = nh[0] * src[1] + nh[1] * src[2];
     if (dim == 3) s += nh[2] * src[3];
+#if ORC_PLANT == 1 /* planted: the factor 2 dropped (normal momentum zeroed, not negated) */
+    for (int d = 0; d < dim; d++) dst[1 + d] = src[1 + d] - nh[d] * s;
+#elif ORC_PLANT == 3 /* planted: the normal component added instead of subtracted */
+    for (int d = 0; d < dim; d++) dst[1 + d] = src[1 + d] + 2 * nh[d] * s;
+#else
     for (int d = 0; d < dim; d++) dst[1 + d] = src[1 + d] - 2 * nh[d] * s;
+#endif
 }
 
 /* ---------------------------------------------------------------- face flux

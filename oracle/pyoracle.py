@@ -68,7 +68,9 @@ def lib():
     global _LIB
     if _LIB is not None:
         return _LIB
-    path = os.path.join(_HERE, "libmst_oracle.so")
+    # MST_ORACLE_LIB selects another build of the same source (tests/test_oracle_planted.py loads
+    # builds with a planted mistake this way)
+    path = os.environ.get("MST_ORACLE_LIB") or os.path.join(_HERE, "libmst_oracle.so")
     if not os.path.exists(path):
         raise RuntimeError(f"oracle library not built: {path} (run __graft_entry__.build())")
     L = C.CDLL(path)

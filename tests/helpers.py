@@ -86,11 +86,16 @@ def normal_sums(raw, S):
 
 
 def residual_parity_error(Rg, Rr, Sr, floor=None):
-    """max over cells and fields of |R_gpu - R_ref| / (S_ref + floor).
+    """max over cells and fields of |R_gpu - R_ref| / S_ref, with S_ref = sum_f sum_d |n_d| |F_d|
+    (the parity scale).
 
-    S_ref = sum_f sum_d |n_d| |F_d| (the parity scale); floor = flux_scale(Q) * normal_sums, which
-    only matters where every face flux of a field is itself roundoff (uniform regions: S ~ 0)."""
-    den = Sr if floor is None else Sr + floor
+    floor (flux_scale(Q) * normal_sums: the natural flux magnitude of the cell's state through its
+    faces) is added to S_ref only in the cells and fields where S_ref is itself round-off, below 1e-6
+    of it: a field whose every face flux vanishes (the mass and energy fluxes of gas at rest) has no
+    scale of its own.  Everywhere else the bar is relative to S_ref alone."""
+    den = Sr
+    if floor is not None:
+        den = np.where(Sr <= 1e-6 * floor, Sr + floor, Sr)
     den = np.where(den > 0, den, 1.0)
     err = np.abs(Rg - Rr) / den
     return float(err.max())

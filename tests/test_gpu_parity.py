@@ -184,7 +184,7 @@ def test_rk_steps_match_oracle(mname, scheme, order, lim, rk, strategy):
     Qg = s.get_state()
     err = rel_state_error(Qg, Qr)
     print(f"{mname} rk{rk} strategy {strategy}: 10-step state err {err:.2e}", flush=True)
-    assert err <= 1e-11
+    assert err <= TOL
 
 
 def test_branch2_orientation_compat():
