@@ -1,13 +1,22 @@
-"""Benchmark of the MI355X residual path: face-flux updates/s and achieved HBM bandwidth.
+"""Benchmark of the MI355X residual path: face-flux updates/s and achieved HBM bandwidth, and the
+time to a 1e-8 residual drop of the steady ramp, warm against cold.
 
-Default workload: 2D isentropic vortex (strength 5, Mach 0.5 freestream, gamma 1.4) on [-10,10]^2,
-1000x1000 quads split into 2,000,000 jittered triangles (+-0.1 h, seed 12345), periodic on all
-sides; Roe + van Leer, SSP-RK3, CFL 0.5 (device-side time step), FP64.  One "step" is one RK3
-time step = 3 residual evaluations (gradient+limiter, face flux, gather+update each) + the CFL
-reduction.  face-updates/s = N_faces * 3 * steps / wall time, inputs resident in HBM.  Every kernel
-strategy (face, cell, patch, fused) is timed on the same context; `value` is the fastest.
-
-Prints ONE JSON line (see the keys below).  Only one GPU is used: --gpus N>1 exits with a message.
+Default run (one JSON line), every configuration measured in the same process:
+- headline (top-level keys): 2D isentropic vortex (strength 5, Mach 0.5 freestream, gamma 1.4) on
+  [-10,10]^2, 1000x1000 quads split into 2,000,000 jittered triangles (+-0.1 h, seed 12345), periodic
+  on all sides; Roe + van Leer, SSP-RK3, CFL 0.5 (device-side time step), FP64.  One "step" is one RK3
+  time step = 3 residual evaluations (gradient+limiter, face flux, gather+update each) + the CFL
+  reduction.  face-updates/s = N_faces * 3 * steps / wall time, inputs resident in HBM.  Every kernel
+  strategy (face, cell, patch, fused) is timed on the same context; `value` is the fastest.
+- configs.pulse3d: the 3D pulse, 160^3 hexes x 6 jittered tets (24,576,000 cells), AUSM + minmod,
+  SSP-RK2, FP64 and the mixed form (FP32 flux and limiter, FP64 accumulation and update) with their
+  ratio and per-field L2 / Linf differences, the dominant kernel's roofline and a CPU sample;
+- configs.ramp: the steady supersonic compression ramp (600x250x2 = 300,000 triangles): time to a
+  1e-8 drop with the explicit and LU-SGS drivers from the freestream (LU-SGS sweeps' roofline, the
+  oracle's LU-SGS as CPU baseline), and the learned warm start (256 solver-generated training cases, a
+  4 x 128 network trained on the GPU) against the freestream start on held-out cases.
+--headline-only skips the two configs (profiler passes).  Other single workloads: --workload pulse3d,
+steady, ramp, batch.  Only one GPU is used: --gpus N>1 exits with a message.
 """
 import argparse
 import json
@@ -246,7 +255,7 @@ def bench_ramp(M, args):
     print(json.dumps(out))
 
 
-def bench_steady(M, args):
+def bench_steady(M, args, emit=True):
     """Steady supersonic compression ramp (the C5 mesh: 600x250x2 = 300,000 triangles; M 2.51,
     theta 9.3 deg, first-order Roe) from the freestream to 1e-8 x the first-step L2(R/V) of every
     field, with the explicit SSP-RK2 driver (CFL 0.8, device time step) and with the LU-SGS driver
@@ -319,7 +328,9 @@ def bench_steady(M, args):
         out["cpu_baseline"] = lusgs_cpu_baseline(m, mach, s.lusgs_colours()[0], lu["iterations"])
         out["cpu_baseline"].update(host_cpu())
     s.close()
-    print(json.dumps(out))
+    if emit:
+        print(json.dumps(out))
+    return out
 
 
 def lusgs_cpu_baseline(m, mach, colour, iterations, budget_s=10.0):
@@ -346,50 +357,15 @@ def lusgs_cpu_baseline(m, mach, colour, iterations, budget_s=10.0):
                        f"OpenMP residual, sequential sweeps), scaled to the GPU run's {iterations} iterations")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="vortex2d")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--retune", action="store_true", help="batch workload: ignore the tuning cache")
-    ap.add_argument("--strategy", choices=("auto", "face", "cell", "patch", "fused"), default="auto",
-                    help="face: flux kernel + gather/update kernel; cell: fused cell kernel; patch: fused patch "
-                         "kernel (face fluxes of a 256-cell patch in LDS); fused: the patch kernel with the "
-                         "gradients and face states in LDS too; auto: time each and report the fastest")
-    ap.add_argument("--order", choices=("hilbert", "rcm"), default="hilbert",
-                    help="cell order of the device layout (Hilbert-curve patches or reverse Cuthill-McKee)")
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison")
-    ap.add_argument("--max-steps", type=int, default=0,
-                    help="steady workload: cap the to-steady runs at this many steps/iterations (profiler passes; "
-                         "the line then carries a note and no time to 1e-8); 0: no cap")
-    ap.add_argument("--compare-steps", type=int, default=200,
-                    help="steps of the FP64 vs mixed state comparison")
-    ap.add_argument("--calibrate-copy", type=int, default=268435456, metavar="BYTES",
-                    help="first launch one plain copy kernel of BYTES (the achievable-bandwidth reference and the "
-                         "scale for hardware byte counters; 0: none)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); the multi-GPU path is a "
-              f"design (mesh partition + halo exchange), not built in this tree.", file=sys.stderr)
-        sys.exit(2)
-    import mstcfd as M
-
-    M.set_cell_order(M.ORDER_HILBERT if args.order == "hilbert" else M.ORDER_RCM)
-    if args.workload == "batch":
-        return bench_batch(M, args)
-    if args.workload == "ramp":
-        return bench_ramp(M, args)
-    if args.workload == "steady":
-        return bench_steady(M, args)
-    copy = None
-    if args.calibrate_copy:
-        M.copy_probe(args.calibrate_copy)  # first launch pays the code-object load
-        ms = min(M.copy_probe(args.calibrate_copy) for _ in range(3))
-        copy = dict(bytes=args.calibrate_copy, ms=ms, gbs=2 * args.calibrate_copy / (ms * 1e-3) / 1e9)
-    mesh, cfg_kw, desc, init = build_workload(M, args.workload)
+def bench_unsteady(M, args, workload, copy=None, cpu_sample_mesh=None):
+    """One unsteady workload on one context: every strategy timed in FP64 and in the mixed form, the
+    mixed form's state difference after compare_steps steps with the same time steps, the dominant
+    kernel's roofline and the CPU baseline (on cpu_sample_mesh() when given: a smaller mesh of the same
+    construction, for a mesh too large for a bounded CPU sample)."""
+    mesh, cfg_kw, desc, init = build_workload(M, workload)
+    t_setup = time.perf_counter()
     solver = M.Solver(mesh, M.make_config(**cfg_kw))
+    t_setup = time.perf_counter() - t_setup
     Q0 = init()
     stages = cfg_kw["rk"]
     names = {"face": M.STRATEGY_FACE, "cell": M.STRATEGY_CELL, "patch": M.STRATEGY_PATCH, "fused": M.STRATEGY_FUSED}
@@ -405,13 +381,13 @@ def main():
     wall, kern, dom = runs[best]["wall"], runs[best]["kernels"], runs[best]["dominant"]
     solver.strategy = names[best]
     value = mesh.n_faces * stages * args.steps / wall
-    desc = dict(desc, strategy=best, cell_order=args.order)
+    desc = dict(desc, strategy=best, cell_order=args.order, context_setup_s=t_setup)
 
     def roofline(st):
         r = runs[st]
         d = r["kernels"][r["dominant"]]
         ach = d["alg_bytes"] / (d["avg_us"] * 1e-6) / 1e9
-        traffic, src = committed_traffic(args.workload, st, r["dominant"])
+        traffic, src = committed_traffic(workload, st, r["dominant"])
         return dict(bound="hbm", kernel=r["dominant"], achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=ach / HBM_PEAK_GBS, traffic=traffic, traffic_source=src, alg_bytes=d["alg_bytes"])
 
@@ -442,6 +418,7 @@ def main():
         nsteps = args.compare_steps
         fixed = dict(cfg_kw, cfl=0.0)
         solver.set_config(M.make_config(**fixed))
+        solver.strategy = names[best]
         solver.set_state(Q0)
         dts = []
         for _ in range(nsteps):
@@ -450,6 +427,7 @@ def main():
             solver.step(1)
         Qd = solver.get_state()
         solver.set_config(M.make_config(precision=M.PREC_MIXED, **fixed))
+        solver.strategy = names[best_m]
         solver.set_state(Q0)
         for dt in dts:
             solver.set_dt(dt)
@@ -460,21 +438,128 @@ def main():
                             strategies={st: 1e3 * runs_m[st][0] / args.steps for st in runs_m},
                             compare_steps=nsteps, rel_l2_per_field=l2, rel_linf_per_field=linf)
         solver.set_config(M.make_config(**cfg_kw))
+        solver.strategy = names[best]
     # the fused-minimum lower bound: per stage each face's two cell indices and normal and
     # each cell's state in, volume and state out, once (2D: 24 N_f + 72 N_c bytes)
     dim = mesh.dim
     lb = mesh.n_faces * (8 + 8 * dim) + mesh.n_cells * (2 * 8 * (dim + 2) + 8)
     out["fused_minimum"] = dict(bytes_per_stage=lb, achieved_gbs=lb * stages * args.steps / wall / 1e9,
                                 frac=lb * stages * args.steps / wall / 1e9 / HBM_PEAK_GBS)
-    if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(mesh, cfg_kw, Q0, threads)
-        out["cpu_baseline"].update(host_cpu())
-        if threads != 8:  # the reference's own OpenMP team size (NUM_CPU_THREADS, include/CONST.h:7)
-            t8 = cpu_baseline(mesh, cfg_kw, Q0, 8, budget_s=5.0)
-            out["cpu_baseline"]["threads8"] = dict(value=t8["value"], sample=t8["sample"])
     Qf = solver.get_state()
     out["state_finite"] = bool(np.isfinite(Qf).all())
+    solver.close()
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        if cpu_sample_mesh is not None:
+            del Qf
+            cmesh, cinit = cpu_sample_mesh()
+            cQ0 = cinit()
+        else:
+            cmesh, cQ0 = mesh, Q0
+        out["cpu_baseline"] = cpu_baseline(cmesh, cfg_kw, cQ0, threads)
+        out["cpu_baseline"].update(host_cpu())
+        if threads != 8:  # the reference's own OpenMP team size (NUM_CPU_THREADS, include/CONST.h:7)
+            t8 = cpu_baseline(cmesh, cfg_kw, cQ0, 8, budget_s=5.0)
+            out["cpu_baseline"]["threads8"] = dict(value=t8["value"], sample=t8["sample"])
+    return out
+
+
+def pulse_sample_mesh(M):
+    """The pulse workload's construction on 64^3 hexes x 6 tets (1,572,864 cells): the CPU sample of
+    the 24.6M-cell configuration (the oracle's unit is face-updates/s, so a bounded sample of the
+    same mesh construction and scheme measures the same rate)."""
+    bc = (M.BC_INLET, M.BC_OUTLET) + (M.BC_SYMMETRY,) * 4
+    mesh = M.Mesh.gen_tet(64, 64, 64, (0, 0, 0), (1, 1, 1), 0.15, 12345, bc)
+    return mesh, lambda: mesh.init_state(M.INIT_PULSE, [1.4, 0.1, 0.1, 0.5, 0.5, 0.5, 0.8])
+
+
+def ramp_section(M, args, held_out=8):
+    """C5 in the default line: the steady runs of bench_steady (one case, explicit and LU-SGS from the
+    freestream, with the sweeps' roofline and the oracle's LU-SGS as CPU baseline), then the learned
+    warm start: 256 training cases solved in one batch, the network trained on the GPU, `held_out`
+    cases of the 300,000-triangle ramp from the freestream and from the prediction to the same
+    absolute residual level."""
+    from mstcfd import ramp as RP
+    steady = bench_steady(M, args, emit=False)
+    t0 = time.perf_counter()
+    X, T, gen = RP.generate(RP.sample_cases(256, 2024), 300, 125, max_steps=20000, check_every=500)
+    net, tr = RP.train(X, T)
+    rows = RP.evaluate(net, RP.sample_cases(held_out, 2025), 600, 250, max_steps=40000, lusgs=(1e4, 1.0))
+    net.close()
+    sm = RP.summary(rows)
+    lu = steady["drivers"]["lusgs"]
+    return dict(metric="time-to-1e-8 residual drop (L2 of R/V, every field), steady ramp",
+                config=steady["config"], unit="s",
+                cold=dict(explicit_s=steady["drivers"]["explicit"]["seconds"], lusgs_s=lu["seconds"],
+                          explicit_iterations=steady["drivers"]["explicit"]["iterations"],
+                          lusgs_iterations=lu["iterations"], lusgs_speedup_vs_explicit=steady["speedup_vs_explicit"]),
+                roofline=steady["roofline"], kernels=steady["kernels"], cpu_baseline=steady.get("cpu_baseline"),
+                warm_start=dict(value=sm.get("lusgs_geomean_warm_speedup"),
+                                unit="x, freestream time / warm-start time (geometric mean over held-out cases that "
+                                     "converge), LU-SGS driver",
+                                explicit_driver=sm.get("geomean_speedup"), summary=sm,
+                                training=gen, network=dict(tr, layers="4-128-128-128-128-4 tanh"),
+                                held_out=held_out, wall_s=time.perf_counter() - t0, cases=rows))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="vortex2d")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="default workload without the pulse3d and ramp configurations (profiler passes)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--retune", action="store_true", help="batch workload: ignore the tuning cache")
+    ap.add_argument("--strategy", choices=("auto", "face", "cell", "patch", "fused"), default="auto",
+                    help="face: flux kernel + gather/update kernel; cell: fused cell kernel; patch: fused patch "
+                         "kernel (face fluxes of a 256-cell patch in LDS); fused: the patch kernel with the "
+                         "gradients and face states in LDS too; auto: time each and report the fastest")
+    ap.add_argument("--order", choices=("hilbert", "rcm"), default="hilbert",
+                    help="cell order of the device layout (Hilbert-curve patches or reverse Cuthill-McKee)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison")
+    ap.add_argument("--max-steps", type=int, default=0,
+                    help="steady workload: cap the to-steady runs at this many steps/iterations (profiler passes; "
+                         "the line then carries a note and no time to 1e-8); 0: no cap")
+    ap.add_argument("--compare-steps", type=int, default=200,
+                    help="steps of the FP64 vs mixed state comparison")
+    ap.add_argument("--held-out", type=int, default=8, help="ramp cases of the warm-start comparison in the default line")
+    ap.add_argument("--calibrate-copy", type=int, default=268435456, metavar="BYTES",
+                    help="first launch one plain copy kernel of BYTES (the achievable-bandwidth reference and the "
+                         "scale for hardware byte counters; 0: none)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); the multi-GPU path is a "
+              f"design (mesh partition + halo exchange), not built in this tree.", file=sys.stderr)
+        sys.exit(2)
+    import mstcfd as M
+
+    M.set_cell_order(M.ORDER_HILBERT if args.order == "hilbert" else M.ORDER_RCM)
+    if args.workload == "batch":
+        return bench_batch(M, args)
+    if args.workload == "ramp":
+        return bench_ramp(M, args)
+    if args.workload == "steady":
+        return bench_steady(M, args)
+    t0 = time.perf_counter()
+    copy = None
+    if args.calibrate_copy:
+        M.copy_probe(args.calibrate_copy)  # first launch pays the code-object load
+        ms = min(M.copy_probe(args.calibrate_copy) for _ in range(3))
+        copy = dict(bytes=args.calibrate_copy, ms=ms, gbs=2 * args.calibrate_copy / (ms * 1e-3) / 1e9)
+    if args.workload == "pulse3d":
+        print(json.dumps(bench_unsteady(M, args, "pulse3d", copy, lambda: pulse_sample_mesh(M))))
+        return
+    out = bench_unsteady(M, args, args.workload, copy)
+    if args.workload == "vortex2d" and not args.headline_only:
+        print(f"bench: vortex2d done at {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+        c3 = bench_unsteady(M, args, "pulse3d", None, lambda: pulse_sample_mesh(M))
+        print(f"bench: pulse3d done at {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+        c5 = ramp_section(M, args, args.held_out)
+        print(f"bench: ramp done at {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+        out["configs"] = dict(pulse3d=c3, ramp=c5)
+    out["wall_s"] = time.perf_counter() - t0
     print(json.dumps(out))
 
 

@@ -362,8 +362,9 @@ int mst_mlp_set_data(mst_mlp *m, const float *X, const float *T, int64_t n);
 /* zero the Adam moments and the step counter */
 int mst_mlp_reset_optimizer(mst_mlp *m);
 /* steps of Adam (beta 0.9 / 0.999, eps 1e-8) on batches of `batch` samples drawn by a counter-based
- * hash of (step, row) (the whole set in order when batch >= n); use_graph: replay one captured step.
- * loss: mean squared error of the last batch */
+ * hash of (step, row) (the whole set in order when batch >= n); use_graph: replay one captured step
+ * (dropped whenever the training set or a buffer it uses is replaced; launched directly instead
+ * while a rocprofiler-sdk tool is loaded in the process).  loss: mean squared error of the last batch */
 int mst_mlp_train(mst_mlp *m, int steps, int batch, double lr, int use_graph, double *loss);
 /* de-normalised outputs Y [n x n_out] of X [n x n_in] (host arrays) */
 int mst_mlp_forward(mst_mlp *m, const float *X, int64_t n, float *Y);

@@ -16,6 +16,7 @@ struct Params {
     T gamma, gm1, eor, delta, dd, ihd;  // dd = delta^2, ihd = 0.5/delta (Harten fix)
     T astar_c;                          // 2 (gamma-1) / (gamma+1)   (SolverAusm.cpp:6-7)
     int compat;
+    int noslip;  // viscous configuration: walls are no-slip (RhoSolver.cpp:160-163, 300-303)
     T qin[5];
 };
 
@@ -30,6 +31,7 @@ inline Params<T> make_params(const mst_config &c) {
     p.ihd = (T)(0.5 / c.entropy_delta);
     p.astar_c = (T)(2.0 * (c.gamma - 1.0) / (c.gamma + 1.0));
     p.compat = c.compat;
+    p.noslip = (c.mu > 0 || c.kappa > 0) ? 1 : 0;
     for (int k = 0; k < 5; k++) p.qin[k] = (T)c.q_inlet[k];
     return p;
 }

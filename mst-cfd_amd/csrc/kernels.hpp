@@ -114,7 +114,12 @@ __device__ __forceinline__ void face_bc(const Params<T> &P, int kind, const T *n
     if (kind == mst::KIND_WALL) {  // RhoSolver.cpp:142-179 / 285-319
 #pragma unroll
         for (int k = 0; k < NV; k++) A[k] = B[k] = Rc[k];
-        reflect<DIM>(B, A, n);
+        if (P.noslip) {  // viscous wall: the ghost's momentum is the interior's negated (:160-163 / 300-303)
+#pragma unroll
+            for (int d = 0; d < DIM; d++) B[1 + d] = -A[1 + d];
+        } else {
+            reflect<DIM>(B, A, n);
+        }
     } else if (kind == mst::KIND_SYMMETRY) {  // :184-211 / 322-350 (A keeps the raw interior state)
 #pragma unroll
         for (int k = 0; k < NV; k++) {

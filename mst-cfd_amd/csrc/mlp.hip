@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <random>
 #include <string>
@@ -233,6 +234,27 @@ namespace {
 
 float *fptr(float *base, int64_t off) { return base + off; }
 
+// The captured training step holds raw pointers to the training set (X, T), the split buffer and
+// the work buffers, and the row count by value: whatever frees or replaces one of them drops it.
+void drop_graph(mst_mlp *q) {
+    if (q->gexec) (void)hipGraphExecDestroy(q->gexec);
+    q->gexec = nullptr;
+}
+
+// Whether a rocprofiler-sdk tool is loaded into this process.  Under rocprofv3 --kernel-trace the
+// first capture/instantiate/replay of the training step ended in a host segmentation fault inside
+// the HIP runtime, before any kernel ran (cause not found); while a tool is present the training
+// steps are launched directly instead (same kernels, same results).
+bool profiler_tool_loaded() {
+    FILE *f = fopen("/proc/self/maps", "r");
+    if (!f) return false;
+    char line[4096];
+    bool found = false;
+    while (!found && fgets(line, sizeof(line), f)) found = strstr(line, "librocprofiler-sdk.so") != nullptr;
+    fclose(f);
+    return found;
+}
+
 void free_work(mst_mlp *q) {
     for (auto *p : q->H)
         if (p) (void)hipFree(p);
@@ -244,8 +266,7 @@ void free_work(mst_mlp *q) {
     if (q->sq) (void)hipFree(q->sq);
     q->Tb = q->sq = nullptr;
     q->cap = 0;
-    if (q->gexec) (void)hipGraphExecDestroy(q->gexec);
-    q->gexec = nullptr;
+    drop_graph(q);
 }
 
 int ensure_work(mst_mlp *q, int64_t rows) {
@@ -308,6 +329,7 @@ int64_t split_need(const mst_mlp *q, int64_t rows) {
 int ensure_split(mst_mlp *q, int64_t rows) {
     const int64_t need = split_need(q, rows);
     if (need <= q->split_cap) return MST_OK;
+    drop_graph(q);
     if (q->split) (void)hipFree(q->split);
     q->split = nullptr;
     q->split_cap = 0;
@@ -541,6 +563,7 @@ int mst_mlp_set_scaling(mst_mlp *q, const float *in_shift, const float *in_scale
 int mst_mlp_set_data(mst_mlp *q, const float *X, const float *T, int64_t n) {
     if (!q || !X || !T || n < 1) return mst::fail(MST_E_ARG, "bad training set");
     HIPCHK(hipStreamSynchronize(q->stream));
+    drop_graph(q);
     if (q->X) (void)hipFree(q->X);
     if (q->T) (void)hipFree(q->T);
     q->X = q->T = nullptr;
@@ -568,9 +591,9 @@ int mst_mlp_train(mst_mlp *q, int steps, int batch, double lr, int use_graph, do
     int rc = ensure_work(q, batch);
     if (!rc) rc = ensure_split(q, batch);
     if (rc) return rc;
+    if (use_graph && profiler_tool_loaded()) use_graph = 0;
     if (use_graph && (!q->gexec || q->g_batch != batch || q->g_lr != (float)lr)) {
-        if (q->gexec) (void)hipGraphExecDestroy(q->gexec);
-        q->gexec = nullptr;
+        drop_graph(q);
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(q->stream, hipStreamCaptureModeThreadLocal));
         rc = train_step_enqueue(q, batch, (float)lr);

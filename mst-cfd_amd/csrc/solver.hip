@@ -251,6 +251,8 @@ int run_viscous(mst_ctx *x, int isrc) {
     a.nc = x->nc;
     a.nf = x->nf;
     a.inc_code = x->inc_code;
+    a.ell = x->ell;
+    a.finfo = x->finfo;
     a.inc_w = x->inc_w;
     a.inc_n = x->inc_n;
     a.vol = x->cvol;

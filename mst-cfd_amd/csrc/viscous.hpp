@@ -2,7 +2,8 @@
 // reference cannot run as written and never adds to the residual (FLAGVISCID 0, include/CONST.h:14).
 //   Q_f      the gradient kernel's face values (eta0 interpolation of the face's two cells; a
 //            boundary face its cell; the last-interior-face quirk has eta0 = 1)         (:373-378)
-//   W_f      = (u_1..u_dim, T) of Q_f, T = e / cv
+//   W_f      = (u_1..u_dim, T) of Q_f, T = e / cv; on a wall face u = 0 (no-slip, adiabatic: the
+//            cell's T), the wall the viscous branch of the convective boundary stands for (:160-163)
 //   grad W_c = (1/V) sum_f W_f n_out                                                    (:380-391)
 //   grad W_f = eta0 grad W_c0 + (1 - eta0) grad W_c1; boundary: grad W_c0                (:392-399)
 //   tau_id   = mu (du_i/dx_d + du_d/dx_i) + lam (div u) delta_id                         (:401-413)
@@ -19,6 +20,8 @@ namespace mstk {
 struct ViscArgs {
     int64_t nc, nf;
     const int32_t *__restrict__ inc_code;  // (neighbour << 1) | is_c1, -1 boundary, -2 empty
+    const int32_t *__restrict__ ell;       // the same slots' (face << 1) | is_c1
+    const int32_t *__restrict__ finfo;     // kind | (L/R mask << 4) per face
     const double *__restrict__ inc_w;      // eta0 of the face
     const double *__restrict__ inc_n;      // the cell's outward normal
     const double *__restrict__ vol;
@@ -72,6 +75,10 @@ __global__ __launch_bounds__(BLOCK) void k_visc_grad(const ViscArgs a) {
         }
         double w[NW];
         visc_w<DIM>(qf, a.cv, w);
+        if (code == -1 && (a.finfo[a.ell[i] >> 1] & 15) == mst::KIND_WALL) {
+#pragma unroll
+            for (int d = 0; d < DIM; d++) w[d] = 0;
+        }
 #pragma unroll
         for (int k = 0; k < NW; k++)
 #pragma unroll
@@ -113,6 +120,10 @@ __global__ __launch_bounds__(BLOCK) void k_visc_flux(const ViscArgs a) {
     }
     double w[NW];
     visc_w<DIM>(qf, a.cv, w);
+    if (c1 < 0 && (a.finfo[f] & 15) == mst::KIND_WALL) {
+#pragma unroll
+        for (int d = 0; d < DIM; d++) w[d] = 0;
+    }
     double div = 0;
 #pragma unroll
     for (int d = 0; d < DIM; d++) div += g[d][d];

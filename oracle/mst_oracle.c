@@ -721,7 +721,14 @@ static void face_flux_core(const orc_mesh *m, const orc_config *cfg, const doubl
             if (second) recon(m, Q, grad, phi, c, f, in);
             else memcpy(in, qc, sizeof(double) * nv);
             memcpy(gh, in, sizeof(double) * nv);
+#if ORC_PLANT == 10 /* planted: the inviscid mirror ghost kept on a viscous wall */
             reflect(m, f, in, gh);
+#else
+            if (cfg->mu > 0 || cfg->kappa > 0) /* FLAGVISCID wall: momentum negated (:160-163, :300-303) */
+                for (int d = 0; d < dim; d++) gh[1 + d] = -in[1 + d];
+            else
+                reflect(m, f, in, gh);
+#endif
         } else if (t == ORC_SYMMETRY) {
             memcpy(in, qc, sizeof(double) * nv); /* :325 interior side not reconstructed */
             if (second) recon(m, Q, grad, phi, c, f, gh);
@@ -808,7 +815,9 @@ static void gather(const orc_mesh *m, const double *F, double *R, double *Sabs) 
  * gradient array by cell id, :398; the heat flux takes one matrix element, :418), and the residual
  * never uses it (FLAGVISCID 0, include/CONST.h:14).  Restated as intended:
  *   Q_f      as the gradient's face values (eta0 interpolation; boundary: the cell; :373-378)
- *   W_f      = (u_1..u_dim, T) of Q_f, T = e / cv with e the internal energy per mass
+ *   W_f      = (u_1..u_dim, T) of Q_f, T = e / cv with e the internal energy per mass; on a wall face
+ *              u = 0 and T the cell's (no-slip, adiabatic): the wall of the reference's viscous
+ *              branch, whose convective ghost negates the momentum (:160-163, :300-303)
  *   grad W_c = (1/V) sum_f W_f n_out                                              (:380-391)
  *   grad W_f = eta0 grad W_c0 + (1 - eta0) grad W_c1; boundary: grad W_c0          (:392-399)
  *   tau_id   = mu (du_i/dx_d + du_d/dx_i) + lam (div u) delta_id                   (:401-413)
@@ -839,6 +848,10 @@ int orc_viscous_flux(const orc_mesh *m, const orc_config *cfg, const double *Q, 
             const int32_t f = m->cface[j];
             double w[MAXD + 1];
             prim_w(Qf + (int64_t)f * nv, dim, cfg->cv, w);
+#if ORC_PLANT != 9 /* planted (9): a slip wall in the viscous gradient */
+            if (m->c1[f] < 0 && m->ftype[f] == ORC_WALL)
+                for (int d = 0; d < dim; d++) w[d] = 0;
+#endif
             for (int k = 0; k < nw; k++)
                 for (int d = 0; d < dim; d++) g[k * dim + d] += w[k] * nout(m, c, f, d);
         }
@@ -857,6 +870,8 @@ int orc_viscous_flux(const orc_mesh *m, const orc_config *cfg, const double *Q, 
         for (int k = 0; k < nw * dim; k++) gf[k] = o >= 0 ? e * g0[k] + (1 - e) * gw[o * nw * dim + k] : g0[k];
         double w[MAXD + 1];
         prim_w(Qf + f * nv, dim, cfg->cv, w);
+        if (m->c1[f] < 0 && m->ftype[f] == ORC_WALL)
+            for (int d = 0; d < dim; d++) w[d] = 0;
         double div = 0;
         for (int d = 0; d < dim; d++) div += gf[d * dim + d];
         for (int d = 0; d < dim; d++) {

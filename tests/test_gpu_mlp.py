@@ -73,6 +73,28 @@ def test_full_batch_adam_matches_cpu_reference_and_graph_replay():
     assert rel <= 2e-3
 
 
+def test_graph_is_rebuilt_when_its_buffers_change():
+    """The captured training step points at the training set and the split buffer: a forward pass, a
+    larger gradient call (which grows the split buffer) and a new training set between training
+    calls must not leave it replaying into freed memory.  Graph replay and direct launches of the
+    same call sequence end on the same parameters bit for bit."""
+    X1, T1 = data(6000, seed=7)
+    X2, T2 = data(9000, seed=8)
+    out = []
+    for graph in (False, True):
+        net, _ = make(seed=9)
+        net.forward(data(200000, seed=10)[0])
+        net.set_data(X1, T1)
+        net.train(10, batch=4096, lr=1e-3, graph=graph)
+        net.gradient(*data(100000, seed=11))
+        net.train(10, batch=4096, lr=1e-3, graph=graph)
+        net.set_data(X2, T2)
+        net.train(10, batch=4096, lr=1e-3, graph=graph)
+        out.append(net.get_params())
+        net.close()
+    assert np.array_equal(out[0], out[1])
+
+
 def test_training_fits_a_smooth_field():
     X, T = data(200000, seed=7)
     net, _ = make(seed=8)

@@ -16,6 +16,7 @@ CFLAGS = ["-O2", "-std=c99", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wno-unk
 
 PINS = "tests/test_oracle_pins.py"
 KAT = "tests/test_oracle_kat.py"
+VISC = "tests/test_oracle_viscous.py"
 # planted mistake -> (what it is, the pins that must catch it)
 PLANTS = {
     1: ("wall ghost: factor 2 dropped", [f"{PINS}::test_wall_ghost_on_slanted_walls",
@@ -34,6 +35,8 @@ PLANTS = {
                                          f"{KAT}::test_freestream_preservation_and_conservation"]),
     8: ("Riemann sides swapped (downwind)", [f"{PINS}::test_one_dimensional_reduction_on_a_strip",
                                              f"{PINS}::test_3d_one_dimensional_reduction"]),
+    9: ("viscous gradient: slip wall", [f"{VISC}::test_noslip_wall_closed_forms"]),
+    10: ("viscous wall: inviscid mirror ghost", [f"{VISC}::test_noslip_wall_closed_forms"]),
 }
 
 

@@ -96,20 +96,67 @@ def test_residual_is_inviscid_minus_gathered_viscous_flux(kind):
     n = int(raw["n_cells"])
     Q = cons(rng.uniform(0.8, 1.2, n), rng.uniform(-0.3, 0.3, (n, dim)), rng.uniform(0.8, 1.2, n))
     kw = dict(dim=dim, order=2, limiter=O.LIM_VANLEER)
-    R0, _ = om.residual(O.make_config(**kw), Q)
     c = cfg(**kw)
     R1, _ = om.residual(c, Q)
     Fv, _ = om.viscous_flux(c, Q)
     S = geo["S"] * geo["dac"][:, None]
-    Rv = np.zeros_like(R0)
-    for f in range(len(raw["c0"])):
-        g = np.einsum("d,dk->k", S[f], Fv[f])
-        Rv[raw["c0"][f]] += g
-        if raw["c1"][f] >= 0:
-            Rv[raw["c1"][f]] -= g
+
+    def gather(F):
+        R = np.zeros((n, dim + 2))
+        for f in range(len(raw["c0"])):
+            g = np.einsum("d,dk->k", S[f], F[f])
+            R[raw["c0"][f]] += g
+            if raw["c1"][f] >= 0:
+                R[raw["c1"][f]] -= g
+        return R
+
+    # the convective part of a viscous configuration (its walls are no-slip), gathered in file order
+    R0 = gather(om.face_flux(c, Q))
+    Rv = gather(Fv)
     scale = np.abs(Rv).max()
     assert scale > 0
     assert np.abs(R1 - (R0 - Rv)).max() <= 1e-12 * max(scale, np.abs(R0).max())
+
+
+def test_noslip_wall_closed_forms():
+    """A linear shear u = a y over a wall at y = 0 (periodic in x, outlet on top), regular mesh.  On a
+    no-slip wall the face velocity is zero, which a linear profile through the wall also gives, so the
+    Green-Gauss gradient stays exact in the wall cells: du/dy = a.  The wall face carries the shear
+    stress mu a in x-momentum, no normal stress and no work (the face temperature is the cell's).  Its
+    convective flux pairs the cell with a ghost of negated momentum (rhoSolver/RhoSolver.cpp:160-163),
+    not the inviscid mirror image."""
+    a = 0.6
+    m = M.Mesh.gen_tri(12, 10, 0.0, 1.2, 0.0, 1.0, 0.0, 1, (M.BC_PERIODIC, M.BC_PERIODIC, M.BC_WALL, M.BC_OUTLET))
+    raw = m.raw()
+    om = O.OracleMesh(raw)
+    geo = om.geometry()
+    xc = geo["ccen"]
+    n = len(xc)
+    Q = cons(np.full(n, 1.1), np.column_stack([a * xc[:, 1], np.zeros(n)]), np.full(n, 0.9))
+    c = cfg(compat=O.COMPAT_REFERENCE & ~O.COMPAT_GRAD_LASTINT)  # the quirk gives one face a one-sided value
+    Fv, gw = om.viscous_flux(c, Q)
+    top = np.zeros(n, bool)
+    top[raw["c0"][raw["ftype"] == M.BC_OUTLET]] = True
+    wall_faces = np.nonzero(raw["ftype"] == M.BC_WALL)[0]
+    wall_cells = raw["c0"][wall_faces]
+    expect = np.array([[0.0, a], [0.0, 0.0]])
+    assert np.allclose(gw[~top, :2, :], expect[None], rtol=0, atol=1e-12)
+    assert np.allclose(gw[wall_cells, :2, :], expect[None], rtol=0, atol=1e-12)
+    for f in wall_faces:
+        # energy: no work at the wall (u_f = 0), only the conduction of the cell's T gradient
+        cw = raw["c0"][f]
+        assert np.allclose(Fv[f, 1], [0.0, MU * a, 0.0, KAPPA * gw[cw, 2, 1]], rtol=0, atol=1e-13), Fv[f, 1]
+    # convective wall flux: Riemann problem of the cell and its negated-momentum ghost
+    F = om.face_flux(c, Q)
+    flags = om.lrflags(c.compat)
+    for f in wall_faces:
+        q = Q[raw["c0"][f]]
+        g = q.copy()
+        g[1:3] = -q[1:3]
+        for d in range(2):
+            L, R = (q, g) if flags[f, d] else (g, q)
+            Fe = O.roe_flux(L, R, d, c)
+            assert np.abs(F[f, d] - Fe).max() <= 1e-14 * np.abs(Fe).max()
 
 
 def shear_wave(om, ccen, mu, steps, dt, order=2):
