@@ -275,6 +275,16 @@ int mst_compute_dt(mst_ctx *ctx, double cfl, double *dt);
 int mst_residual_norms(mst_ctx *ctx, double *out);
 /* Time::goNextTimeStep monitor (time/Time.cpp:69-76): max_i |New-Old|/Old per field */
 int mst_monitor_maxrel(mst_ctx *ctx, double *out);
+/* NEW, no counterpart in the reference: the same monitor over caller-owned DEVICE arrays (cell-major,
+ * nv doubles per cell; 16-byte aligned when nv == 4), writing its nv maxima to the device array out,
+ * stream-ordered on stream (a hipStream_t; NULL: the default stream), without synchronising.  For
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops): it calls no
+ * allocator; its scratch is a fixed 80 KiB array per device (2048 blocks x 5 partials), held until the
+ * process ends.  n_cells up to MST_NOALLOC_MAX_CELLS.  Calls that overlap on one device race on that
+ * scratch. */
+#define MST_NOALLOC_MAX_CELLS ((int64_t)1 << 40)
+int mst_monitor_maxrel_noalloc(const double *qnew, const double *qold, int64_t n_cells, int nv, double *out,
+                               void *stream);
 
 /* ---------------------------------------------------------------- inspection (tests) */
 /* residual R_i = sum_f sum_d n_out,d F_f[:,d] (rhoSolver/RhoSolver.cpp:51-55) of the Old state, file order */

@@ -1040,16 +1040,16 @@ __global__ __launch_bounds__(BLOCK) void k_cfl(int64_t nc, const double *__restr
 }
 
 // Time::goNextTimeStep monitor (time/Time.cpp:69-76): max |new-old|/old, values that do not compare
-// greater are skipped
+// greater are skipped.  Block b strides over the cells b*BLOCK + i*gridDim.x*BLOCK, so a grid smaller
+// than the cell count works too; part receives one partial per block and field.
 template <int NV>
 __global__ __launch_bounds__(BLOCK) void k_maxrel(int64_t nc, const double *__restrict__ Qnew,
                                                   const double *__restrict__ Qold, double *__restrict__ part) {
     const int64_t blk = block_id();
-    const int64_t c = blk * BLOCK + threadIdx.x;
     double v[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) v[k] = 0;
-    if (c < nc) {
+    for (int64_t c = blk * BLOCK + threadIdx.x; c < nc; c += (int64_t)gridDim.x * BLOCK) {
         double a[NV], b[NV];
         ldq<NV>(Qnew + c * NV, a);
         ldq<NV>(Qold + c * NV, b);

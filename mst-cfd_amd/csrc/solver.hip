@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1009,6 +1010,36 @@ int resolve_strategy(mst_ctx *x) {
 #define CTX_GUARD(x)                                           \
     if (!(x)) return mst::fail(MST_E_ARG, "null context");
 
+// ------------------------------------------------------------------ allocation-free monitor
+// The monitor over caller-owned device states, for callers that cannot allocate.  Its only scratch,
+// one partial maximum per block and field, lives in the file-scope array below: NOALLOC_GRID blocks
+// stride over any number of cells, so the array never grows.  Each device holds its own copy; its
+// address is looked up once per device (hipGetSymbolAddress, with that device current) and kept.
+// Every block writes its partials before the finish reads them, so nothing needs clearing between
+// calls; calls that overlap on one device share the array and race.
+namespace {
+constexpr int NOALLOC_GRID = 2048;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+__device__ __align__(256) double scratch[NOALLOC_GRID * 5];
+std::mutex noalloc_mu;
+double *noalloc_scratch[NOALLOC_MAX_DEVICES];
+
+int noalloc_scratch_ptr(double **out) {
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return mst::fail(MST_E_UNSUPPORTED, "device index out of range");
+    std::lock_guard<std::mutex> g(noalloc_mu);
+    if (!noalloc_scratch[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return mst::fail(MST_E_HIP, "hipGetSymbolAddress returned no address for the monitor scratch");
+        noalloc_scratch[dev] = static_cast<double *>(p);
+    }
+    *out = noalloc_scratch[dev];
+    return MST_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int mst_ctx_create(const mst_mesh *m, const mst_config *cfg, mst_ctx **out) {
@@ -1359,6 +1390,40 @@ int mst_monitor_maxrel(mst_ctx *ctx, double *out) {
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     HIPCHK(hipMemcpy(out, ctx->scal + 6, sizeof(double) * ctx->nv, hipMemcpyDeviceToHost));
+    return MST_OK;
+}
+
+int mst_monitor_maxrel_noalloc(const double *qnew, const double *qold, int64_t n_cells, int nv, double *out,
+                               void *stream) {
+    if (!qnew || !qold || !out) return mst::fail(MST_E_ARG, "null device pointer");
+    if (nv != 4 && nv != 5) return mst::fail(MST_E_ARG, "nv must be 4 (2D) or 5 (3D)");
+    if (n_cells < 1 || n_cells > MST_NOALLOC_MAX_CELLS)
+        return mst::fail(MST_E_ARG, "n_cells must be in [1, MST_NOALLOC_MAX_CELLS]");
+    if (nv == 4 && (((uintptr_t)qnew | (uintptr_t)qold) & 15u))
+        return mst::fail(MST_E_ARG, "2D states must be 16-byte aligned (records are loaded as 16-byte pairs)");
+    double *part = nullptr;
+    int rc = noalloc_scratch_ptr(&part);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int64_t grid = std::min<int64_t>((n_cells + BLOCK - 1) / BLOCK, NOALLOC_GRID);
+    if (nv == 4)
+        hipLaunchKernelGGL(k_maxrel<4>, dim3((uint32_t)grid), dim3(BLOCK), 0, st, n_cells, qnew, qold, part);
+    else
+        hipLaunchKernelGGL(k_maxrel<5>, dim3((uint32_t)grid), dim3(BLOCK), 0, st, n_cells, qnew, qold, part);
+    HIPCHK(hipGetLastError());
+    FinArgs f;
+    memset(&f, 0, sizeof(f));
+    f.nblk = grid;  // <= 2 * FIN_CHUNK: one finish block
+    f.chunk = grid;
+    f.count = 1.0;
+    f.n = nv;
+    for (int k = 0; k < nv; k++) {
+        f.src[k] = part + k * grid;
+        f.kind[k] = 2;
+        f.dst[k] = out + k;
+    }
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(FIN), 0, st, f);
+    HIPCHK(hipGetLastError());
     return MST_OK;
 }
 

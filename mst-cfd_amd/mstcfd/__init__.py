@@ -53,7 +53,7 @@ EXPORTS = [
     "mst_mesh_cell_centres", "mst_batch_set_strategy", "mst_batch_get_strategy", "mst_set_cell_order",
     "mst_get_cell_order", "mst_set_steady_driver", "mst_lusgs_step", "mst_lusgs_colours", "mst_prep_cell_colours",
     "mst_write_tecplot", "mst_checkpoint_write", "mst_checkpoint_read", "mst_batch_set_steady_driver",
-    "mst_batch_lusgs_step", "mst_mesh_write_cache", "mst_mesh_read_cache",
+    "mst_batch_lusgs_step", "mst_mesh_write_cache", "mst_mesh_read_cache", "mst_monitor_maxrel_noalloc",
 ]
 
 
@@ -158,6 +158,7 @@ def lib():
         "mst_compute_dt": (C.c_int, [vp, C.c_double, C.POINTER(C.c_double)]),
         "mst_residual_norms": (C.c_int, [vp, F64P]),
         "mst_monitor_maxrel": (C.c_int, [vp, F64P]),
+        "mst_monitor_maxrel_noalloc": (C.c_int, [vp, vp, C.c_int64, C.c_int, vp, vp]),
         "mst_get_residual": (C.c_int, [vp, F64P, C.c_int64]),
         "mst_get_face_flux": (C.c_int, [vp, F64P, C.c_int64]),
         "mst_get_gradient": (C.c_int, [vp, F64P, F64P, C.c_int64]),
