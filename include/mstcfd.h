@@ -378,6 +378,17 @@ int mst_mlp_reset_optimizer(mst_mlp *m);
 int mst_mlp_train(mst_mlp *m, int steps, int batch, double lr, int use_graph, double *loss);
 /* de-normalised outputs Y [n x n_out] of X [n x n_in] (host arrays) */
 int mst_mlp_forward(mst_mlp *m, const float *X, int64_t n, float *Y);
+/* per-dispatch timing of the network's kernels (start/stop HIP events carried by each launch; training
+ * then launches directly instead of replaying its graph): count, summed milliseconds and summed
+ * floating-point operations (GEMMs: 2 M N K) per kernel class */
+#define MST_MLP_K_GEMM_FWD 0   /* forward GEMMs (bias + tanh epilogues) */
+#define MST_MLP_K_GEMM_WGRAD 1 /* weight-gradient GEMMs (batch split) */
+#define MST_MLP_K_GEMM_DGRAD 2 /* activation-gradient GEMMs (tanh' epilogue) */
+#define MST_MLP_K_OTHER 3      /* batch gather, loss gradient, split sums, bias gradients, Adam */
+#define MST_MLP_K_COUNT 4
+int mst_mlp_timing_enable(mst_mlp *m, int on);
+int mst_mlp_timing_get(mst_mlp *m, int kclass, int64_t *count, double *total_ms, double *total_flop);
+int mst_mlp_timing_reset(mst_mlp *m);
 /* gradient of the mean squared error over (X, T) w.r.t. the parameters, and the loss (host arrays) */
 int mst_mlp_gradient(mst_mlp *m, const float *X, const float *T, int64_t n, float *grad, double *loss);
 /* Old state := decoded network prediction at the cell centres.  Features: (x_c (dim), cond[0..n_cond)),

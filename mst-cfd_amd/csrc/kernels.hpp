@@ -228,6 +228,10 @@ struct IncView {
     int64_t stride;
 };
 
+// Every memory read of the cell is issued before the first is used: the slots' codes, weights,
+// normals and offsets and the volume (independent of each other), then all the neighbour states (an
+// empty or boundary slot loads the cell's own state, unused).  That is two dependent load rounds per
+// cell instead of one or two per slot, with the same arithmetic in the same slot order.
 template <int DIM, int MAXF, int LIM, typename T>
 __device__ __forceinline__ int grad_limited(const T *__restrict__ Q, const IncView<T> &a, const int64_t g,
                                            const int64_t c, T (&qc)[DIM + 2], T (&lgr)[(DIM + 2) * DIM],
@@ -235,7 +239,27 @@ __device__ __forceinline__ int grad_limited(const T *__restrict__ Q, const IncVi
     constexpr int NV = DIM + 2;
     constexpr bool F32 = std::is_same<T, float>::value;
     T qmax[NV], qmin[NV];
+    int32_t code[MAXF];
+    T eta[MAXF], n[MAXF][DIM];
+#pragma unroll
+    for (int s = 0; s < MAXF; s++) {
+        const int64_t i = (int64_t)s * a.stride + g;
+        code[s] = a.code[i];
+        eta[s] = a.w[i];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            n[s][d] = a.n[i * DIM + d];
+            dxs[s][d] = a.dx[i * DIM + d];
+        }
+    }
+    const double vol = a.vol[g];
     ldq<NV>(Q + c * NV, qc);
+    T qn[MAXF][NV];
+#pragma unroll
+    for (int s = 0; s < MAXF; s++) {
+        const int64_t nb = code[s] >= 0 ? (int64_t)(code[s] >> 1) : c;
+        ldq<NV>(Q + nb * NV, qn[s]);
+    }
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         qmax[k] = qmin[k] = qc[k];
@@ -245,33 +269,23 @@ __device__ __forceinline__ int grad_limited(const T *__restrict__ Q, const IncVi
     int nfc = 0;
 #pragma unroll
     for (int s = 0; s < MAXF; s++) {
-        const int64_t i = (int64_t)s * a.stride + g;
-        const int32_t code = a.code[i];
-        if (code == -2) break;
-        const int64_t nb = code >= 0 ? (int64_t)(code >> 1) : -1;
-        const bool is1 = code >= 0 && (code & 1);
-        const T eta = a.w[i];
-        T n[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            n[d] = a.n[i * DIM + d];
-            dxs[s][d] = a.dx[i * DIM + d];
-        }
+        if (code[s] == -2) continue;  // empty slots are the last ones of the cell
+        const bool interior = code[s] >= 0;
+        const bool is1 = interior && (code[s] & 1);
         T qf[NV];
-        if (nb >= 0) {
-            T qn[NV];
-            ldq<NV>(Q + nb * NV, qn);
+        if (interior) {
             if constexpr (F32) {
 #pragma unroll
-                for (int k = 0; k < NV; k++) qf[k] = eta * (qn[k] - qc[k]);
+                for (int k = 0; k < NV; k++) qf[k] = eta[s] * (qn[s][k] - qc[k]);
             } else {
 #pragma unroll
-                for (int k = 0; k < NV; k++) qf[k] = is1 ? eta * qn[k] + (1 - eta) * qc[k] : eta * qc[k] + (1 - eta) * qn[k];
+                for (int k = 0; k < NV; k++)
+                    qf[k] = is1 ? eta[s] * qn[s][k] + (1 - eta[s]) * qc[k] : eta[s] * qc[k] + (1 - eta[s]) * qn[s][k];
             }
 #pragma unroll
             for (int k = 0; k < NV; k++) {
-                qmax[k] = fmax(qmax[k], qn[k]);
-                qmin[k] = fmin(qmin[k], qn[k]);
+                qmax[k] = fmax(qmax[k], qn[s][k]);
+                qmin[k] = fmin(qmin[k], qn[s][k]);
             }
         } else {
 #pragma unroll
@@ -280,10 +294,10 @@ __device__ __forceinline__ int grad_limited(const T *__restrict__ Q, const IncVi
 #pragma unroll
         for (int k = 0; k < NV; k++)
 #pragma unroll
-            for (int d = 0; d < DIM; d++) t[k][d] += qf[k] * n[d];
+            for (int d = 0; d < DIM; d++) t[k][d] += qf[k] * n[s][d];
         nfc = s + 1;
     }
-    const T iV = (T)frcp(a.vol[g]);
+    const T iV = (T)frcp(vol);
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         ph[k] = 1;
@@ -872,14 +886,13 @@ struct FusedLds {
 
 // Resident waves per SIMD requested for the fused kernel, where the bound costs no more than a register
 // of spill: FP64 triangles with Roe or unlimited AUSM 4 (at most 128 VGPRs; the compiler would keep 3
-// and the kernel, bound by its dependent load rounds, runs about 15 % slower), FP32 triangles 7 (at
-// most 72 VGPRs with one register spilled; 5 % faster than 6, which the kernel's LDS also allows
-// since its block reductions reuse the dynamic LDS), other 2D FP32 cells 4.  Elsewhere the bound
-// would spill, so the compiler chooses.
+// and the kernel, bound by its dependent load rounds, runs about 15 % slower), FP32 triangles 6 (at
+// most 80 VGPRs; with the gradient's loads all issued up front, 7 waves spill 12 registers and run
+// 7 % slower), other 2D FP32 cells 4.  Elsewhere the bound would spill, so the compiler chooses.
 template <int DIM, int SCHEME, int LIM, typename T, int MAXF>
 constexpr int fused_waves() {
     if constexpr (DIM != 2) return 1;
-    if constexpr (sizeof(T) == 4) return MAXF == 3 ? (MST_FUSED_WAVES32 > 0 ? MST_FUSED_WAVES32 : 7) : 4;
+    if constexpr (sizeof(T) == 4) return MAXF == 3 ? (MST_FUSED_WAVES32 > 0 ? MST_FUSED_WAVES32 : 6) : 4;
     return ((SCHEME == MST_SCHEME_ROE || LIM == MST_LIM_NONE) && MAXF == 3) ? 4 : 1;
 }
 

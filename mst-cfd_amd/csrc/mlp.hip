@@ -13,6 +13,7 @@
 // Adam.  A training step (batch gather by a counter-based hash, forward, backward, Adam, step
 // counter) reads its step number from device memory, so it is captured once into a hipGraph and
 // replayed.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -228,11 +229,45 @@ struct mst_mlp {
     hipGraphExec_t gexec = nullptr;
     int g_batch = 0;
     float g_lr = 0;
+    // per-dispatch timing (mst_mlp_timing_*): start/stop events carried by each launch of a class,
+    // and the class's floating-point operations per launch (GEMMs: 2 M N K)
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[MST_MLP_K_COUNT];
+    std::vector<double> flops[MST_MLP_K_COUNT];
+    std::vector<hipEvent_t> pool;
 };
 
 namespace {
 
 float *fptr(float *base, int64_t off) { return base + off; }
+
+hipEvent_t take_event(mst_mlp *q) {
+    if (!q->pool.empty()) {
+        hipEvent_t e = q->pool.back();
+        q->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    return hipEventCreate(&e) == hipSuccess ? e : nullptr;
+}
+
+// one launch on the network's stream; with timing on, the dispatch carries start/stop events of
+// class kclass (no marker packets between the kernels)
+template <typename K, typename... Args>
+int mlp_launch(mst_mlp *q, int kclass, double flop, K kernel, dim3 grid, dim3 block, Args... args) {
+    if (q->timing) {
+        hipEvent_t e0 = take_event(q), e1 = take_event(q);
+        if (!e0 || !e1) return mst::fail(MST_E_HIP, "hipEventCreate failed");
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, q->stream, e0, e1, 0, args...);
+        HIPCHK(hipGetLastError());
+        q->ev[kclass].push_back({e0, e1});
+        q->flops[kclass].push_back(flop);
+        return MST_OK;
+    }
+    hipLaunchKernelGGL(kernel, grid, block, 0, q->stream, args...);
+    HIPCHK(hipGetLastError());
+    return MST_OK;
+}
 
 // The captured training step holds raw pointers to the training set (X, T), the split buffer and
 // the work buffers, and the row count by value: whatever frees or replaces one of them drops it.
@@ -286,12 +321,12 @@ int ensure_work(mst_mlp *q, int64_t rows) {
 // C = A B (+ epilogue); with kchunk < K the K range is split over gridDim.z blocks writing
 // C + z*cz (returned in *nz)
 template <int EPI>
-int gemm(mst_mlp *q, GemmArgs a, int64_t kchunk = 0, int *nz = nullptr) {
+int gemm(mst_mlp *q, int kclass, GemmArgs a, int64_t kchunk = 0, int *nz = nullptr) {
     a.kchunk = kchunk > 0 ? kchunk : (a.K > 0 ? a.K : 1);
     const int64_t z = (a.K + a.kchunk - 1) / a.kchunk;
     dim3 grid((unsigned)((a.N + TN - 1) / TN), (unsigned)((a.M + TM - 1) / TM), (unsigned)(z > 0 ? z : 1));
-    hipLaunchKernelGGL(k_gemm<EPI>, grid, dim3(GT), 0, q->stream, a);
-    HIPCHK(hipGetLastError());
+    int rc = mlp_launch(q, kclass, 2.0 * (double)a.M * (double)a.N * (double)a.K, k_gemm<EPI>, grid, dim3(GT), a);
+    if (rc) return rc;
     if (nz) *nz = (int)(z > 0 ? z : 1);
     return MST_OK;
 }
@@ -313,7 +348,7 @@ int forward(mst_mlp *q, int64_t rows) {
         a.sbn = q->din[l];
         a.ldc = q->dout[l];
         a.bias = q->p + q->boff[l];
-        int rc = l + 1 < q->L ? gemm<EPI_BIAS_TANH>(q, a) : gemm<EPI_BIAS>(q, a);
+        int rc = l + 1 < q->L ? gemm<EPI_BIAS_TANH>(q, MST_MLP_K_GEMM_FWD, a) : gemm<EPI_BIAS>(q, MST_MLP_K_GEMM_FWD, a);
         if (rc) return rc;
     }
     return MST_OK;
@@ -341,10 +376,10 @@ int ensure_split(mst_mlp *q, int64_t rows) {
 // gradient of the mean squared error over rows (targets in Tb) into q->g; per-element squares in sq
 int backward(mst_mlp *q, int64_t rows) {
     const int64_t ny = rows * q->n_out;
-    hipLaunchKernelGGL(k_loss_grad, dim3(nblk(ny)), dim3(GT), 0, q->stream, ny, q->n_out, (const float *)q->H[q->L],
-                       (const float *)q->Tb, (const float *)(q->scal + 2 * q->n_in),
-                       (const float *)(q->scal + 2 * q->n_in + q->n_out), 1.f / (float)ny, q->D[q->L], q->sq);
-    HIPCHK(hipGetLastError());
+    int rc0 = mlp_launch(q, MST_MLP_K_OTHER, 0.0, k_loss_grad, dim3(nblk(ny)), dim3(GT), ny, q->n_out,
+                         (const float *)q->H[q->L], (const float *)q->Tb, (const float *)(q->scal + 2 * q->n_in),
+                         (const float *)(q->scal + 2 * q->n_in + q->n_out), 1.f / (float)ny, q->D[q->L], q->sq);
+    if (rc0) return rc0;
     const int64_t kchunk = split_chunk(rows);
     if (split_need(q, rows) > q->split_cap) return mst::fail(MST_E_ARG, "mlp: split buffer too small");
     for (int l = q->L - 1; l >= 0; l--) {
@@ -364,15 +399,15 @@ int backward(mst_mlp *q, int64_t rows) {
         a.ldc = q->din[l];
         a.cz = (int64_t)q->dout[l] * q->din[l];
         int nz = 1;
-        int rc = gemm<EPI_NONE>(q, a, kchunk, &nz);
+        int rc = gemm<EPI_NONE>(q, MST_MLP_K_GEMM_WGRAD, a, kchunk, &nz);
         if (rc) return rc;
         const int64_t nw = (int64_t)q->dout[l] * q->din[l];
-        hipLaunchKernelGGL(k_sum_splits, dim3(nblk(nw)), dim3(GT), 0, q->stream, nw, nz, (const float *)q->split,
-                           q->g + q->woff[l]);
-        HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(k_colsum, dim3(q->dout[l]), dim3(GT), 0, q->stream, rows, (int64_t)q->dout[l],
-                           (const float *)q->D[l + 1], q->g + q->boff[l]);
-        HIPCHK(hipGetLastError());
+        rc = mlp_launch(q, MST_MLP_K_OTHER, 0.0, k_sum_splits, dim3(nblk(nw)), dim3(GT), nw, nz,
+                        (const float *)q->split, q->g + q->woff[l]);
+        if (rc) return rc;
+        rc = mlp_launch(q, MST_MLP_K_OTHER, 0.0, k_colsum, dim3(q->dout[l]), dim3(GT), rows, (int64_t)q->dout[l],
+                        (const float *)q->D[l + 1], q->g + q->boff[l]);
+        if (rc) return rc;
         if (l == 0) break;
         // dZ_{l-1} = (dZ_l W_l) * (1 - H[l]^2)
         GemmArgs b;
@@ -389,26 +424,23 @@ int backward(mst_mlp *q, int64_t rows) {
         b.sbn = 1;
         b.ldc = q->din[l];
         b.P = q->H[l];
-        rc = gemm<EPI_DTANH>(q, b);
+        rc = gemm<EPI_DTANH>(q, MST_MLP_K_GEMM_DGRAD, b);
         if (rc) return rc;
     }
     return MST_OK;
 }
 
 int train_step_enqueue(mst_mlp *q, int batch, float lr) {
-    hipLaunchKernelGGL(k_gather, dim3(nblk(batch)), dim3(GT), 0, q->stream, (int64_t)batch, q->n_data, q->n_in,
-                       q->n_out, (const float *)q->X, (const float *)q->T, (const float *)q->scal,
-                       (const float *)(q->scal + q->n_in), (const int64_t *)q->step, q->seed, q->H[0], q->Tb);
-    HIPCHK(hipGetLastError());
-    int rc = forward(q, batch);
+    int rc = mlp_launch(q, MST_MLP_K_OTHER, 0.0, k_gather, dim3(nblk(batch)), dim3(GT), (int64_t)batch, q->n_data,
+                        q->n_in, q->n_out, (const float *)q->X, (const float *)q->T, (const float *)q->scal,
+                        (const float *)(q->scal + q->n_in), (const int64_t *)q->step, q->seed, q->H[0], q->Tb);
+    if (!rc) rc = forward(q, batch);
     if (!rc) rc = backward(q, batch);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_adam, dim3(nblk(q->np)), dim3(GT), 0, q->stream, q->np, q->p, (const float *)q->g, q->m, q->v,
-                       q->step, lr, 0.9f, 0.999f, 1e-8f);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, q->stream, q->step);
-    HIPCHK(hipGetLastError());
-    return MST_OK;
+    rc = mlp_launch(q, MST_MLP_K_OTHER, 0.0, k_adam, dim3(nblk(q->np)), dim3(GT), q->np, q->p, (const float *)q->g,
+                    q->m, q->v, q->step, lr, 0.9f, 0.999f, 1e-8f);
+    if (!rc) rc = mlp_launch(q, MST_MLP_K_OTHER, 0.0, k_step_advance, dim3(1), dim3(64), q->step);
+    return rc;
 }
 
 double sum_host(const float *dptr, int64_t n) {
@@ -519,6 +551,12 @@ int mst_mlp_destroy(mst_mlp *q) {
     if (!q) return MST_OK;
     if (q->stream) (void)hipStreamSynchronize(q->stream);
     free_work(q);
+    for (auto &v : q->ev)
+        for (auto &pr : v) {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+    for (auto e : q->pool) (void)hipEventDestroy(e);
     for (void *b : {(void *)q->p, (void *)q->g, (void *)q->m, (void *)q->v, (void *)q->scal, (void *)q->step,
                     (void *)q->X, (void *)q->T, (void *)q->split})
         if (b) (void)hipFree(b);
@@ -591,7 +629,7 @@ int mst_mlp_train(mst_mlp *q, int steps, int batch, double lr, int use_graph, do
     int rc = ensure_work(q, batch);
     if (!rc) rc = ensure_split(q, batch);
     if (rc) return rc;
-    if (use_graph && profiler_tool_loaded()) use_graph = 0;
+    if (use_graph && (q->timing || profiler_tool_loaded())) use_graph = 0;  // per-dispatch events need direct launches
     if (use_graph && (!q->gexec || q->g_batch != batch || q->g_lr != (float)lr)) {
         drop_graph(q);
         hipGraph_t g = nullptr;
@@ -661,6 +699,44 @@ int mst_mlp_gradient(mst_mlp *q, const float *X, const float *T, int64_t n, floa
     if (e != hipSuccess) return mst::fail(MST_E_HIP, hipGetErrorString(e));
     if (grad) HIPCHK(hipMemcpy(grad, q->g, sizeof(float) * q->np, hipMemcpyDeviceToHost));
     if (loss) *loss = sum_host(q->sq, n * q->n_out) / ((double)n * q->n_out);
+    return MST_OK;
+}
+
+int mst_mlp_timing_enable(mst_mlp *q, int on) {
+    if (!q) return mst::fail(MST_E_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(q->stream));
+    q->timing = on != 0;
+    return MST_OK;
+}
+
+int mst_mlp_timing_get(mst_mlp *q, int kclass, int64_t *count, double *total_ms, double *total_flop) {
+    if (!q || kclass < 0 || kclass >= MST_MLP_K_COUNT || !count || !total_ms)
+        return mst::fail(MST_E_ARG, "bad kernel class");
+    HIPCHK(hipStreamSynchronize(q->stream));
+    double tot = 0, fl = 0;
+    for (size_t i = 0; i < q->ev[kclass].size(); i++) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, q->ev[kclass][i].first, q->ev[kclass][i].second));
+        tot += ms;
+        fl += q->flops[kclass][i];
+    }
+    *count = (int64_t)q->ev[kclass].size();
+    *total_ms = tot;
+    if (total_flop) *total_flop = fl;
+    return MST_OK;
+}
+
+int mst_mlp_timing_reset(mst_mlp *q) {
+    if (!q) return mst::fail(MST_E_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(q->stream));
+    for (int k = 0; k < MST_MLP_K_COUNT; k++) {
+        for (auto &pr : q->ev[k]) {
+            q->pool.push_back(pr.first);
+            q->pool.push_back(pr.second);
+        }
+        q->ev[k].clear();
+        q->flops[k].clear();
+    }
     return MST_OK;
 }
 

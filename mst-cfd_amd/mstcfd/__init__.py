@@ -32,6 +32,10 @@ INIT_REFERENCE, INIT_UNIFORM, INIT_RIEMANN, INIT_VORTEX, INIT_PULSE, INIT_FREEST
 BATCH_BUCKETS = 4
 ORDER_HILBERT, ORDER_RCM = 0, 1
 STEADY_EXPLICIT, STEADY_LUSGS = 0, 1
+MLP_K_GEMM_FWD, MLP_K_GEMM_WGRAD, MLP_K_GEMM_DGRAD, MLP_K_OTHER = 0, 1, 2, 3
+MLP_KERNEL_NAMES = {MLP_K_GEMM_FWD: "k_gemm forward", MLP_K_GEMM_WGRAD: "k_gemm weight gradient",
+                    MLP_K_GEMM_DGRAD: "k_gemm activation gradient",
+                    MLP_K_OTHER: "gather, loss gradient, split sums, bias gradients, Adam"}
 
 # public exported symbols of include/mstcfd.h (checked by the CPU tests)
 EXPORTS = [
@@ -54,6 +58,7 @@ EXPORTS = [
     "mst_get_cell_order", "mst_set_steady_driver", "mst_lusgs_step", "mst_lusgs_colours", "mst_prep_cell_colours",
     "mst_write_tecplot", "mst_checkpoint_write", "mst_checkpoint_read", "mst_batch_set_steady_driver",
     "mst_batch_lusgs_step", "mst_mesh_write_cache", "mst_mesh_read_cache", "mst_monitor_maxrel_noalloc",
+    "mst_mlp_timing_enable", "mst_mlp_timing_get", "mst_mlp_timing_reset",
 ]
 
 
@@ -199,6 +204,10 @@ def lib():
         "mst_mlp_train": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_double)]),
         "mst_mlp_forward": (C.c_int, [vp, F32P, C.c_int64, F32P]),
         "mst_mlp_gradient": (C.c_int, [vp, F32P, F32P, C.c_int64, F32P, C.POINTER(C.c_double)]),
+        "mst_mlp_timing_enable": (C.c_int, [vp, C.c_int]),
+        "mst_mlp_timing_get": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_double)]),
+        "mst_mlp_timing_reset": (C.c_int, [vp]),
         "mst_ctx_warm_start": (C.c_int, [vp, vp, F64P, C.c_int]),
         "mst_run_to_steady": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, F64P, C.POINTER(C.c_int),
                                         C.POINTER(C.c_double)]),
@@ -732,6 +741,20 @@ class MLP:
         _check(lib().mst_mlp_train(self.h, int(steps), int(batch), float(lr), 1 if graph else 0, C.byref(loss)),
                "mst_mlp_train")
         return loss.value
+
+    def timing(self, on=True):
+        """Per-dispatch timing of the network's kernels (training launches directly while it is on)."""
+        _check(lib().mst_mlp_timing_enable(self.h, 1 if on else 0), "mst_mlp_timing_enable")
+
+    def timing_get(self, kclass):
+        """(launches, total ms, total floating-point operations) of a kernel class (MLP_K_*)."""
+        n, ms, fl = C.c_int64(), C.c_double(), C.c_double()
+        _check(lib().mst_mlp_timing_get(self.h, int(kclass), C.byref(n), C.byref(ms), C.byref(fl)),
+               "mst_mlp_timing_get")
+        return n.value, ms.value, fl.value
+
+    def timing_reset(self):
+        _check(lib().mst_mlp_timing_reset(self.h), "mst_mlp_timing_reset")
 
     def forward(self, X):
         X = np.ascontiguousarray(X, np.float32)

@@ -95,6 +95,36 @@ def test_graph_is_rebuilt_when_its_buffers_change():
     assert np.array_equal(out[0], out[1])
 
 
+def test_kernel_timing_counts_and_leaves_results_unchanged():
+    """Per-dispatch timing of the network's kernels: the launches and GEMM operation counts of a step
+    follow from the layer shapes, and a timed run (direct launches) ends on the graph run's
+    parameters bit for bit."""
+    X, T = data(20000, seed=12)
+    out = []
+    for timed in (False, True):
+        net, _ = make(seed=13)
+        net.set_data(X, T)
+        if timed:
+            net.timing(True)
+        net.train(6, batch=2048, lr=1e-3, graph=True)
+        out.append(net.get_params())
+        if timed:
+            layers = net.layers()  # (n_out, n_in) per layer
+            fwd = sum(2.0 * 2048 * o * i for (o, i) in layers)
+            n, ms, fl = net.timing_get(M.MLP_K_GEMM_FWD)
+            assert n == 6 * len(layers) and ms > 0 and fl == 6 * fwd
+            n, ms, fl = net.timing_get(M.MLP_K_GEMM_WGRAD)
+            assert n == 6 * len(layers) and fl == 6 * fwd
+            n, ms, fl = net.timing_get(M.MLP_K_GEMM_DGRAD)
+            assert n == 6 * (len(layers) - 1) and fl == 6 * (fwd - 2.0 * 2048 * layers[0][0] * layers[0][1])
+            n, ms, fl = net.timing_get(M.MLP_K_OTHER)
+            assert n == 6 * (2 * len(layers) + 4) and fl == 0
+            net.timing_reset()
+            assert net.timing_get(M.MLP_K_GEMM_FWD)[0] == 0
+        net.close()
+    assert np.array_equal(out[0], out[1])
+
+
 def test_training_fits_a_smooth_field():
     X, T = data(200000, seed=7)
     net, _ = make(seed=8)
