@@ -76,7 +76,9 @@ private:
 };
 
 // the device solver with RhoSolver's method set (rhoSolver/RhoSolver.h:14-24); one context for the
-// whole run, where the reference constructs its solver every step (time/Time.cpp:58)
+// whole run, where the reference constructs its solver every step (time/Time.cpp:58).  The states live
+// in HBM; the host arrays getOldValue / getNewValue return are mirrors filled only when they are read
+// after the device state changed (a time loop that never reads them moves no state over PCIe).
 class RhoSolverGPU {
 public:
     RhoSolverGPU(const Mesh &mesh, const mst_config &cfg, const std::vector<double> &initial)
@@ -103,20 +105,48 @@ public:
     void setDT(double dt) { check(mst_set_dt(ctx_, dt)); }  // RhoSolver.cpp:33-35
     void solve() {                                           // RhoSolver.cpp:37-68
         check(mst_solve(ctx_));
-        check(mst_get_new_state(ctx_, new_.data(), n_));
+        new_is_old_ = false;
+        new_stale_ = true;
     }
-    const double *getOldValue() const { return old_.data(); }  // RhoSolver.cpp:504-506
-    const double *getNewValue() const { return new_.data(); }  // RhoSolver.cpp:507-509
-    void updateNewToOld() {                                    // RhoSolver.cpp:513-517
+    const double *getOldValue() {  // RhoSolver.cpp:504-506
+        if (old_stale_) {
+            check(mst_get_state(ctx_, old_.data(), n_));
+            old_stale_ = false;
+        }
+        return old_.data();
+    }
+    const double *getNewValue() {  // RhoSolver.cpp:507-509
+        if (new_is_old_) return getOldValue();
+        if (new_stale_) {
+            check(mst_get_new_state(ctx_, new_.data(), n_));
+            new_stale_ = false;
+        }
+        return new_.data();
+    }
+    void updateNewToOld() {  // RhoSolver.cpp:513-517: Old := New, and New keeps that value
         check(mst_update_new_to_old(ctx_));
-        old_ = new_;
+        if (!new_is_old_) {
+            if (!new_stale_) {
+                old_.swap(new_);
+                old_stale_ = false;
+            } else {
+                old_stale_ = true;
+            }
+        }
+        new_is_old_ = true;
+    }
+    // the monitor of Time::goNextTimeStep (time/Time.cpp:69-76), max |New - Old| / Old per field, reduced
+    // on the device: the time loop's per-step check without reading the states back
+    std::vector<double> monitorMaxRel() {
+        std::vector<double> r(nv_);
+        check(mst_monitor_maxrel(ctx_, r.data()));
+        return r;
     }
 
-    // device-resident stepping (SSP-RK stages and the CFL time step of the configuration): the
-    // states stay in HBM; the host mirrors are refreshed at the end
+    // device-resident stepping (SSP-RK stages and the CFL time step of the configuration)
     void step(int n) {
         check(mst_step(ctx_, n));
-        refresh();
+        old_stale_ = new_is_old_ = true;
     }
     // the implicit steady driver (LU-SGS) to tol x the first iteration's norms
     int runToSteady(double tol, int maxIterations, int checkEvery, double cfl = 1e4, double omega = 1.0) {
@@ -125,7 +155,7 @@ public:
         int steps = 0;
         double drop = 0;
         check(mst_run_to_steady(ctx_, tol, maxIterations, checkEvery, ref.data(), &steps, &drop));
-        refresh();
+        old_stale_ = new_is_old_ = true;
         return steps;
     }
     std::vector<double> residualNorms() const {
@@ -136,14 +166,12 @@ public:
     mst_ctx *handle() const { return ctx_; }
 
 private:
-    void refresh() {
-        check(mst_get_state(ctx_, old_.data(), n_));
-        new_ = old_;
-    }
     mst_ctx *ctx_ = nullptr;
     int64_t n_;
     int nv_;
     std::vector<double> old_, new_;
+    bool old_stale_ = false, new_stale_ = false;
+    bool new_is_old_ = true;  // New equals Old (after updateNewToOld, step or the initial state)
 };
 
 }  // namespace mstcfd
