@@ -416,32 +416,40 @@ struct UpdArgs {
 // return to it (the reference checks nothing at all).
 enum { UPD_RESIDUAL = 0, UPD_STAGE = 1, UPD_L2 = 2, UPD_DT = 4, UPD_CHECK = 8 };
 
-// Block-level reduction of W values (fixed tree order): sums, or a min when MIN is set; thread 0..W-1
-// of the block write the results.  The per-block partials are combined by k_finish in block order,
-// so every reduction is bit-reproducible (no float atomics, no cross-block fences).
-// sh: W * NT doubles of LDS.  The first barrier also orders the block's earlier readers of sh before
-// the writes, so a kernel may pass LDS it has finished with.
+// Block-level reduction of W values: sums, or a min when MIN is set; thread 0..W-1 of the block write
+// the results.  A fixed xor butterfly inside each wave (every lane ends with the wave's value), then
+// the waves' values in wave order: one barrier after the first instead of one per tree level.  The
+// per-block partials are combined by k_finish in block order, so every reduction is bit-reproducible
+// (no float atomics, no cross-block fences).
+// sh: at least W * NT / 64 doubles of LDS.  The first barrier also orders the block's earlier readers
+// of sh before the writes, so a kernel may pass LDS it has finished with.
 template <int W, bool MIN, int NT = BLOCK>
 __device__ __forceinline__ void block_partial(double (&v)[W], double *__restrict__ out, int64_t stride, double *sh) {
-    __syncthreads();  // a previous call's readers of sh[.][0] are done (persistent loops reuse it)
+    constexpr int NW = NT / 64;
+    static_assert(NT % 64 == 0, "blocks are whole waves");
 #pragma unroll
-    for (int k = 0; k < W; k++) sh[k * NT + threadIdx.x] = v[k];
-    __syncthreads();
-    for (int w = NT / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
+    for (int k = 0; k < W; k++)
 #pragma unroll
-            for (int k = 0; k < W; k++) {
-                const double y = sh[k * NT + threadIdx.x + w];
-                sh[k * NT + threadIdx.x] = MIN ? fmin(sh[k * NT + threadIdx.x], y) : sh[k * NT + threadIdx.x] + y;
-            }
+        for (int m = 32; m > 0; m >>= 1) {
+            const double y = __shfl_xor(v[k], m, 64);
+            v[k] = MIN ? fmin(v[k], y) : v[k] + y;
         }
-        __syncthreads();
+    __syncthreads();  // a previous call's readers of sh are done (persistent loops reuse it)
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < W; k++) sh[k * NW + (threadIdx.x >> 6)] = v[k];
     }
-    if ((int)threadIdx.x < W) out[threadIdx.x * stride] = sh[threadIdx.x * NT];
+    __syncthreads();
+    if ((int)threadIdx.x < W) {
+        double s = sh[threadIdx.x * NW];
+#pragma unroll
+        for (int w = 1; w < NW; w++) s = MIN ? fmin(s, sh[threadIdx.x * NW + w]) : s + sh[threadIdx.x * NW + w];
+        out[threadIdx.x * stride] = s;
+    }
 }
 template <int W, bool MIN, int NT = BLOCK>
 __device__ __forceinline__ void block_partial(double (&v)[W], double *__restrict__ out, int64_t stride) {
-    __shared__ double sh[W * NT];
+    __shared__ double sh[W * (NT / 64)];
     block_partial<W, MIN, NT>(v, out, stride, sh);
 }
 
@@ -539,7 +547,7 @@ __device__ __forceinline__ void stage_cell(const UpdArgs &a, int64_t c, const do
 
 // stage_cell for the lanes of one block, then the block's partials of |R/V|^2 and the CFL time step.
 // Every lane of the block must call it (block-level reductions).
-// lds (DYN): NV * BLOCK doubles of the kernel's dynamic LDS for the reductions, in place of a static
+// lds (DYN): NV * BLOCK / 64 doubles of the kernel's dynamic LDS for the reductions, in place of a static
 // array (which would count against every block's LDS for the whole kernel).
 template <int NV, bool STAGE, bool F32, bool DYN = false>
 __device__ __forceinline__ void stage_tail(const UpdArgs &a, int64_t c, bool valid, const double (&r)[NV],
@@ -1031,7 +1039,7 @@ __global__ __launch_bounds__(BLOCK, (fused_waves<DIM, SCHEME, LIM, T, MAXF>())) 
     });
     double r[NV];
     patch_sum<NV, MAXF, F32>(L.g, L.npf, cl, r);
-    stage_tail<NV, true, F32, true>(a, c, valid, r, blk, a.l2, a.dtred, &in, sh);  // (sh at least NV * BLOCK doubles)
+    stage_tail<NV, true, F32, true>(a, c, valid, r, blk, a.l2, a.dtred, &in, sh);  // (sh at least NV * BLOCK / 64 doubles)
 }
 
 // ------------------------------------------------------------------------------------ reductions

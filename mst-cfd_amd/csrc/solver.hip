@@ -444,9 +444,9 @@ int run_fused_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int 
     ha.cell = x->hcell;
     ha.loc = x->hloc;
     x->hg.args<T>(ha);
-    // the face states and fluxes, and afterwards the block reductions (NV doubles per lane)
+    // the face states and fluxes, and afterwards the block reductions (NV doubles per wave)
     const size_t shm = std::max((size_t)FusedLds<T>::bytes_per_face_field * x->nv * x->max_pf,
-                                sizeof(double) * x->nv * BLOCK);
+                                sizeof(double) * x->nv * (BLOCK / 64));
     const bool roe = x->cfg.scheme == MST_SCHEME_ROE;
     const int lim = x->cfg.limiter;
 #define MST_FUSED_L(SC, LM, MF) \
