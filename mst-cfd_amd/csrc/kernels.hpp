@@ -228,38 +228,51 @@ struct IncView {
     int64_t stride;
 };
 
-// Every memory read of the cell is issued before the first is used: the slots' codes, weights,
-// normals and offsets and the volume (independent of each other), then all the neighbour states (an
-// empty or boundary slot loads the cell's own state, unused).  That is two dependent load rounds per
-// cell instead of one or two per slot, with the same arithmetic in the same slot order.
-template <int DIM, int MAXF, int LIM, typename T>
-__device__ __forceinline__ int grad_limited(const T *__restrict__ Q, const IncView<T> &a, const int64_t g,
-                                           const int64_t c, T (&qc)[DIM + 2], T (&lgr)[(DIM + 2) * DIM],
-                                           T (&dxs)[MAXF][DIM], T (&t)[DIM + 2][DIM], T (&ph)[DIM + 2]) {
-    constexpr int NV = DIM + 2;
-    constexpr bool F32 = std::is_same<T, float>::value;
-    T qmax[NV], qmin[NV];
+// The memory reads of one cell's limited gradient.  grad_fetch issues every read before the first is
+// used: the slots' codes, weights, normals (unless NRM is false: the caller supplies them) and offsets
+// and the volume (independent of each other), then all the neighbour states (an empty or boundary
+// slot loads the cell's own state, unused).  That is two dependent load rounds per cell instead of
+// one or two per slot.
+template <int DIM, int MAXF, typename T>
+struct GradIn {
     int32_t code[MAXF];
     T eta[MAXF], n[MAXF][DIM];
+    T qc[DIM + 2], qn[MAXF][DIM + 2];
+    double vol;
+};
+
+template <int DIM, int MAXF, bool NRM, typename T>
+__device__ __forceinline__ void grad_fetch(const T *__restrict__ Q, const IncView<T> &a, const int64_t g,
+                                           const int64_t c, GradIn<DIM, MAXF, T> &in, T (&dxs)[MAXF][DIM]) {
+    constexpr int NV = DIM + 2;
 #pragma unroll
     for (int s = 0; s < MAXF; s++) {
         const int64_t i = (int64_t)s * a.stride + g;
-        code[s] = a.code[i];
-        eta[s] = a.w[i];
+        in.code[s] = a.code[i];
+        in.eta[s] = a.w[i];
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
-            n[s][d] = a.n[i * DIM + d];
+            if constexpr (NRM) in.n[s][d] = a.n[i * DIM + d];
             dxs[s][d] = a.dx[i * DIM + d];
         }
     }
-    const double vol = a.vol[g];
-    ldq<NV>(Q + c * NV, qc);
-    T qn[MAXF][NV];
+    in.vol = a.vol[g];
+    ldq<NV>(Q + c * NV, in.qc);
 #pragma unroll
     for (int s = 0; s < MAXF; s++) {
-        const int64_t nb = code[s] >= 0 ? (int64_t)(code[s] >> 1) : c;
-        ldq<NV>(Q + nb * NV, qn[s]);
+        const int64_t nb = in.code[s] >= 0 ? (int64_t)(in.code[s] >> 1) : c;
+        ldq<NV>(Q + nb * NV, in.qn[s]);
     }
+}
+
+// The limited gradient of a cell from its fetched inputs; returns the number of used slots.
+template <int DIM, int MAXF, int LIM, typename T>
+__device__ __forceinline__ int grad_eval(const GradIn<DIM, MAXF, T> &in, const T (&dxs)[MAXF][DIM],
+                                         T (&lgr)[(DIM + 2) * DIM], T (&t)[DIM + 2][DIM], T (&ph)[DIM + 2]) {
+    constexpr int NV = DIM + 2;
+    constexpr bool F32 = std::is_same<T, float>::value;
+    const T(&qc)[NV] = in.qc;
+    T qmax[NV], qmin[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         qmax[k] = qmin[k] = qc[k];
@@ -269,23 +282,23 @@ __device__ __forceinline__ int grad_limited(const T *__restrict__ Q, const IncVi
     int nfc = 0;
 #pragma unroll
     for (int s = 0; s < MAXF; s++) {
-        if (code[s] == -2) continue;  // empty slots are the last ones of the cell
-        const bool interior = code[s] >= 0;
-        const bool is1 = interior && (code[s] & 1);
+        if (in.code[s] == -2) continue;  // empty slots are the last ones of the cell
+        const bool interior = in.code[s] >= 0;
+        const bool is1 = interior && (in.code[s] & 1);
         T qf[NV];
         if (interior) {
             if constexpr (F32) {
 #pragma unroll
-                for (int k = 0; k < NV; k++) qf[k] = eta[s] * (qn[s][k] - qc[k]);
+                for (int k = 0; k < NV; k++) qf[k] = in.eta[s] * (in.qn[s][k] - qc[k]);
             } else {
 #pragma unroll
                 for (int k = 0; k < NV; k++)
-                    qf[k] = is1 ? eta[s] * qn[s][k] + (1 - eta[s]) * qc[k] : eta[s] * qc[k] + (1 - eta[s]) * qn[s][k];
+                    qf[k] = is1 ? in.eta[s] * in.qn[s][k] + (1 - in.eta[s]) * qc[k] : in.eta[s] * qc[k] + (1 - in.eta[s]) * in.qn[s][k];
             }
 #pragma unroll
             for (int k = 0; k < NV; k++) {
-                qmax[k] = fmax(qmax[k], qn[s][k]);
-                qmin[k] = fmin(qmin[k], qn[s][k]);
+                qmax[k] = fmax(qmax[k], in.qn[s][k]);
+                qmin[k] = fmin(qmin[k], in.qn[s][k]);
             }
         } else {
 #pragma unroll
@@ -294,10 +307,10 @@ __device__ __forceinline__ int grad_limited(const T *__restrict__ Q, const IncVi
 #pragma unroll
         for (int k = 0; k < NV; k++)
 #pragma unroll
-            for (int d = 0; d < DIM; d++) t[k][d] += qf[k] * n[s][d];
+            for (int d = 0; d < DIM; d++) t[k][d] += qf[k] * in.n[s][d];
         nfc = s + 1;
     }
-    const T iV = (T)frcp(vol);
+    const T iV = (T)frcp(in.vol);
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         ph[k] = 1;
@@ -342,6 +355,18 @@ __device__ __forceinline__ int grad_limited(const T *__restrict__ Q, const IncVi
     for (int k = 0; k < NV; k++)
 #pragma unroll
         for (int d = 0; d < DIM; d++) lgr[k * DIM + d] = ph[k] * t[k][d];
+    return nfc;
+}
+
+template <int DIM, int MAXF, int LIM, typename T>
+__device__ __forceinline__ int grad_limited(const T *__restrict__ Q, const IncView<T> &a, const int64_t g,
+                                           const int64_t c, T (&qc)[DIM + 2], T (&lgr)[(DIM + 2) * DIM],
+                                           T (&dxs)[MAXF][DIM], T (&t)[DIM + 2][DIM], T (&ph)[DIM + 2]) {
+    GradIn<DIM, MAXF, T> in;
+    grad_fetch<DIM, MAXF, true, T>(Q, a, g, c, in, dxs);
+    const int nfc = grad_eval<DIM, MAXF, LIM, T>(in, dxs, lgr, t, ph);
+#pragma unroll
+    for (int k = 0; k < DIM + 2; k++) qc[k] = in.qc[k];
     return nfc;
 }
 
@@ -873,15 +898,18 @@ struct HaloGeo {
 // Each face's flux overwrites its own states, which only the lane that solves the face reads.
 // FP64: states T[2][NV][npf], fluxes over the c0-side states, 16 * NV * npf bytes.  FP32: the two
 // sides' states of a component side by side (float2[NV][npf]), exactly the bytes of that
-// component's FP64 flux, 8 * NV * npf bytes.
+// component's FP64 flux, 8 * NV * npf bytes.  With the face normals staged (2D), the c0 outward normals
+// of the patch faces follow, double[DIM][npf] (8 * DIM * npf bytes).
 template <typename T>
 struct FusedLds {
     T *rec;
     double *g;
+    double *nrm;
     int npf;
     __device__ FusedLds(double *sh, int npf_, int nv) : npf(npf_) {
         rec = reinterpret_cast<T *>(sh);
         g = sh;
+        nrm = sh + (bytes_per_face_field / 8) * nv * npf_;
     }
     __device__ T &r(int side, int k, int l, int nv) {
         if constexpr (std::is_same<T, double>::value)
@@ -891,6 +919,19 @@ struct FusedLds {
     }
     static constexpr int bytes_per_face_field = std::is_same<T, double>::value ? 16 : 8;
 };
+
+// Whether the fused kernel stages its patch faces' normals in LDS (2D: a patch cell's incidence
+// normal is then read from there, + or - its face's normal, instead of from the incidence arrays)
+template <int DIM>
+constexpr bool fused_lds_normals() {
+    return DIM == 2;
+}
+
+// dynamic LDS bytes of the fused kernel for patches of at most max_pf faces
+inline size_t fused_lds_bytes(int dim, int nv, int max_pf, bool f32, bool staged_normals) {
+    const size_t states = (size_t)(f32 ? 8 : 16) * nv * max_pf;
+    return states + (staged_normals ? (size_t)8 * dim * max_pf : 0);
+}
 
 // Resident waves per SIMD requested for the fused kernel, where the bound costs no more than a register
 // of spill: FP64 triangles with Roe or unlimited AUSM 4 (at most 128 VGPRs; the compiler would keep 3
@@ -911,7 +952,7 @@ constexpr int fused_waves() {
 // SCHEME is SCHEME_RUNTIME.  pre() runs between the gradient and the flux phases, when the gradient's
 // registers are free: the caller fetches its update's inputs there, so their latency overlaps the
 // face solves.
-template <int DIM, int SCHEME, int LIM, typename T, int MAXF, typename Pre>
+template <int DIM, int SCHEME, int LIM, typename T, int MAXF, bool NRM_LDS = false, typename Pre>
 __device__ __forceinline__ void fused_patch(const GradArgs<T> &ga, const FluxArgs<T> &fa, bool roe, const PatchArgs &p,
                                             const HaloArgs &h, int64_t patch, int64_t cb, int64_t ce, int64_t nc,
                                             int32_t f0, FusedLds<T> &L, int (&cl)[MAXF], Pre pre) {
@@ -946,7 +987,60 @@ __device__ __forceinline__ void fused_patch(const GradArgs<T> &ga, const FluxArg
     const IncView<T> own = cell_view(ga);
     const IncView<T> hv{h.code, static_cast<const T *>(h.w), static_cast<const T *>(h.n),
                         static_cast<const T *>(h.dx), h.vol, h.nh};
-    for (int i0 = 0; i0 < next; i0 += BLOCK) {  // i0 == 0: the patch cells, then the halo
+    if constexpr (NRM_LDS) {
+        // The patch cells' reads first (their normals excepted), then the patch faces' normals into
+        // LDS, a barrier, and each incidence normal from LDS: + the face's normal on its c0 side, - on
+        // its c1 side, which is how the incidence arrays hold it.  A slot whose face carries no flux
+        // (not a patch face) reads the incidence array.
+        GradIn<DIM, MAXF, T> gin;
+        T dxs[MAXF][DIM];
+        const bool mine = c < ce;
+        if (mine) grad_fetch<DIM, MAXF, false, T>(ga.Q, own, c, c, gin, dxs);
+#pragma unroll
+        for (int q = 0; q < PF; q++) {
+            const int i = threadIdx.x + q * BLOCK;
+            if (i < npf)
+#pragma unroll
+                for (int d = 0; d < DIM; d++) L.nrm[d * npf + i] = pf_n[q][d];
+        }
+        for (int i = threadIdx.x + PF * BLOCK; i < npf; i += BLOCK) {
+            const int64_t f = p.face[f0 + i];
+#pragma unroll
+            for (int d = 0; d < DIM; d++) L.nrm[d * npf + i] = fa.fn[f * DIM + d];
+        }
+        __syncthreads();
+        if (mine) {
+#pragma unroll
+            for (int s = 0; s < MAXF; s++) {
+                const int l = cl[s];
+                if (l >= 0) {
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) {
+                        const double v = L.nrm[d * npf + (l >> 1)];
+                        gin.n[s][d] = (T)((l & 1) ? -v : v);
+                    }
+                } else if (gin.code[s] != -2) {
+#pragma unroll
+                    for (int d = 0; d < DIM; d++) gin.n[s][d] = own.n[((int64_t)s * nc + c) * DIM + d];
+                }
+            }
+            T lgr[NV * DIM], t[NV][DIM], ph[NV];
+            const int nfc = grad_eval<DIM, MAXF, LIM, T>(gin, dxs, lgr, t, ph);
+#pragma unroll
+            for (int s = 0; s < MAXF; s++) {
+                if (s >= nfc) break;
+                const int l = cl[s];
+                if (l < 0) continue;
+                T q[NV];
+#pragma unroll
+                for (int k = 0; k < NV; k++) q[k] = gin.qc[k];
+                recon<DIM>(q, lgr, dxs[s]);
+#pragma unroll
+                for (int k = 0; k < NV; k++) L.r(l & 1, k, l >> 1, NV) = q[k];
+            }
+        }
+    }
+    for (int i0 = NRM_LDS ? BLOCK : 0; i0 < next; i0 += BLOCK) {  // i0 == 0: the patch cells, then the halo
         const int i = i0 + (int)threadIdx.x;
         if (i >= next) break;
         int64_t cc, g;
@@ -990,14 +1084,19 @@ __device__ __forceinline__ void fused_patch(const GradArgs<T> &ga, const FluxArg
         if (q < PF) {
             f = pf_f[q];
             info = pf_info[q];
+            if constexpr (!NRM_LDS)
 #pragma unroll
-            for (int d = 0; d < DIM; d++) n64[d] = pf_n[q][d];
+                for (int d = 0; d < DIM; d++) n64[d] = pf_n[q][d];
         } else {
             f = p.face[f0 + i];
             info = fa.finfo[f];
+            if constexpr (!NRM_LDS)
 #pragma unroll
-            for (int d = 0; d < DIM; d++) n64[d] = fa.fn[f * DIM + d];
+                for (int d = 0; d < DIM; d++) n64[d] = fa.fn[f * DIM + d];
         }
+        if constexpr (NRM_LDS)
+#pragma unroll
+            for (int d = 0; d < DIM; d++) n64[d] = L.nrm[d * npf + i];
         const int kind = info & 15;
         T X[NV], Y[NV];
         if (kind == mst::KIND_INTERIOR || kind == mst::KIND_PERIODIC) {
@@ -1034,7 +1133,8 @@ __global__ __launch_bounds__(BLOCK, (fused_waves<DIM, SCHEME, LIM, T, MAXF>())) 
     const int64_t c = cb + threadIdx.x;
     const bool valid = c < a.nc;
     StageIn<NV> in;
-    fused_patch<DIM, SCHEME, LIM, T, MAXF>(ga, fa, SCHEME == MST_SCHEME_ROE, p, h, blk, cb, ce, a.nc, f0, L, cl, [&]() {
+    fused_patch<DIM, SCHEME, LIM, T, MAXF, fused_lds_normals<DIM>()>(ga, fa, SCHEME == MST_SCHEME_ROE, p, h, blk, cb, ce,
+                                                                      a.nc, f0, L, cl, [&]() {
         if (valid) stage_load<NV>(a, c, in);
     });
     double r[NV];

@@ -79,6 +79,7 @@ struct mst_ctx {
     int max_pf = 0;        // most faces of one patch (sizes the kernel's LDS)
     int64_t n_psolve = 0;  // Riemann problems per patch launch
     bool patch_ok = false;
+    bool fused_ok = false;  // the fused kernel's LDS (states and staged normals) fits 64 KiB
     // fused strategy: each patch's halo cells (across its border faces) and their slots' local faces
     int32_t *hstart = nullptr, *hcell = nullptr;
     int16_t *hloc = nullptr;
@@ -429,7 +430,7 @@ int ensure_halo_geo(mst_ctx *x);
 template <int DIM, typename T>
 int run_fused_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
     if (x->cfg.order < 2) return run_patch_t<DIM, T>(x, mode, is, in, ca, cb, stage0, iout);
-    if (!x->patch_ok) return mst::fail(MST_E_UNSUPPORTED, "no patch tables for this mesh");
+    if (!x->fused_ok) return mst::fail(MST_E_UNSUPPORTED, "fused strategy: a patch's face states exceed 64 KiB of LDS");
     int rc = ensure_halo_geo(x);
     if (rc) return rc;
     const GradArgs<T> ga = grad_args<T>(x, is, false);
@@ -444,8 +445,10 @@ int run_fused_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int 
     ha.cell = x->hcell;
     ha.loc = x->hloc;
     x->hg.args<T>(ha);
-    // the face states and fluxes, and afterwards the block reductions (NV doubles per wave)
-    const size_t shm = std::max((size_t)FusedLds<T>::bytes_per_face_field * x->nv * x->max_pf,
+    // the face states and fluxes (and in 2D the faces' normals), and afterwards the block reductions
+    // (NV doubles per wave)
+    const size_t shm = std::max(fused_lds_bytes(DIM, x->nv, x->max_pf, std::is_same<T, float>::value,
+                                                fused_lds_normals<DIM>()),
                                 sizeof(double) * x->nv * (BLOCK / 64));
     const bool roe = x->cfg.scheme == MST_SCHEME_ROE;
     const int lim = x->cfg.limiter;
@@ -629,7 +632,7 @@ int build_incidence(mst_ctx *x) {
 // 16 * NV bytes per face of the largest patch within 64 KiB.
 int build_patches(mst_ctx *x) {
     const mst::Prepared &P = x->prep.P;
-    x->patch_ok = false;
+    x->patch_ok = x->fused_ok = false;
     if (x->slots > 4) return MST_OK;
     const int64_t nc = x->nc, nb = (nc + BLOCK - 1) / BLOCK;
     std::vector<int32_t> start(nb + 1, 0), face, list, hstart(nb + 1, 0), hcell, halo;
@@ -695,6 +698,7 @@ int build_patches(mst_ctx *x) {
     x->max_pf = (int)max_pf;
     x->n_psolve = (int64_t)face.size();
     x->patch_ok = true;
+    x->fused_ok = fused_lds_bytes(x->dim, x->nv, x->max_pf, false, x->dim == 2) <= 65536;
     return MST_OK;
 }
 
@@ -979,6 +983,7 @@ int resolve_strategy(mst_ctx *x) {
     int pick = MST_STRATEGY_FACE, rc = MST_OK;
     for (int s : {MST_STRATEGY_FACE, MST_STRATEGY_CELL, MST_STRATEGY_PATCH, MST_STRATEGY_FUSED}) {
         if ((s == MST_STRATEGY_PATCH || s == MST_STRATEGY_FUSED) && !x->patch_ok) continue;
+        if (s == MST_STRATEGY_FUSED && !x->fused_ok) continue;
         x->resolved = s;
         rc = run_stage(x, UPD_STAGE, x->iold, x->iold, 0.0, 1.0, 1, spare);  // warm-up
         if (rc) break;
@@ -1215,6 +1220,8 @@ int mst_set_strategy(mst_ctx *ctx, int strategy) {
         return mst::fail(MST_E_ARG, "bad strategy");
     if ((strategy == MST_STRATEGY_PATCH || strategy == MST_STRATEGY_FUSED) && !ctx->patch_ok)
         return mst::fail(MST_E_UNSUPPORTED, "patch strategy: needs at most 4 faces per cell and 64 KiB of fluxes per patch");
+    if (strategy == MST_STRATEGY_FUSED && !ctx->fused_ok)
+        return mst::fail(MST_E_UNSUPPORTED, "fused strategy: a patch's face states exceed 64 KiB of LDS");
     ctx->strategy = strategy;
     ctx->resolved = strategy == MST_STRATEGY_AUTO ? -1 : strategy;
     return MST_OK;
@@ -1670,7 +1677,9 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
                      4.0 * (double)ctx->n_psolve + 2.0 * ctx->slots * nc;
             break;
         case MST_K_FUSED: {  // gradient inputs + per-face inputs + update, each once; no records
-            const double grad_in = 4.0 * ctx->slots * nc + ninc * (r + 2 * r * D) + nc * (r * NV + 8);
+            // per incidence: code, weight, offset, and in 3D the normal (2D takes it from its face's)
+            const double inc_geo = ctx->dim == 2 ? r + r * D : r + 2 * r * D;
+            const double grad_in = 4.0 * ctx->slots * nc + ninc * inc_geo + nc * (r * NV + 8);
             const double flux_f = nf * (4 + 8 * D) + 4.0 * (double)ctx->n_psolve + nbnd * 4;
             *bytes = grad_in + flux_f + upd - ell + 2.0 * ctx->slots * nc - (mixed ? 0 : nc * 8 * NV);
             break;
