@@ -243,7 +243,7 @@ int read_checkpoint(const std::string &path, const mst_mesh &m, double *Q, int64
     if (!fp) return fail(MST_E_IO, "cannot open " + path);
     CkptHeader h;
     const bool got = fread(&h, sizeof h, 1, fp) == 1;
-    if (!got || memcmp(h.magic, CKPT_MAGIC, 8) != 0 || h.version != 1) {
+    if (!got || memcmp(h.magic, CKPT_MAGIC, 8) != 0 || h.version != 1 || h.nv != (uint32_t)(m.dim + 2)) {
         fclose(fp);
         return fail(MST_E_IO, path + ": not a checkpoint of this format");
     }

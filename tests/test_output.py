@@ -1,5 +1,7 @@
 """On-disk formats: the Tecplot result file (Work::writedataRhoBasedMshNodePlt, work/Work.cpp:204-304,
 with its fixes) and the binary checkpoint (replacing the .cellPUVT dump, work/Work.cpp:432-514).  CPU only."""
+import struct
+
 import numpy as np
 import pytest
 
@@ -164,6 +166,28 @@ def test_checkpoint_round_trip_and_refusals(tmp_path):
     (tmp_path / "text.ckpt").write_bytes(b"not a checkpoint at all" * 10)
     with pytest.raises(M.MstError, match="format"):
         m.read_checkpoint(tmp_path / "text.ckpt")
+    # a header edited to 5 fields per cell, with 5 fields of data and their checksum: only the field
+    # count can refuse it (the caller's buffer holds dim + 2 per cell)
+    def fnv1a(b):
+        h = 1469598103934665603
+        for x in b:
+            h = ((h ^ x) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+        return h
+    head = bytearray(raw[:80])
+    assert struct.unpack_from("<I", head, 16)[0] == 4  # nv
+    struct.pack_into("<I", head, 16, 5)
+    data = rng.standard_normal(m.n_cells * 5).tobytes()
+    struct.pack_into("<Q", head, 72, fnv1a(data))
+    (tmp_path / "nv5.ckpt").write_bytes(bytes(head) + data)
+    with pytest.raises(M.MstError, match="format"):
+        m.read_checkpoint(tmp_path / "nv5.ckpt")
+    # the same edit back to 4 fields reads (the forging is right)
+    struct.pack_into("<I", head, 16, 4)
+    data4 = data[: m.n_cells * 4 * 8]
+    struct.pack_into("<Q", head, 72, fnv1a(data4))
+    (tmp_path / "nv4.ckpt").write_bytes(bytes(head) + data4)
+    assert np.array_equal(m.read_checkpoint(tmp_path / "nv4.ckpt")[0].reshape(-1),
+                          np.frombuffer(data4, np.float64))
 
 
 @pytest.mark.parametrize("name", ["tri", "periodic", "quad", "tet", "fluent"])
