@@ -304,8 +304,8 @@ __global__ __launch_bounds__(BLOCK, (kb_fused_waves<DIM, T, MAXF>())) void kb_fu
         FusedLds<T> L(sh, p.start[I.pad + 1] - f0, NV);
         int cl[MAXF];
         // (no early fetch of the update's inputs here: with both schemes inlined it would cost a wave)
-        fused_patch<DIM, SCHEME_RUNTIME, LIM, T, MAXF>(a.g, fa, roe, p, h, I.pad, I.begin, I.end, a.u.nc, f0, L, cl,
-                                                       []() {});
+        fused_patch<DIM, SCHEME_RUNTIME, LIM, T, MAXF, fused_lds_normals<DIM>()>(a.g, fa, roe, p, h, I.pad, I.begin,
+                                                                                I.end, a.u.nc, f0, L, cl, []() {});
         UpdArgs ua = a.u;
         ua.bad = a.u.bad + I.cs;
         ua.dt = a.dts + I.cs;
@@ -653,7 +653,8 @@ int build_patches(mst_batch *b) {
     if (rc) return rc;
     b->max_pf = (int)max_pf;
     b->patch_ok = true;
-    b->fused_ok = max_pf * b->nv * 16 <= 65536;  // the fused kernel keeps states and fluxes in LDS
+    // the fused kernel keeps states and fluxes (2D: and the faces' normals) in LDS
+    b->fused_ok = fused_lds_bytes(b->dim, b->nv, (int)max_pf, false, b->dim == 2) <= 65536;
     if (b->fused_ok && !hcell.empty()) {
         const IncView<double> f64{b->inc_code, b->inc_w, b->inc_n, b->inc_dx, b->cvol, b->NC};
         const IncView<float> f32{b->mixed ? b->inc_code : nullptr, b->inc_w32, b->inc_n32, b->inc_dx32, b->cvol, b->NC};
@@ -929,7 +930,7 @@ int run_fused(mst_batch *b, int stage, int is, int in, double ca, double cb, int
     ha.cell = b->hcell;
     ha.loc = b->hloc;
     b->hg.args<T>(ha);
-    const size_t shm = (size_t)FusedLds<T>::bytes_per_face_field * b->nv * b->max_pf;
+    const size_t shm = fused_lds_bytes(DIM, b->nv, b->max_pf, std::is_same<T, float>::value, fused_lds_normals<DIM>());
 #define MST_BFUSED(LM, MF) return launch_persistent_shm(b, kb_fused<DIM, LM, T, MF>, BLOCK, shm, a.n_items, a, pa, ha);
 #define MST_BFUSED_L(MF)                                          \
     if (b->limiter == MST_LIM_NONE) { MST_BFUSED(MST_LIM_NONE, MF) } \
