@@ -933,6 +933,81 @@ inline size_t fused_lds_bytes(int dim, int nv, int max_pf, bool f32, bool staged
     return states + (staged_normals ? (size_t)8 * dim * max_pf : 0);
 }
 
+// Order 2, FP32 records (the mixed form): the patch kernel with its face states staged in LDS.  Each
+// lane copies its cell's slot records into the LDS slots of their faces (one coalesced run per slot),
+// the halo's lanes the outside sides of the border faces; the faces are then solved from LDS as in the
+// fused kernel.  It reads neither the faces' record indices nor, for interior faces, their cells, and
+// each record once per patch; the inputs of every face's solve are the records k_flux would read, so
+// the results are bit-identical.  LDS: the fused kernel's FP32 layout, 8 * NV bytes per face (the
+// size of the FP64 fluxes that overwrite them).
+template <int DIM, int SCHEME, int MAXF>
+__global__ __launch_bounds__(BLOCK) void k_patch_rec(const FluxArgs<float> fa, const UpdArgs a, const PatchArgs p,
+                                                     const HaloArgs h) {
+    constexpr int NV = DIM + 2;
+    extern __shared__ double gsh[];
+    const int64_t blk = block_id();
+    const int32_t f0 = p.start[blk];
+    const int npf = p.start[blk + 1] - f0;
+    FusedLds<float> L(gsh, npf, NV);
+    const int64_t c = blk * BLOCK + threadIdx.x;
+    const bool valid = c < a.nc;
+    int cl[MAXF];
+#pragma unroll
+    for (int s = 0; s < MAXF; s++) cl[s] = valid ? p.loc[(int64_t)s * a.nc + c] : -1;
+    const int32_t h0 = h.start[blk];
+    const int nh = h.start[blk + 1] - h0;
+#pragma unroll
+    for (int s = 0; s < MAXF; s++) {
+        const int l = cl[s];
+        if (l < 0) continue;
+        float q[NV];
+        ldq<NV>(fa.rec + ((int64_t)s * a.nc + c) * NV, q);
+#pragma unroll
+        for (int k = 0; k < NV; k++) L.r(l & 1, k, l >> 1, NV) = q[k];
+    }
+    for (int j = threadIdx.x; j < nh; j += BLOCK) {
+        const int64_t hc = h.cell[h0 + j];
+#pragma unroll
+        for (int s = 0; s < MAXF; s++) {
+            const int l = h.loc[(int64_t)(h0 + j) * MAXF + s];
+            if (l < 0) continue;
+            float q[NV];
+            ldq<NV>(fa.rec + ((int64_t)s * a.nc + hc) * NV, q);
+#pragma unroll
+            for (int k = 0; k < NV; k++) L.r(l & 1, k, l >> 1, NV) = q[k];
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < npf; i += BLOCK) {
+        const int64_t f = p.face[f0 + i];
+        const int info = fa.finfo[f];
+        const int kind = info & 15;
+        double n64[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) n64[d] = fa.fn[f * DIM + d];
+        float X[NV], Y[NV];
+        if (kind == mst::KIND_INTERIOR || kind == mst::KIND_PERIODIC) {
+#pragma unroll
+            for (int k = 0; k < NV; k++) {
+                X[k] = L.r(0, k, i, NV);
+                Y[k] = L.r(1, k, i, NV);
+            }
+        } else {
+            ldq<NV>(fa.Q + (int64_t)fa.fc0[f] * NV, X);
+#pragma unroll
+            for (int k = 0; k < NV; k++) Y[k] = L.r(0, k, i, NV);
+        }
+        double g[NV];
+        face_solve<DIM, SCHEME, float>(fa.P, SCHEME == MST_SCHEME_ROE, kind, info >> 4, n64, X, Y, g);
+#pragma unroll
+        for (int k = 0; k < NV; k++) L.g[k * npf + i] = g[k];
+    }
+    __syncthreads();
+    double r[NV];
+    patch_sum<NV, MAXF, true>(L.g, npf, cl, r);
+    stage_tail<NV, true, true>(a, c, valid, r, blk, a.l2, a.dtred);
+}
+
 // Resident waves per SIMD requested for the fused kernel, where the bound costs no more than a register
 // of spill: FP64 triangles with Roe or unlimited AUSM 4 (at most 128 VGPRs; the compiler would keep 3
 // and the kernel, bound by its dependent load rounds, runs about 15 % slower), FP32 triangles 6 (at

@@ -390,6 +390,20 @@ int run_patch_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int 
     const size_t shm = (size_t)x->max_pf * x->nv * sizeof(double);
     const bool second = x->cfg.order >= 2;
     const bool roe = x->cfg.scheme == MST_SCHEME_ROE;
+    if constexpr (std::is_same<T, float>::value) {
+        if (second) {  // the mixed form stages its face states in LDS from the records (k_patch_rec)
+            HaloArgs ha{};
+            ha.start = x->hstart;
+            ha.cell = x->hcell;
+            ha.loc = x->hloc;
+#define MST_PATCH_REC(MF)                                                                                      \
+    return roe ? launch_shm(x, MST_K_PATCH, k_patch_rec<DIM, MST_SCHEME_ROE, MF>, x->nc, shm, fa, a, pa, ha) \
+               : launch_shm(x, MST_K_PATCH, k_patch_rec<DIM, MST_SCHEME_AUSM, MF>, x->nc, shm, fa, a, pa, ha);
+            if (x->slots == 3) { MST_PATCH_REC(3) }
+            MST_PATCH_REC(4)
+#undef MST_PATCH_REC
+        }
+    }
 #define MST_PATCH(MF)                                                                                          \
     if (roe)                                                                                                   \
         return second ? launch_shm(x, MST_K_PATCH, k_patch<DIM, MST_SCHEME_ROE, true, T, MF>, x->nc, shm, fa, a, pa) \
