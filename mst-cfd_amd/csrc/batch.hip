@@ -895,6 +895,7 @@ int run_patch(mst_batch *b, int stage, int is, int in, double ca, double cb, int
     pa.start = b->pstart;
     pa.face = b->pface;
     pa.loc = b->ploc;
+    pa.fi = nullptr;  // the batch reads its faces' kinds from finfo
     const size_t shm = (size_t)b->max_pf * b->nv * sizeof(double);
 #define MST_BPATCH(MF)                                                                                        \
     if (b->order >= 2) return launch_persistent_shm(b, kb_patch<DIM, true, T, MF>, BLOCK, shm, a.n_items, a, pa); \
@@ -925,6 +926,7 @@ int run_fused(mst_batch *b, int stage, int is, int in, double ca, double cb, int
     pa.start = b->pstart;
     pa.face = b->pface;
     pa.loc = b->ploc;
+    pa.fi = nullptr;  // the batch reads its faces' kinds from finfo
     HaloArgs ha;
     ha.start = b->hstart;
     ha.cell = b->hcell;

@@ -742,7 +742,26 @@ struct PatchArgs {
     const int32_t *__restrict__ start;  // nblk + 1: each block's range in face
     const int32_t *__restrict__ face;   // the faces of each patch that carry a flux, ascending
     const int16_t *__restrict__ loc;    // slot-major per cell: (local face << 1) | is_c1, -1: no flux
+    // the same entries with their faces' kind and L/R mask packed in (face | kind << 26 | mask << 29),
+    // so a patch face costs one load instead of two dependent ones; null: read face, then finfo
+    const uint32_t *__restrict__ fi;
 };
+
+constexpr int FI_SHIFT = 26;
+
+// patch face entry j: its face and finfo word (kind | L/R mask << 4)
+__device__ __forceinline__ void patch_face(const PatchArgs &p, const int32_t *__restrict__ finfo, int64_t j, int64_t &f,
+                                           int &info) {
+    if (p.fi) {
+        const uint32_t v = p.fi[j];
+        f = (int64_t)(v & ((1u << FI_SHIFT) - 1));
+        info = (int)((v >> FI_SHIFT) & 7u) | (int)(v >> (FI_SHIFT + 3)) << 4;
+    } else {
+        f = p.face[j];
+        info = finfo[f];
+    }
+}
+
 
 // sum of the lane's cell's face fluxes from LDS, in slot order (the face strategy's order)
 template <int NV, int MAXF, bool F32>
@@ -775,8 +794,9 @@ __global__ __launch_bounds__(BLOCK) void k_patch(const FluxArgs<T> fa, const Upd
     const int32_t f0 = p.start[blk];
     const int npf = p.start[blk + 1] - f0;
     for (int i = threadIdx.x; i < npf; i += BLOCK) {
-        const int64_t f = p.face[f0 + i];
-        const int info = fa.finfo[f];
+        int64_t f;
+        int info;
+        patch_face(p, fa.finfo, f0 + i, f, info);
         double g[NV];
         face_flux<DIM, SCHEME, SECOND, T>(fa, f, info & 15, info >> 4, g);
 #pragma unroll
@@ -979,8 +999,9 @@ __global__ __launch_bounds__(BLOCK) void k_patch_rec(const FluxArgs<float> fa, c
     }
     __syncthreads();
     for (int i = threadIdx.x; i < npf; i += BLOCK) {
-        const int64_t f = p.face[f0 + i];
-        const int info = fa.finfo[f];
+        int64_t f;
+        int info;
+        patch_face(p, fa.finfo, f0 + i, f, info);
         const int kind = info & 15;
         double n64[DIM];
 #pragma unroll
@@ -1046,8 +1067,7 @@ __device__ __forceinline__ void fused_patch(const GradArgs<T> &ga, const FluxArg
         pf_f[q] = -1;
         pf_info[q] = 0;
         if (i < npf) {
-            pf_f[q] = p.face[f0 + i];
-            pf_info[q] = fa.finfo[pf_f[q]];
+            patch_face(p, fa.finfo, f0 + i, pf_f[q], pf_info[q]);
 #pragma unroll
             for (int d = 0; d < DIM; d++) pf_n[q][d] = fa.fn[pf_f[q] * DIM + d];
         }
@@ -1163,8 +1183,7 @@ __device__ __forceinline__ void fused_patch(const GradArgs<T> &ga, const FluxArg
 #pragma unroll
                 for (int d = 0; d < DIM; d++) n64[d] = pf_n[q][d];
         } else {
-            f = p.face[f0 + i];
-            info = fa.finfo[f];
+            patch_face(p, fa.finfo, f0 + i, f, info);
             if constexpr (!NRM_LDS)
 #pragma unroll
                 for (int d = 0; d < DIM; d++) n64[d] = fa.fn[f * DIM + d];

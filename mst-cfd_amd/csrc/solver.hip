@@ -76,6 +76,7 @@ struct mst_ctx {
     // patch strategy: per block of BLOCK cells its flux-carrying faces, per cell slot the local index
     int32_t *pstart = nullptr, *pface = nullptr;
     int16_t *ploc = nullptr;
+    uint32_t *pfi = nullptr;  // pface with the faces' kinds and L/R masks packed in (null: 2^26 faces or more)
     int max_pf = 0;        // most faces of one patch (sizes the kernel's LDS)
     int64_t n_psolve = 0;  // Riemann problems per patch launch
     bool patch_ok = false;
@@ -387,6 +388,7 @@ int run_patch_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int 
     pa.start = x->pstart;
     pa.face = x->pface;
     pa.loc = x->ploc;
+    pa.fi = x->pfi;
     const size_t shm = (size_t)x->max_pf * x->nv * sizeof(double);
     const bool second = x->cfg.order >= 2;
     const bool roe = x->cfg.scheme == MST_SCHEME_ROE;
@@ -424,6 +426,7 @@ int run_patch_residual_t(mst_ctx *x, int mode, int is) {
     pa.start = x->pstart;
     pa.face = x->pface;
     pa.loc = x->ploc;
+    pa.fi = x->pfi;
     const size_t shm = (size_t)x->max_pf * x->nv * sizeof(double);
     const bool second = x->cfg.order >= 2;
     const bool roe = x->cfg.scheme == MST_SCHEME_ROE;
@@ -454,6 +457,7 @@ int run_fused_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int 
     pa.start = x->pstart;
     pa.face = x->pface;
     pa.loc = x->ploc;
+    pa.fi = x->pfi;
     HaloArgs ha;
     ha.start = x->hstart;
     ha.cell = x->hcell;
@@ -703,6 +707,7 @@ int build_patches(mst_ctx *x) {
     if (max_pf * x->nv * 16 > 65536) return MST_OK;
     int rc = dupload(&x->pstart, start);
     if (!rc) rc = dupload(&x->pface, face);
+    if (!rc && x->nf < ((int64_t)1 << FI_SHIFT)) rc = dalloc(&x->pfi, face.size());  // packed by upload_config
     if (!rc) rc = dupload(&x->ploc, loc);
     if (!rc) rc = dupload(&x->hstart, hstart);
     if (!rc) rc = dupload(&x->hcell, hcell);
@@ -766,12 +771,28 @@ int ensure_halo_geo(mst_ctx *x) {
     return MST_OK;
 }
 
+// the packed entries of patch_face, from the face lists and the current finfo (kind < 8, mask < 8)
+__global__ __launch_bounds__(BLOCK) void k_pack_faces(int64_t n, const int32_t *__restrict__ face,
+                                                      const int32_t *__restrict__ finfo, uint32_t *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const int32_t f = face[i];
+    const int info = finfo[f];
+    out[i] = (uint32_t)f | (uint32_t)(info & 7) << FI_SHIFT | (uint32_t)((info >> 4) & 7) << (FI_SHIFT + 3);
+}
+
 int upload_config(mst_ctx *x) {
     std::vector<int32_t> mask;
     mst::lr_masks(x->prep.P, x->cfg.compat, mask);
     std::vector<int32_t> info(x->nf);
     for (int64_t i = 0; i < x->nf; i++) info[i] = x->prep.P.fkind[i] | (mask[i] << 4);
     HIPCHK(hipMemcpy(x->finfo, info.data(), sizeof(int32_t) * x->nf, hipMemcpyHostToDevice));
+    if (x->pfi && x->n_psolve > 0) {  // the patch face entries with the new kinds and masks
+        const int64_t n = x->n_psolve;
+        hipLaunchKernelGGL(k_pack_faces, dim3((uint32_t)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, x->stream, n,
+                           (const int32_t *)x->pface, (const int32_t *)x->finfo, x->pfi);
+        HIPCHK(hipGetLastError());
+    }
     std::vector<double> eta = x->prep.P.feta;
     if ((x->cfg.compat & MST_COMPAT_GRAD_LASTINT) && x->prep.P.quirk_face >= 0) eta[x->prep.P.quirk_face] = 1.0;
     HIPCHK(hipMemcpy(x->feta, eta.data(), sizeof(double) * x->nf, hipMemcpyHostToDevice));
@@ -797,7 +818,7 @@ void release(mst_ctx *x) {
                     x->cshalf, x->Q[0], x->Q[1], x->Q[2], x->Q[3], x->G, x->rec, x->graw, x->phi, x->R,
                     x->scal, x->part, x->fin_mid, x->bad, x->Q32[0], x->Q32[1], x->Q32[2], x->Q32[3], x->rec32,
                     x->frec0, x->frec1, x->inc_code, x->inc_w, x->inc_n, x->inc_dx, x->inc_w32, x->inc_n32,
-                    x->inc_dx32, x->pstart, x->pface, x->ploc, x->hstart, x->hcell, x->hloc,
+                    x->inc_dx32, x->pstart, x->pface, x->pfi, x->ploc, x->hstart, x->hcell, x->hloc,
                     x->lu_cells, x->lu_nb, x->lu_face, x->lu_n, x->lu_r, x->lu_D, x->lu_dQ, x->gw};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -1687,14 +1708,15 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
             *bytes = flux_in + upd - (mixed ? 0 : (second ? nbnd : nc) * 8 * NV);
             break;
         case MST_K_PATCH:  // as the cell kernel, with the patch face lists and local slots for the table
+            // (a face's kind comes with its packed list entry when there is one)
             *bytes = flux_in + upd - (mixed ? 0 : (second ? nbnd : nc) * 8 * NV) - ell +
-                     4.0 * (double)ctx->n_psolve + 2.0 * ctx->slots * nc;
+                     4.0 * (double)ctx->n_psolve + 2.0 * ctx->slots * nc - (ctx->pfi ? 4.0 * nf : 0.0);
             break;
         case MST_K_FUSED: {  // gradient inputs + per-face inputs + update, each once; no records
             // per incidence: code, weight, offset, and in 3D the normal (2D takes it from its face's)
             const double inc_geo = ctx->dim == 2 ? r + r * D : r + 2 * r * D;
             const double grad_in = 4.0 * ctx->slots * nc + ninc * inc_geo + nc * (r * NV + 8);
-            const double flux_f = nf * (4 + 8 * D) + 4.0 * (double)ctx->n_psolve + nbnd * 4;
+            const double flux_f = nf * ((ctx->pfi ? 0 : 4) + 8 * D) + 4.0 * (double)ctx->n_psolve + nbnd * 4;
             *bytes = grad_in + flux_f + upd - ell + 2.0 * ctx->slots * nc - (mixed ? 0 : nc * 8 * NV);
             break;
         }
