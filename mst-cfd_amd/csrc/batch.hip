@@ -233,8 +233,9 @@ __global__ __launch_bounds__(BLOCK) void kb_patch(const BArgs<T> a, const PatchA
         const int32_t f0 = p.start[I.pad];
         const int npf = p.start[I.pad + 1] - f0;
         for (int i = threadIdx.x; i < npf; i += BLOCK) {
-            const int64_t f = p.face[f0 + i];
-            const int info = fa.finfo[f];
+            int64_t f;
+            int info;
+            patch_face(p, fa.finfo, f0 + i, f, info);
             const int kind = info & 15;
             double n64[DIM];
             T X[NV], Y[NV];
@@ -513,6 +514,7 @@ struct mst_batch {
                                        // PATCH: patch kernel over BLOCK-cell items
     // patch tables (global indices): per patch its flux-carrying faces, per cell slot the local index
     int32_t *pstart = nullptr, *pface = nullptr;
+    uint32_t *pfi = nullptr;  // pface with the faces' kinds and L/R masks packed in (null: 2^26 faces or more)
     int16_t *ploc = nullptr;
     std::vector<int32_t> patch_base;  // first patch of each case
     int max_pf = 0;
@@ -570,7 +572,7 @@ void release(mst_batch *b) {
                     b->frec0, b->frec1, b->rec32, b->P64, b->P32, b->scheme, b->dts, b->gam,
                     b->cfl, b->l2, b->ncells, b->citems, b->fitems, b->ci0, b->ci1, b->part, b->cnt, b->bad,
                     b->inc_code, b->inc_w, b->inc_n, b->inc_dx, b->inc_w32, b->inc_n32, b->inc_dx32,
-                    b->pstart, b->pface, b->ploc, b->hstart, b->hcell, b->hloc, b->lu_cells, b->lu_nb,
+                    b->pstart, b->pface, b->pfi, b->ploc, b->hstart, b->hcell, b->hloc, b->lu_cells, b->lu_nb,
                     b->lu_face, b->lu_pcase, b->lu_fcase, b->lu_n, b->lu_r, b->lu_D, b->lu_dQ, b->lu_cstart,
                     b->lu_active};
     for (void *p : bufs)
@@ -646,6 +648,15 @@ int build_patches(mst_batch *b) {
     if (max_pf * b->nv * sizeof(double) > 65536) return MST_OK;
     int rc = dupload(&b->pstart, start);
     if (!rc) rc = dupload(&b->pface, face);
+    if (!rc && b->NF < ((int64_t)1 << FI_SHIFT) && !face.empty()) {  // finfo is final for a batch
+        rc = dalloc(&b->pfi, face.size());
+        if (!rc) {
+            const int64_t n = (int64_t)face.size();
+            hipLaunchKernelGGL(k_pack_faces<0>, dim3((uint32_t)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, b->stream, n,
+                               (const int32_t *)b->pface, (const int32_t *)b->finfo, b->pfi);
+            HIPCHK(hipGetLastError());
+        }
+    }
     if (!rc) rc = dupload(&b->ploc, loc);
     if (!rc) rc = dupload(&b->hstart, hstart);
     if (!rc) rc = dupload(&b->hcell, hcell);
@@ -895,7 +906,7 @@ int run_patch(mst_batch *b, int stage, int is, int in, double ca, double cb, int
     pa.start = b->pstart;
     pa.face = b->pface;
     pa.loc = b->ploc;
-    pa.fi = nullptr;  // the batch reads its faces' kinds from finfo
+    pa.fi = b->pfi;
     const size_t shm = (size_t)b->max_pf * b->nv * sizeof(double);
 #define MST_BPATCH(MF)                                                                                        \
     if (b->order >= 2) return launch_persistent_shm(b, kb_patch<DIM, true, T, MF>, BLOCK, shm, a.n_items, a, pa); \
@@ -926,7 +937,7 @@ int run_fused(mst_batch *b, int stage, int is, int in, double ca, double cb, int
     pa.start = b->pstart;
     pa.face = b->pface;
     pa.loc = b->ploc;
-    pa.fi = nullptr;  // the batch reads its faces' kinds from finfo
+    pa.fi = b->pfi;
     HaloArgs ha;
     ha.start = b->hstart;
     ha.cell = b->hcell;

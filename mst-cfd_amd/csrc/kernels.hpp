@@ -762,6 +762,17 @@ __device__ __forceinline__ void patch_face(const PatchArgs &p, const int32_t *__
     }
 }
 
+// the packed entries of patch_face, from the face lists and the current finfo (kind < 8, mask < 8)
+template <int U>
+__global__ __launch_bounds__(BLOCK) void k_pack_faces(int64_t n, const int32_t *__restrict__ face,
+                                                      const int32_t *__restrict__ finfo, uint32_t *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const int32_t f = face[i];
+    const int info = finfo[f];
+    out[i] = (uint32_t)f | (uint32_t)(info & 7) << FI_SHIFT | (uint32_t)((info >> 4) & 7) << (FI_SHIFT + 3);
+}
+
 
 // sum of the lane's cell's face fluxes from LDS, in slot order (the face strategy's order)
 template <int NV, int MAXF, bool F32>

@@ -771,16 +771,6 @@ int ensure_halo_geo(mst_ctx *x) {
     return MST_OK;
 }
 
-// the packed entries of patch_face, from the face lists and the current finfo (kind < 8, mask < 8)
-__global__ __launch_bounds__(BLOCK) void k_pack_faces(int64_t n, const int32_t *__restrict__ face,
-                                                      const int32_t *__restrict__ finfo, uint32_t *__restrict__ out) {
-    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const int32_t f = face[i];
-    const int info = finfo[f];
-    out[i] = (uint32_t)f | (uint32_t)(info & 7) << FI_SHIFT | (uint32_t)((info >> 4) & 7) << (FI_SHIFT + 3);
-}
-
 int upload_config(mst_ctx *x) {
     std::vector<int32_t> mask;
     mst::lr_masks(x->prep.P, x->cfg.compat, mask);
@@ -789,7 +779,7 @@ int upload_config(mst_ctx *x) {
     HIPCHK(hipMemcpy(x->finfo, info.data(), sizeof(int32_t) * x->nf, hipMemcpyHostToDevice));
     if (x->pfi && x->n_psolve > 0) {  // the patch face entries with the new kinds and masks
         const int64_t n = x->n_psolve;
-        hipLaunchKernelGGL(k_pack_faces, dim3((uint32_t)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, x->stream, n,
+        hipLaunchKernelGGL(k_pack_faces<0>, dim3((uint32_t)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, x->stream, n,
                            (const int32_t *)x->pface, (const int32_t *)x->finfo, x->pfi);
         HIPCHK(hipGetLastError());
     }
