@@ -127,6 +127,20 @@ def initial_error(s, m, mach, theta, net, Q_steady):
     return e(fs), e(Qn)
 
 
+def initial_residual(s, m, mach, theta, net):
+    """L2(R/V) per field of the freestream and of the network's prediction: the level each start's
+    residual has to fall from.  A driver whose residual falls at a fixed rate per iteration needs
+    log(start / target) / log(rate) iterations, so a start saves the fraction
+    log(freestream / prediction) / log(freestream / target) of them at most."""
+    V = s.prep.geometry()["vol"][:, None]
+    norm = lambda R: np.sqrt(((R / V) ** 2).mean(0))
+    s.set_state(freestream(m, mach))
+    r_fs = norm(s.residual())
+    s.warm_start(net, [mach, theta])
+    r_net = norm(s.residual())
+    return r_fs, r_net
+
+
 def time_to_steady(s, m, mach, theta, net=None, ref=None, tol=1e-8, max_steps=60000, check_every=200):
     """One run of solver s (mesh m) from the freestream (net None) or from the network's prediction;
     returns steps, seconds (prediction included), final ratio and the reference norms.  A prediction
@@ -167,12 +181,14 @@ def evaluate(net, cases, nx, ny, tol=1e-8, max_steps=60000, check_every=200, lus
         s = Solver(m, config(m, M))
         cold = time_to_steady(s, m, M, th, None, None, tol, max_steps, check_every)
         e_fs, e_net = initial_error(s, m, M, th, net, s.get_state())
+        r_fs, r_net = initial_residual(s, m, M, th, net)
         warm = time_to_steady(s, m, M, th, net, cold["ref"], tol, max_steps, check_every)
         row = dict(mach=M, theta=th, cold_steps=cold["steps"], cold_s=cold["seconds"], cold_ratio=cold["ratio"],
                    warm_steps=warm["steps"], warm_s=warm["seconds"], warm_ratio=warm["ratio"],
                    warm_start=warm["start"], converged=bool(cold["converged"] and warm["converged"]),
                    speedup=cold["seconds"] / warm["seconds"],
-                   freestream_error=e_fs.tolist(), network_error=e_net.tolist())
+                   freestream_error=e_fs.tolist(), network_error=e_net.tolist(),
+                   freestream_residual=r_fs.tolist(), network_residual=r_net.tolist())
         if lusgs is not None:
             s.set_steady_driver(STEADY_LUSGS, *lusgs)
             s.lusgs_colours()  # the colouring is built once per mesh, outside the timed runs
@@ -194,8 +210,11 @@ def summary(rows):
     sp = np.array([r["speedup"] for r in ok]) if ok else np.array([np.nan])
     st = np.array([r["cold_steps"] / r["warm_steps"] for r in ok]) if ok else np.array([np.nan])
     gain = np.array([np.mean(r["freestream_error"]) / np.mean(r["network_error"]) for r in rows])
+    # largest over the fields of the prediction's initial L2(R/V) over the freestream's
+    rres = np.array([max(b / a for a, b in zip(r["freestream_residual"], r["network_residual"])) for r in rows])
     out = dict(cases=len(rows), converged=len(ok), geomean_speedup=float(np.exp(np.log(sp).mean())),
                geomean_initial_error_reduction=float(np.exp(np.log(gain).mean())),
+               geomean_initial_residual_ratio=float(np.exp(np.log(rres).mean())),
                median_speedup=float(np.median(sp)), geomean_step_ratio=float(np.exp(np.log(st).mean())),
                total_cold_s=float(sum(r["cold_s"] for r in ok)), total_warm_s=float(sum(r["warm_s"] for r in ok)))
     lu = [r for r in rows if r.get("lu_converged") and r["converged"]]
