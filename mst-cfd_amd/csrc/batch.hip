@@ -479,6 +479,9 @@ int bucket_of(int64_t nc) {
 struct mst_batch {
     int n = 0, dim = 2, nv = 4, maxf = 3, slots = 3, order = 1, limiter = 0, rk = 1;
     bool mixed = false;
+    // the stage being enqueued: pre-combined stage states (mixed form) and the successor's coefficients
+    int upd_pre = 0;
+    double upd_wa = 0, upd_wb = 0;
     std::vector<mst_config> cfgs;
     std::vector<mst::Prepared> preps;
     std::vector<int64_t> cstart, fstart;
@@ -495,6 +498,7 @@ struct mst_batch {
     int32_t *inc_code = nullptr;  // gradient incidence layout (slot-major, stride NC)
     double *inc_w = nullptr, *inc_n = nullptr, *inc_dx = nullptr;
     float *inc_w32 = nullptr, *inc_n32 = nullptr, *inc_dx32 = nullptr;
+    float *ivol32 = nullptr;  // (float) 1/V, the FP32 gradient's volume input
     Params<double> *P64 = nullptr;
     Params<float> *P32 = nullptr;
     int32_t *scheme = nullptr;
@@ -571,7 +575,7 @@ void release(mst_batch *b) {
                     b->Q[0], b->Q[1], b->Q[2], b->Q[3], b->G, b->rec, b->Q32[0], b->Q32[1], b->Q32[2], b->Q32[3],
                     b->frec0, b->frec1, b->rec32, b->P64, b->P32, b->scheme, b->dts, b->gam,
                     b->cfl, b->l2, b->ncells, b->citems, b->fitems, b->ci0, b->ci1, b->part, b->cnt, b->bad,
-                    b->inc_code, b->inc_w, b->inc_n, b->inc_dx, b->inc_w32, b->inc_n32, b->inc_dx32,
+                    b->inc_code, b->inc_w, b->inc_n, b->inc_dx, b->inc_w32, b->inc_n32, b->inc_dx32, b->ivol32,
                     b->pstart, b->pface, b->pfi, b->ploc, b->hstart, b->hcell, b->hloc, b->lu_cells, b->lu_nb,
                     b->lu_face, b->lu_pcase, b->lu_fcase, b->lu_n, b->lu_r, b->lu_D, b->lu_dQ, b->lu_cstart,
                     b->lu_active};
@@ -668,7 +672,7 @@ int build_patches(mst_batch *b) {
     b->fused_ok = fused_lds_bytes(b->dim, b->nv, (int)max_pf, false, b->dim == 2) <= 65536;
     if (b->fused_ok && !hcell.empty()) {
         const IncView<double> f64{b->inc_code, b->inc_w, b->inc_n, b->inc_dx, b->cvol, b->NC};
-        const IncView<float> f32{b->mixed ? b->inc_code : nullptr, b->inc_w32, b->inc_n32, b->inc_dx32, b->cvol, b->NC};
+        const IncView<float> f32{b->mixed ? b->inc_code : nullptr, b->inc_w32, b->inc_n32, b->inc_dx32, b->ivol32, b->NC};
         const int64_t nh = (int64_t)hcell.size();
         HIPCHK(b->dim == 2 ? b->hg.fill<2>(b->slots, b->NC, nh, b->hcell, f64, f32, b->stream)
                           : b->hg.fill<3>(b->slots, b->NC, nh, b->hcell, f64, f32, b->stream));
@@ -735,7 +739,6 @@ BArgs<T> base_args(const mst_batch *b, int isrc) {
     GradArgs<T> &g = a.g;
     g.nc = b->NC;
     g.inc_code = b->inc_code;
-    g.vol = b->cvol;
     g.graw = nullptr;
     g.phi = nullptr;
     FluxArgs<T> &f = a.f;
@@ -753,6 +756,7 @@ BArgs<T> base_args(const mst_batch *b, int isrc) {
         g.inc_dx = b->inc_dx32;
         g.Q = b->Q32[isrc];
         g.rec = b->rec32;
+        g.vol = b->ivol32;
         f.Q = b->Q32[isrc];
         f.rec = b->rec32;
         a.P = b->P32;
@@ -762,6 +766,7 @@ BArgs<T> base_args(const mst_batch *b, int isrc) {
         g.inc_dx = b->inc_dx;
         g.Q = b->Q[isrc];
         g.rec = b->rec;
+        g.vol = b->cvol;
         f.Q = b->Q[isrc];
         f.rec = b->rec;
         a.P = b->P64;
@@ -774,6 +779,9 @@ BArgs<T> base_args(const mst_batch *b, int isrc) {
     u.sh = b->cshalf;
     u.maxf = b->maxf;
     u.bad = b->bad;
+    u.pre = b->upd_pre;
+    u.wa = b->upd_wa;
+    u.wb = b->upd_wb;
     a.scheme = b->scheme;
     a.dts = b->dts;
     a.gam = b->gam;
@@ -995,9 +1003,20 @@ int enqueue_step_t(mst_batch *b, int iold, int inew) {
     int spare[2], w = 0;
     for (int i = 0; i < 4; i++)
         if (i != iold && i != inew) spare[w++] = i;
+    // mixed form: a stage that has a successor stores the successor's combination A Q^n + B Q_s in place
+    // of its FP64 state (the FP32 kernels read only the shadow), so no later stage reads Q^n again
+    // (SSP-RK2: none; RK3: the middle stage)
+    const bool pre = b->mixed;
+    struct Reset {
+        mst_batch *b;
+        ~Reset() { b->upd_pre = 0, b->upd_wa = b->upd_wb = 0; }
+    } reset{b};
     int src = iold;
     for (int s = 0; s < ns; s++) {
         const int dst = (s == ns - 1) ? inew : spare[s & 1];
+        b->upd_pre = pre;
+        b->upd_wa = pre && s < ns - 1 ? A[s + 1] : 0;
+        b->upd_wb = pre && s < ns - 1 ? B[s + 1] : 0;
         int rc;
         if (b->strategy == MST_STRATEGY_FUSED && b->order >= 2) {
             rc = run_fused<DIM, T>(b, s, src, iold, A[s], B[s], s == 0 ? 1 : 0, dst, s == 0, s == ns - 1, s == ns - 1);
@@ -1526,6 +1545,13 @@ int mst_batch_create(int n_cases, const mst_mesh *const *meshes, const mst_confi
         TRY(dupload(&b->inc_w32, inc_w32));
         TRY(dupload(&b->inc_n32, n32));
         TRY(dupload(&b->inc_dx32, d32));
+        TRY(dalloc(&b->ivol32, b->NC));
+        hipLaunchKernelGGL(k_recip_vol<0>, dim3((uint32_t)((b->NC + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, b->stream,
+                           b->NC, (const double *)b->cvol, b->ivol32);
+        if (hipGetLastError() != hipSuccess) {
+            release(b);
+            return mst::fail(MST_E_HIP, "k_recip_vol launch failed");
+        }
     }
     TRY(dupload(&b->P64, p64));
     TRY(dupload(&b->P32, p32));

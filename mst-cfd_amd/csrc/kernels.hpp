@@ -194,6 +194,11 @@ __global__ __launch_bounds__(BLOCK, MST_FLUX_WAVES) void k_flux(const FluxArgs<T
 }
 
 // ------------------------------------------------------------------------------------ gradient + limiter
+// The gradient's volume input: FP64 the cell volume, FP32 its reciprocal (float) frcp(V), computed
+// once per mesh (k_recip_vol) with the arithmetic the FP64 form applies to V.
+template <typename T>
+using GradVol = typename std::conditional<std::is_same<T, float>::value, float, double>::type;
+
 template <typename T>
 struct GradArgs {
     int64_t nc;
@@ -206,7 +211,7 @@ struct GradArgs {
     const T *__restrict__ inc_n;
     const T *__restrict__ inc_dx;
     const T *__restrict__ Q;
-    const double *__restrict__ vol;
+    const GradVol<T> *__restrict__ vol;
     T *__restrict__ rec;  // out: reconstructed face state of every incidence, Q_c + phi grad Q . dx
     double *__restrict__ graw;
     double *__restrict__ phi;
@@ -224,7 +229,7 @@ template <typename T>
 struct IncView {
     const int32_t *code;
     const T *w, *n, *dx;
-    const double *vol;
+    const GradVol<T> *vol;
     int64_t stride;
 };
 
@@ -238,7 +243,7 @@ struct GradIn {
     int32_t code[MAXF];
     T eta[MAXF], n[MAXF][DIM];
     T qc[DIM + 2], qn[MAXF][DIM + 2];
-    double vol;
+    GradVol<T> vol;
 };
 
 template <int DIM, int MAXF, bool NRM, typename T>
@@ -310,7 +315,11 @@ __device__ __forceinline__ int grad_eval(const GradIn<DIM, MAXF, T> &in, const T
             for (int d = 0; d < DIM; d++) t[k][d] += qf[k] * in.n[s][d];
         nfc = s + 1;
     }
-    const T iV = (T)frcp(in.vol);
+    T iV;
+    if constexpr (F32)
+        iV = in.vol;
+    else
+        iV = frcp(in.vol);
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         ph[k] = 1;
@@ -422,6 +431,13 @@ struct UpdArgs {
     const double *__restrict__ dt;  // [0] dt of this stage
     double ca, cb;
     int stage0;
+    // mixed form (FP32 kernels read only the FP32 shadow, so an intermediate stage's FP64 state is
+    // needed only by the next stage's update): pre = Qs holds the stage's combination
+    // ca Q^n + cb Q_s, and the stage computes Qs - cb dt/V R; wa, wb (wb != 0 on a stage with a
+    // successor) = that successor's coefficients: Qout receives wa Q^n + wb Q_{s+1} and Qout32 the
+    // shadow of Q_{s+1} itself
+    int pre;
+    double wa, wb;
     int maxf;       // ELL slots (cell strategy)
     int l2, dtred;  // cell strategy: reduce |R/V|^2 / the CFL time step in this launch
     int check;      // flag rho <= 0 / p <= 0 / NaN in the output (the state that ends a step)
@@ -520,11 +536,13 @@ struct StageIn {
     double V, qs[NV], qn[NV];
 };
 
-template <int NV>
+template <int NV, bool F32>
 __device__ __forceinline__ void stage_load(const UpdArgs &a, int64_t c, StageIn<NV> &in) {
     in.V = a.vol[c];
     ldq<NV>(a.Qs + c * NV, in.qs);
-    if (!a.stage0) ldq<NV>(a.Qn + c * NV, in.qn);
+    // Q^n: the stage combination (FP64 form), or the successor's combination after the first stage
+    const bool pre = F32 && a.pre;
+    if (!a.stage0 && (!pre || a.wb != 0)) ldq<NV>(a.Qn + c * NV, in.qn);
 }
 
 template <int NV, bool F32>
@@ -533,15 +551,27 @@ __device__ __forceinline__ void stage_cell(const UpdArgs &a, int64_t c, const St
     constexpr int DIM = NV - 2;
     const double V = in.V;
     const double dtv = a.dt[0] / V;
+    const bool pre = F32 && a.pre;
     double out[NV];
-    if (a.stage0) {
+    if (pre) {  // Qs = ca Q^n + cb Q_s (cb = 1 on the first stage: the same arithmetic as below)
+        const double h = a.cb * dtv;
+#pragma unroll
+        for (int k = 0; k < NV; k++) out[k] = in.qs[k] - h * r[k];
+    } else if (a.stage0) {
 #pragma unroll
         for (int k = 0; k < NV; k++) out[k] = in.qs[k] - dtv * r[k];
     } else {
 #pragma unroll
         for (int k = 0; k < NV; k++) out[k] = a.ca * in.qn[k] + a.cb * (in.qs[k] - dtv * r[k]);
     }
-    stq<NV>(a.Qout + c * NV, out);
+    if (pre && a.wb != 0) {
+        double w[NV];
+#pragma unroll
+        for (int k = 0; k < NV; k++) w[k] = a.wa * (a.stage0 ? in.qs[k] : in.qn[k]) + a.wb * out[k];
+        stq<NV>(a.Qout + c * NV, w);
+    } else {
+        stq<NV>(a.Qout + c * NV, out);
+    }
     if constexpr (F32) stq<NV>(a.Qout32 + c * NV, out);
     if (a.check) {
         double ke = 0;
@@ -566,7 +596,7 @@ template <int NV, bool F32>
 __device__ __forceinline__ void stage_cell(const UpdArgs &a, int64_t c, const double (&r)[NV], double (&red)[NV + 1],
                                            const bool L2, const bool DT) {
     StageIn<NV> in;
-    stage_load<NV>(a, c, in);
+    stage_load<NV, F32>(a, c, in);
     stage_cell<NV, F32>(a, c, in, r, red, L2, DT);
 }
 
@@ -778,6 +808,12 @@ __global__ __launch_bounds__(BLOCK) void k_pack_faces(int64_t n, const int32_t *
     out[i] = (uint32_t)f | (uint32_t)(info & 7) << FI_SHIFT | (uint32_t)((info >> 4) & 7) << (FI_SHIFT + 3);
 }
 
+// The FP32 gradient's volume input: (float) frcp(V) per cell (GradVol)
+template <int U>
+__global__ __launch_bounds__(BLOCK) void k_recip_vol(int64_t n, const double *__restrict__ vol, float *__restrict__ out) {
+    const int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (c < n) out[c] = (float)frcp(vol[c]);
+}
 
 // sum of the lane's cell's face fluxes from LDS, in slot order (the face strategy's order)
 template <int NV, int MAXF, bool F32>
@@ -844,8 +880,7 @@ struct HaloArgs {
     // mostly unused line per slot and array for each scattered halo cell.  w, n, dx are of the
     // kernel's real type.
     const int32_t *__restrict__ code;
-    const void *w, *n, *dx;
-    const double *__restrict__ vol;
+    const void *w, *n, *dx, *vol;
     int64_t nh;
 };
 
@@ -853,7 +888,7 @@ struct HaloArgs {
 template <int DIM, typename T>
 __global__ __launch_bounds__(BLOCK) void k_halo_geo(int64_t nh, int slots, int64_t nc, const int32_t *__restrict__ hcell,
                                                     IncView<T> src, int32_t *__restrict__ code, T *__restrict__ w,
-                                                    T *__restrict__ n, T *__restrict__ dx, double *__restrict__ vol) {
+                                                    T *__restrict__ n, T *__restrict__ dx, GradVol<T> *__restrict__ vol) {
     const int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (j >= nh) return;
     const int64_t c = hcell[j];
@@ -874,10 +909,11 @@ __global__ __launch_bounds__(BLOCK) void k_halo_geo(int64_t nh, int slots, int64
 struct HaloGeo {
     int32_t *code = nullptr;
     double *w = nullptr, *n = nullptr, *dx = nullptr, *vol = nullptr;
-    float *w32 = nullptr, *n32 = nullptr, *dx32 = nullptr;
+    float *w32 = nullptr, *n32 = nullptr, *dx32 = nullptr, *vol32 = nullptr;
     int64_t nh = 0;
     void release() {
-        for (void *p : {(void *)code, (void *)w, (void *)n, (void *)dx, (void *)vol, (void *)w32, (void *)n32, (void *)dx32})
+        for (void *p : {(void *)code, (void *)w, (void *)n, (void *)dx, (void *)vol, (void *)w32, (void *)n32, (void *)dx32,
+                        (void *)vol32})
             if (p) (void)hipFree(p);
         *this = HaloGeo();
     }
@@ -902,6 +938,7 @@ struct HaloGeo {
             get(&w32, m);
             get(&n32, m * DIM);
             get(&dx32, m * DIM);
+            get(&vol32, (size_t)nh_);
         }
         if (e != hipSuccess) {
             release();
@@ -913,7 +950,7 @@ struct HaloGeo {
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if (f32.code) {
             hipLaunchKernelGGL((k_halo_geo<DIM, float>), grid, dim3(BLOCK), 0, st, nh, slots, nc, hcell, f32, code, w32,
-                               n32, dx32, vol);
+                               n32, dx32, vol32);
             e = hipGetLastError();
         }
         return e;
@@ -925,7 +962,7 @@ struct HaloGeo {
         h.w = F32 ? (const void *)w32 : (const void *)w;
         h.n = F32 ? (const void *)n32 : (const void *)n;
         h.dx = F32 ? (const void *)dx32 : (const void *)dx;
-        h.vol = vol;
+        h.vol = F32 ? (const void *)vol32 : (const void *)vol;
         h.nh = nh;
     }
 };
@@ -1097,7 +1134,7 @@ __device__ __forceinline__ void fused_patch(const GradArgs<T> &ga, const FluxArg
     const int32_t hc = BLOCK + (int)threadIdx.x < next ? h.cell[h0 + threadIdx.x] : 0;  // first halo pass
     const IncView<T> own = cell_view(ga);
     const IncView<T> hv{h.code, static_cast<const T *>(h.w), static_cast<const T *>(h.n),
-                        static_cast<const T *>(h.dx), h.vol, h.nh};
+                        static_cast<const T *>(h.dx), static_cast<const GradVol<T> *>(h.vol), h.nh};
     if constexpr (NRM_LDS) {
         // The patch cells' reads first (their normals excepted), then the patch faces' normals into
         // LDS, a barrier, and each incidence normal from LDS: + the face's normal on its c0 side, - on
@@ -1245,7 +1282,7 @@ __global__ __launch_bounds__(BLOCK, (fused_waves<DIM, SCHEME, LIM, T, MAXF>())) 
     StageIn<NV> in;
     fused_patch<DIM, SCHEME, LIM, T, MAXF, fused_lds_normals<DIM>()>(ga, fa, SCHEME == MST_SCHEME_ROE, p, h, blk, cb, ce,
                                                                       a.nc, f0, L, cl, [&]() {
-        if (valid) stage_load<NV>(a, c, in);
+        if (valid) stage_load<NV, F32>(a, c, in);
     });
     double r[NV];
     patch_sum<NV, MAXF, F32>(L.g, L.npf, cl, r);

@@ -59,6 +59,9 @@ struct mst_ctx {
     Params<double> p64;
     Params<float> p32;
     bool mixed = false;
+    // the stage being launched: pre-combined stage states (mixed form) and the successor's coefficients
+    int upd_pre = 0;
+    double upd_wa = 0, upd_wb = 0;
     int strategy = MST_STRATEGY_AUTO;
     int resolved = -1;  // strategy in use (AUTO: measured at the first stage after a configuration)
     hipStream_t stream = nullptr;
@@ -97,6 +100,7 @@ struct mst_ctx {
     int32_t *inc_code = nullptr;
     double *inc_w = nullptr, *inc_n = nullptr, *inc_dx = nullptr;
     float *inc_w32 = nullptr, *inc_n32 = nullptr, *inc_dx32 = nullptr;
+    float *ivol32 = nullptr;  // (float) 1/V, the FP32 gradient's volume input
     double *scal = nullptr;  // [0] dt, [1..5] l2 norms, [6..10] maxrel
     double *part = nullptr;  // per-block partials
     double *fin_mid = nullptr;  // first-level results of the two-level finish
@@ -180,14 +184,15 @@ GradArgs<T> grad_args(const mst_ctx *x, int isrc, bool debug) {
         a.inc_dx = x->inc_dx32;
         a.Q = x->Q32[isrc];
         a.rec = x->rec32;
+        a.vol = x->ivol32;
     } else {
         a.inc_w = x->inc_w;
         a.inc_n = x->inc_n;
         a.inc_dx = x->inc_dx;
         a.Q = x->Q[isrc];
         a.rec = x->rec;
+        a.vol = x->cvol;
     }
-    a.vol = x->cvol;
     a.graw = debug ? x->graw : nullptr;
     a.phi = debug ? x->phi : nullptr;
     return a;
@@ -312,6 +317,9 @@ UpdArgs upd_args(const mst_ctx *x, int mode, int is, int in, double ca, double c
     a.ca = ca;
     a.cb = cb;
     a.stage0 = stage0;
+    a.pre = x->upd_pre;
+    a.wa = x->upd_wa;
+    a.wb = x->upd_wb;
     a.maxf = x->maxf;
     a.l2 = (mode & UPD_L2) != 0;
     a.dtred = (mode & UPD_DT) != 0;
@@ -753,6 +761,11 @@ int upload_incidence_weights(mst_ctx *x, const std::vector<double> &eta) {
                 rc = dupload(pr.second, f);
                 if (rc) return rc;
             }
+            rc = dalloc(&x->ivol32, x->nc);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_recip_vol<0>, dim3((uint32_t)((x->nc + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, x->stream,
+                               x->nc, (const double *)x->cvol, x->ivol32);
+            HIPCHK(hipGetLastError());
         }
         HIPCHK(hipMemcpy(x->inc_w32, w32.data(), sizeof(float) * n, hipMemcpyHostToDevice));
     }
@@ -764,7 +777,7 @@ int upload_incidence_weights(mst_ctx *x, const std::vector<double> &eta) {
 int ensure_halo_geo(mst_ctx *x) {
     if (!x->hg_dirty || !x->patch_ok || x->n_halo == 0) return MST_OK;
     const IncView<double> f64{x->inc_code, x->inc_w, x->inc_n, x->inc_dx, x->cvol, x->nc};
-    const IncView<float> f32{x->mixed ? x->inc_code : nullptr, x->inc_w32, x->inc_n32, x->inc_dx32, x->cvol, x->nc};
+    const IncView<float> f32{x->mixed ? x->inc_code : nullptr, x->inc_w32, x->inc_n32, x->inc_dx32, x->ivol32, x->nc};
     HIPCHK(x->dim == 2 ? x->hg.fill<2>(x->slots, x->nc, x->n_halo, x->hcell, f64, f32, x->stream)
                        : x->hg.fill<3>(x->slots, x->nc, x->n_halo, x->hcell, f64, f32, x->stream));
     x->hg_dirty = false;
@@ -808,7 +821,7 @@ void release(mst_ctx *x) {
                     x->cshalf, x->Q[0], x->Q[1], x->Q[2], x->Q[3], x->G, x->rec, x->graw, x->phi, x->R,
                     x->scal, x->part, x->fin_mid, x->bad, x->Q32[0], x->Q32[1], x->Q32[2], x->Q32[3], x->rec32,
                     x->frec0, x->frec1, x->inc_code, x->inc_w, x->inc_n, x->inc_dx, x->inc_w32, x->inc_n32,
-                    x->inc_dx32, x->pstart, x->pface, x->pfi, x->ploc, x->hstart, x->hcell, x->hloc,
+                    x->inc_dx32, x->ivol32, x->pstart, x->pface, x->pfi, x->ploc, x->hstart, x->hcell, x->hloc,
                     x->lu_cells, x->lu_nb, x->lu_face, x->lu_n, x->lu_r, x->lu_D, x->lu_dQ, x->gw};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -1287,11 +1300,22 @@ int mst_step(mst_ctx *ctx, int nsteps) {
         for (int i = 0; i < 4; i++)
             if (i != ctx->iold && i != ctx->inew) spare[w++] = i;
         const bool l2 = it == nsteps - 1;
+        // mixed form: a stage that has a successor stores the successor's combination A Q^n + B Q_s in
+        // place of its FP64 state (the FP32 kernels read only the shadow), so no later stage reads Q^n
+        // again (SSP-RK2: none; RK3: the middle stage).  The viscous kernels read the FP64 stage state.
+        const bool pre = ctx->mixed && !ctx->viscous;
+        struct Reset {
+            mst_ctx *x;
+            ~Reset() { x->upd_pre = 0, x->upd_wa = x->upd_wb = 0; }
+        } reset{ctx};
         int src = ctx->iold;
         for (int s = 0; s < ns; s++) {
             const int dst = (s == ns - 1) ? ctx->inew : spare[s & 1];
             const int mode = UPD_STAGE | ((l2 && s == 0) ? UPD_L2 : 0) | ((cfl && s == ns - 1) ? UPD_DT : 0) |
                              (s == ns - 1 ? UPD_CHECK : 0);
+            ctx->upd_pre = pre;
+            ctx->upd_wa = pre && s < ns - 1 ? A[s + 1] : 0;
+            ctx->upd_wb = pre && s < ns - 1 ? B[s + 1] : 0;
             rc = run_stage(ctx, mode, src, ctx->iold, A[s], B[s], s == 0 ? 1 : 0, dst);
             if (rc) return rc;
             src = dst;
@@ -1680,8 +1704,10 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
     double flux_in = nf * (12 + 8 * D) + nc * r * NV;
     if (second) flux_in = nf * (12 + 8 * D + 8) + ninc * r * NV + nbnd * r * NV;
     // update: cell faces, stage state, volume, new state (+ FP32 shadow), Q^n on stages after the
-    // first, the CFL half-sums on the last stage when the time step is reduced there
-    double upd = ell + nc * (8 * NV + 8 + 8 * NV) + nc * 8 * NV * (stages - 1) / stages;
+    // first (mixed form: pre-combined stage states, Q^n only on RK3's middle stage), the CFL half-sums
+    // on the last stage when the time step is reduced there
+    const double qn_stages = mixed && !ctx->viscous ? std::max(stages - 2, 0.0) : stages - 1;
+    double upd = ell + nc * (8 * NV + 8 + 8 * NV) + nc * 8 * NV * qn_stages / stages;
     if (mixed) upd += nc * 4 * NV;
     if (ctx->cfg.cfl > 0) upd += nc * 8 * D / stages;
     switch (kernel) {
@@ -1689,7 +1715,7 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
             *bytes = flux_in + nf * 8 * NV;
             break;
         case MST_K_GRAD:  // incidences: code, weight, normal, offset, record out; cells: state, volume
-            *bytes = 4.0 * ctx->slots * nc + ninc * (r + 2 * r * D + r * NV) + nc * (r * NV + 8);
+            *bytes = 4.0 * ctx->slots * nc + ninc * (r + 2 * r * D + r * NV) + nc * (r * NV + r);
             break;
         case MST_K_UPDATE:
             *bytes = upd + nf * 8 * NV;
@@ -1705,7 +1731,7 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
         case MST_K_FUSED: {  // gradient inputs + per-face inputs + update, each once; no records
             // per incidence: code, weight, offset, and in 3D the normal (2D takes it from its face's)
             const double inc_geo = ctx->dim == 2 ? r + r * D : r + 2 * r * D;
-            const double grad_in = 4.0 * ctx->slots * nc + ninc * inc_geo + nc * (r * NV + 8);
+            const double grad_in = 4.0 * ctx->slots * nc + ninc * inc_geo + nc * (r * NV + r);
             const double flux_f = nf * ((ctx->pfi ? 0 : 4) + 8 * D) + 4.0 * (double)ctx->n_psolve + nbnd * 4;
             *bytes = grad_in + flux_f + upd - ell + 2.0 * ctx->slots * nc - (mixed ? 0 : nc * 8 * NV);
             break;
