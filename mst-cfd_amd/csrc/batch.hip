@@ -1452,6 +1452,7 @@ int mst_batch_create(int n_cases, const mst_mesh *const *meshes, const mst_confi
     std::vector<double> inc_w((size_t)b->slots * b->NC, 0.0), inc_n((size_t)b->slots * b->NC * dim, 0.0),
         inc_dx((size_t)b->slots * b->NC * dim, 0.0);
     std::vector<float> inc_w32(b->mixed ? (size_t)b->slots * b->NC : 0, 0.f);
+    std::vector<float> inc_n32(b->mixed ? (size_t)b->slots * b->NC * dim : 0, 0.f);
     std::vector<Params<double>> p64(n_cases);
     std::vector<Params<float>> p32(n_cases);
     for (int i = 0; i < n_cases; i++) {
@@ -1498,7 +1499,12 @@ int mst_batch_create(int n_cases, const mst_mesh *const *meshes, const mst_confi
                     inc_n[gi * dim + d] = nn[d];
                     inc_dx[gi * dim + d] = dd[d];
                 }
-                if (b->mixed) inc_w32[gi] = (code >= 0 && (code & 1)) ? (float)e : 1.0f - (float)e;
+                if (b->mixed) {
+                    inc_w32[gi] = (code >= 0 && (code & 1)) ? (float)e : 1.0f - (float)e;
+                    // the FP32 3D gradient's weighted normals (k_weighted_normals: the same arithmetic)
+                    const double w = dim == 3 && code >= 0 ? ((code & 1) ? e : 1.0 - e) : 1.0;
+                    for (int d = 0; d < dim; d++) inc_n32[gi * dim + d] = (float)(w * nn[d]);
+                }
             }
         p64[i] = make_params<double>(c);
         p32[i] = make_params<float>(c);
@@ -1541,9 +1547,9 @@ int mst_batch_create(int n_cases, const mst_mesh *const *meshes, const mst_confi
     TRY(dupload(&b->inc_n, inc_n));
     TRY(dupload(&b->inc_dx, inc_dx));
     if (b->mixed) {
-        std::vector<float> n32(inc_n.begin(), inc_n.end()), d32(inc_dx.begin(), inc_dx.end());
+        std::vector<float> d32(inc_dx.begin(), inc_dx.end());
         TRY(dupload(&b->inc_w32, inc_w32));
-        TRY(dupload(&b->inc_n32, n32));
+        TRY(dupload(&b->inc_n32, inc_n32));
         TRY(dupload(&b->inc_dx32, d32));
         TRY(dalloc(&b->ivol32, b->NC));
         hipLaunchKernelGGL(k_recip_vol<0>, dim3((uint32_t)((b->NC + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, b->stream,

@@ -233,6 +233,15 @@ struct IncView {
     int64_t stride;
 };
 
+// FP32 in 3D: the difference-form weight is folded into the incidence normal (the FP32 normal array
+// holds (float)(w n), k_weighted_normals) and the weights are not read.  The FP64 form sums whole face
+// states, which needs the weight alone; 2D FP32 keeps them apart because the 2D fused kernel takes its
+// normals from its faces'.
+template <int DIM, typename T>
+constexpr bool grad_wn() {
+    return DIM == 3 && std::is_same<T, float>::value;
+}
+
 // The memory reads of one cell's limited gradient.  grad_fetch issues every read before the first is
 // used: the slots' codes, weights, normals (unless NRM is false: the caller supplies them) and offsets
 // and the volume (independent of each other), then all the neighbour states (an empty or boundary
@@ -254,7 +263,7 @@ __device__ __forceinline__ void grad_fetch(const T *__restrict__ Q, const IncVie
     for (int s = 0; s < MAXF; s++) {
         const int64_t i = (int64_t)s * a.stride + g;
         in.code[s] = a.code[i];
-        in.eta[s] = a.w[i];
+        if constexpr (!grad_wn<DIM, T>()) in.eta[s] = a.w[i];
 #pragma unroll
         for (int d = 0; d < DIM; d++) {
             if constexpr (NRM) in.n[s][d] = a.n[i * DIM + d];
@@ -294,7 +303,8 @@ __device__ __forceinline__ int grad_eval(const GradIn<DIM, MAXF, T> &in, const T
         if (interior) {
             if constexpr (F32) {
 #pragma unroll
-                for (int k = 0; k < NV; k++) qf[k] = in.eta[s] * (in.qn[s][k] - qc[k]);
+                for (int k = 0; k < NV; k++)
+                    qf[k] = grad_wn<DIM, T>() ? in.qn[s][k] - qc[k] : in.eta[s] * (in.qn[s][k] - qc[k]);
             } else {
 #pragma unroll
                 for (int k = 0; k < NV; k++)
@@ -806,6 +816,20 @@ __global__ __launch_bounds__(BLOCK) void k_pack_faces(int64_t n, const int32_t *
     const int32_t f = face[i];
     const int info = finfo[f];
     out[i] = (uint32_t)f | (uint32_t)(info & 7) << FI_SHIFT | (uint32_t)((info >> 4) & 7) << (FI_SHIFT + 3);
+}
+
+// The FP32 3D gradient's normals (grad_wn): (float)(w n) per incidence, w = 1 - eta on the c0 side and eta
+// on the c1 side of the face, in FP64; boundary and empty slots keep (float) n (unused)
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_weighted_normals(int64_t n, const int32_t *__restrict__ code,
+                                                            const double *__restrict__ eta, const double *__restrict__ nrm,
+                                                            float *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const int32_t cd = code[i];
+    const double w = cd >= 0 ? ((cd & 1) ? eta[i] : 1.0 - eta[i]) : 1.0;
+#pragma unroll
+    for (int d = 0; d < DIM; d++) out[i * DIM + d] = (float)(w * nrm[i * DIM + d]);
 }
 
 // The FP32 gradient's volume input: (float) frcp(V) per cell (GradVol)

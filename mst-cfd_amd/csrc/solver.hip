@@ -756,6 +756,11 @@ int upload_incidence_weights(mst_ctx *x, const std::vector<double> &eta) {
             std::vector<double> h(n * dim);
             std::vector<float> f(n * dim);
             for (auto pr : {std::make_pair(x->inc_n, &x->inc_n32), std::make_pair(x->inc_dx, &x->inc_dx32)}) {
+                if (dim == 3 && pr.second == &x->inc_n32) {  // weighted normals, filled below
+                    rc = dalloc(&x->inc_n32, n * dim);
+                    if (rc) return rc;
+                    continue;
+                }
                 HIPCHK(hipMemcpy(h.data(), pr.first, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
                 for (size_t k = 0; k < h.size(); k++) f[k] = (float)h[k];
                 rc = dupload(pr.second, f);
@@ -768,6 +773,12 @@ int upload_incidence_weights(mst_ctx *x, const std::vector<double> &eta) {
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipMemcpy(x->inc_w32, w32.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+        if (x->dim == 3) {  // the FP32 3D gradient reads the weights folded into its normals
+            hipLaunchKernelGGL(k_weighted_normals<3>, dim3((uint32_t)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, x->stream,
+                               n, (const int32_t *)x->inc_code, (const double *)x->inc_w, (const double *)x->inc_n,
+                               x->inc_n32);
+            HIPCHK(hipGetLastError());
+        }
     }
     x->hg_dirty = true;
     return MST_OK;
@@ -1696,6 +1707,7 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
     const bool mixed = ctx->mixed, second = ctx->cfg.order >= 2;
     const double r = mixed ? 4 : 8;
     const double ell = 4.0 * ctx->maxf * nc;
+    const double w_inc = mixed && ctx->dim == 3 ? 0 : r;  // incidence weight bytes of the gradient
     const double stages = ctx->cfg.rk;
     // faces: c0, c1, info, FP64 normal; cells: the arithmetic state; order 2: face-cell offsets
     // (FP64: face centres, partner centres and cell centres) and limited gradients
@@ -1714,8 +1726,9 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
         case MST_K_FLUX:
             *bytes = flux_in + nf * 8 * NV;
             break;
-        case MST_K_GRAD:  // incidences: code, weight, normal, offset, record out; cells: state, volume
-            *bytes = 4.0 * ctx->slots * nc + ninc * (r + 2 * r * D + r * NV) + nc * (r * NV + r);
+        case MST_K_GRAD:  // incidences: code, weight (not FP32 3D: folded into the normal), normal, offset,
+                          // record out; cells: state, volume
+            *bytes = 4.0 * ctx->slots * nc + ninc * (w_inc + 2 * r * D + r * NV) + nc * (r * NV + r);
             break;
         case MST_K_UPDATE:
             *bytes = upd + nf * 8 * NV;
@@ -1730,7 +1743,7 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
             break;
         case MST_K_FUSED: {  // gradient inputs + per-face inputs + update, each once; no records
             // per incidence: code, weight, offset, and in 3D the normal (2D takes it from its face's)
-            const double inc_geo = ctx->dim == 2 ? r + r * D : r + 2 * r * D;
+            const double inc_geo = ctx->dim == 2 ? r + r * D : w_inc + 2 * r * D;
             const double grad_in = 4.0 * ctx->slots * nc + ninc * inc_geo + nc * (r * NV + r);
             const double flux_f = nf * ((ctx->pfi ? 0 : 4) + 8 * D) + 4.0 * (double)ctx->n_psolve + nbnd * 4;
             *bytes = grad_in + flux_f + upd - ell + 2.0 * ctx->slots * nc - (mixed ? 0 : nc * 8 * NV);
