@@ -151,11 +151,14 @@ def test_batch_autotune_cache_and_state(tmp_path):
 
 
 @pytest.mark.parametrize("strategy", [M.STRATEGY_CELL, M.STRATEGY_PATCH, M.STRATEGY_FUSED])
-def test_batch_3d_cases(strategy):
+@pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_MIXED])
+def test_batch_3d_cases(precision, strategy):
+    """3D cases, RK3.  Mixed: the batch builds the FP32 gradient's weighted normals on the host, a
+    single context on the device; both must give the same bits."""
     bc = (M.BC_INLET, M.BC_OUTLET) + (M.BC_SYMMETRY,) * 4
     meshes = [M.Mesh.gen_tet(n, 4, 4, jitter=0.1, seed=n, bc=bc) for n in (4, 9, 6)]
     qin = [1.0, 0.8, 0.0, 0.0, 1 / 0.4 / 1.4 + 0.32]
-    cfgs = [dict(scheme=s, order=2, limiter=M.LIM_VANLEER, rk=3, cfl=0.4, q_inlet=qin, dim=3)
+    cfgs = [dict(scheme=s, order=2, limiter=M.LIM_VANLEER, rk=3, cfl=0.4, q_inlet=qin, dim=3, precision=precision)
             for s in (M.AUSM, M.ROE, M.AUSM)]
     Q0 = [m.init_state(M.INIT_PULSE, [1.4, 0.1, 0.15, 0.5, 0.5, 0.5, 0.8]) for m in meshes]
     b = make_batch(meshes, cfgs, Q0)

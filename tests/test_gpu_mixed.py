@@ -100,6 +100,47 @@ def test_mixed_pulse3d_density_error():
     assert linf[0] <= 2e-5
 
 
+@pytest.mark.parametrize("dim,rk", [(2, 2), (2, 3), (3, 1), (3, 2), (3, 3)])
+def test_mixed_step_is_the_ssp_combination_of_its_residuals(dim, rk):
+    """One mixed step equals the SSP-RK combination of the mixed residuals of its stage states,
+    Q_{s+1} = A_s Q^n + B_s (Q_s - dt/V R(Q_s)), to FP64 rounding: the intermediate stages store the
+    next stage's A Q^n + B Q_s in place of their state, which must not change what a stage computes.
+    A wrong coefficient or a Q^n taken from the wrong buffer moves the state by O(dt R)."""
+    if dim == 2:
+        m = M.Mesh.gen_tri(40, 40, -10, 10, -10, 10, 0.1, 7, (M.BC_PERIODIC,) * 4)
+        kw = dict(scheme=M.ROE, order=2, limiter=M.LIM_VANLEER, rk=rk)
+    else:
+        bc = (M.BC_INLET, M.BC_OUTLET) + (M.BC_SYMMETRY,) * 4
+        m = M.Mesh.gen_tet(8, 8, 8, (0, 0, 0), (1, 1, 1), 0.15, 12345, bc)
+        qin = list(cons([1.0], [[0.8, 0, 0]], [1 / 1.4])[0])
+        kw = dict(scheme=M.AUSM, order=2, limiter=M.LIM_MINMOD, rk=rk, q_inlet=qin, dim=3)
+    s = M.Solver(m, M.make_config(precision=M.PREC_MIXED, **kw))
+    geo = s.prep.geometry()
+    Q0 = vortex_exact(geo["ccen"]) if dim == 2 else pulse_state(geo["ccen"])
+    dtv = None
+    A, B = {1: ([0.0], [1.0]), 2: ([0.0, 0.5], [1.0, 0.5]), 3: ([0.0, 0.75, 1 / 3], [1.0, 0.25, 2 / 3])}[rk]
+    Q = Q0
+    for st in range(rk):
+        s.set_state(Q)
+        if dtv is None:
+            dt = s.compute_dt(0.5)
+            dtv = dt / geo["vol"][:, None]
+        R = s.residual()
+        Q = A[st] * Q0 + B[st] * (Q - dtv * R)
+    s.set_state(Q0)
+    s.set_dt(dt)
+    s.step(1)
+    Qg = s.get_state()
+    scale = np.abs(Q0).max(0)
+    scale[1:-1] = scale[1:-1].max()  # the momentum components share the momentum scale
+    err = (np.abs(Qg - Q).max(0) / scale).max()
+    moved = (np.abs(Q - Q0).max(0) / scale).max()
+    print(f"mixed dim={dim} rk={rk}: step vs SSP combination {err:.2e} (the step moves the state {moved:.2e})",
+          flush=True)
+    assert moved > 1e-4
+    assert err <= 1e-14
+
+
 def test_mixed_freestream_and_switching():
     m = M.Mesh.gen_tri(40, 40, -1, 1, -1, 1, 0.2, 4, (M.BC_PERIODIC,) * 4)
     s = M.Solver(m, M.make_config(order=2, limiter=M.LIM_MINMOD, precision=M.PREC_MIXED))
