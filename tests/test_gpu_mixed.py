@@ -52,6 +52,26 @@ def test_mixed_residual_close_to_oracle(scheme, order, lim):
     assert 1e-9 < err <= 2e-6  # FP32 flux arithmetic: well above FP64 rounding, far below 1e-5
 
 
+@pytest.mark.parametrize("scheme,lim", [(M.ROE, M.LIM_MINMOD), (M.AUSM, M.LIM_VANLEER)])
+def test_mixed_3d_residual_close_to_oracle(scheme, lim):
+    """Tetrahedra with every boundary kind, order 2: the FP32 3D gradient reads the difference-form
+    weights folded into its normals, (float)(w n), against the oracle's FP64 eta-weighted face sums."""
+    bc = (M.BC_INLET, M.BC_OUTLET, M.BC_SYMMETRY, M.BC_WALL, M.BC_SYMMETRY, M.BC_SYMMETRY)
+    m = M.Mesh.gen_tet(10, 8, 6, (0, 0, 0), (1, 0.8, 0.7), 0.15, 21, bc)
+    om = O.OracleMesh(m.raw())
+    geo = om.geometry()
+    Q = smooth_state(geo["ccen"], seed=5, mach=0.7)
+    qin = list(cons([1.2], [[0.6, 0.1, 0.1]], [1.1])[0])
+    kw = dict(scheme=scheme, order=2, limiter=lim, q_inlet=qin, dim=3)
+    s = M.Solver(m, M.make_config(precision=M.PREC_MIXED, **kw))
+    s.set_state(Q)
+    Rg = s.residual()
+    Rr, Sr = om.residual(O.make_config(**kw), Q)
+    err = residual_parity_error(Rg, Rr, Sr, flux_scale(Q) * normal_sums(m.raw(), geo["S"])[:, None])
+    print(f"mixed 3D residual scheme={scheme} lim={lim}: err/S = {err:.2e}", flush=True)
+    assert 1e-9 < err <= 2e-6
+
+
 def test_mixed_conserves_exactly_and_tracks_fp64():
     """Each FP32 face flux enters both cells with opposite signs in FP64, so periodic totals stay
     conserved to FP64 rounding; the state follows the FP64 run to FP32 accuracy."""
