@@ -158,3 +158,23 @@ def quad_grid(n, bc=3, h=1.0, jitter=0.0, seed=0):
     return dict(dim=2, xyz=xy, fnodes=np.array(fn, np.int32), c0=np.array(c0, np.int32), c1=np.array(c1, np.int32),
                 ftype=np.array([2] * nint + [bc] * (nf - nint), np.int32), partner=np.full(nf, -1, np.int32),
                 n_cells=n * n, n_int=nint)
+
+
+def periodic_in_x(raw):
+    """The raw arrays of a mesh whose boundary faces on its lowest and highest x planes match in
+    (y, z) (an unjittered gen_tet box), with those faces paired as periodic partners (Fluent type
+    12): the 3D periodic meshes the generators do not make."""
+    out = {k: np.array(v) for k, v in raw.items()}
+    cen = out["xyz"][out["fnodes"]].mean(1)
+    bnd = out["c1"] < 0
+    x0, x1 = out["xyz"][:, 0].min(), out["xyz"][:, 0].max()
+    lo = np.where(bnd & (np.abs(cen[:, 0] - x0) < 1e-12))[0]
+    hi = np.where(bnd & (np.abs(cen[:, 0] - x1) < 1e-12))[0]
+    key = lambda i: tuple(np.round(cen[i, 1:], 9))
+    pair = {key(j): j for j in hi}
+    assert len(lo) == len(hi) == len(pair) and len(lo) > 0
+    for i in lo:
+        j = pair[key(i)]
+        out["partner"][i], out["partner"][j] = j, i
+        out["ftype"][i] = out["ftype"][j] = 12
+    return out

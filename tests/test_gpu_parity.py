@@ -7,8 +7,8 @@ import numpy as np
 import pytest
 
 import mstcfd as M
-from helpers import (quad_grid, cons, flip_faces, flux_scale, jumpy_state, normal_sums, rel_state_error, residual_parity_error,
-                     smooth_state, sod_state)
+from helpers import (quad_grid, cons, flip_faces, flux_scale, jumpy_state, normal_sums, periodic_in_x, rel_state_error,
+                     residual_parity_error, smooth_state, sod_state)
 from oracle import pyoracle as O
 
 pytestmark = pytest.mark.gpu
@@ -37,6 +37,10 @@ def mesh(name):
         elif name == "tet":
             m = M.Mesh.gen_tet(6, 5, 4, (0, 0, 0), (1, 0.8, 0.7), 0.15, 11,
                                (M.BC_INLET, M.BC_OUTLET, M.BC_SYMMETRY, M.BC_WALL, M.BC_SYMMETRY, M.BC_SYMMETRY))
+        elif name == "tet_periodic":  # periodic in x, walls and symmetry planes around
+            bc = (M.BC_WALL, M.BC_WALL, M.BC_SYMMETRY, M.BC_SYMMETRY, M.BC_WALL, M.BC_SYMMETRY)
+            box = M.Mesh.gen_tet(6, 5, 4, (0, 0, 0), (1, 0.8, 0.7), 0.0, 11, bc)
+            m = M.Mesh.from_raw(periodic_in_x(box.raw()))
         else:
             raise KeyError(name)
         _MESHES[name] = (m, O.OracleMesh(m.raw()), M.Prep(m).geometry())
@@ -95,6 +99,7 @@ CASES = [
     ("strip_wall", ROE, 2, M.LIM_MINMOD), ("allbc", ROE, 2, M.LIM_NONE), ("allbc", AUSM, 2, M.LIM_MINMOD),
     ("allbc", ROE, 2, M.LIM_VANLEER), ("periodic", ROE, 2, M.LIM_VANLEER), ("periodic", AUSM, 2, M.LIM_NONE),
     ("tet", AUSM, 1, 0), ("tet", ROE, 1, 0), ("tet", AUSM, 2, M.LIM_MINMOD), ("tet", ROE, 2, M.LIM_VANLEER),
+    ("tet_periodic", ROE, 1, 0), ("tet_periodic", AUSM, 2, M.LIM_MINMOD),
 ]
 
 
@@ -116,7 +121,7 @@ def test_residual_parity(mname, scheme, order, lim, kind):
 
 
 @pytest.mark.parametrize("mname,lim", [("allbc", M.LIM_MINMOD), ("periodic", M.LIM_VANLEER), ("tet", M.LIM_MINMOD),
-                                       ("strip_wall", M.LIM_NONE)])
+                                       ("strip_wall", M.LIM_NONE), ("tet_periodic", M.LIM_VANLEER)])
 def test_gradient_and_limiter_parity(mname, lim):
     m, om, geo = mesh(mname)
     dim = m.dim
@@ -226,7 +231,8 @@ def test_monitor_maxrel_matches_reference_rule():
 
 @pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_MIXED])
 @pytest.mark.parametrize("mname,scheme,lim", [("periodic", ROE, M.LIM_VANLEER), ("allbc", AUSM, M.LIM_MINMOD),
-                                              ("tet", ROE, M.LIM_MINMOD), ("quad", AUSM, M.LIM_VANLEER)])
+                                              ("tet", ROE, M.LIM_MINMOD), ("quad", AUSM, M.LIM_VANLEER),
+                                              ("tet_periodic", AUSM, M.LIM_MINMOD)])
 def test_cell_strategy_matches_face_strategy(mname, scheme, lim, precision):
     """The fused cell kernel solves each interior face twice, the patch kernel each face of a patch
     once (a border face in both patches), always from the same memory inputs with the same
@@ -253,7 +259,8 @@ def test_cell_strategy_matches_face_strategy(mname, scheme, lim, precision):
 
 
 @pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_MIXED])
-@pytest.mark.parametrize("mname,scheme", [("periodic", ROE), ("allbc", AUSM), ("tet", ROE), ("quad", AUSM)])
+@pytest.mark.parametrize("mname,scheme", [("periodic", ROE), ("allbc", AUSM), ("tet", ROE), ("quad", AUSM),
+                                          ("tet_periodic", ROE)])
 def test_first_order_strategies_match(mname, scheme, precision):
     """Order 1: the patch kernel stages the cell states of its faces in LDS; every strategy still
     reproduces the face split bit for bit."""
