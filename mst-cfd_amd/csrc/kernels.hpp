@@ -707,8 +707,10 @@ __device__ __forceinline__ void cell_fluxes(const FluxArgs<T> &fa, const int32_t
     // 2D, and FP32 in 3D: every face of the cell is loaded before the first solve.  FP64 3D keeps the
     // per-slot loop (the preloaded form needs 178-200 VGPRs there).  FP32 3D cells in the per-slot
     // loop got outlet fluxes that differ from the face strategy's (measured at order 1: up to 10 % of
-    // the flux on every outlet face, against F(Q) n, which every other strategy reproduces); the
-    // preloaded form, the same arithmetic, reproduces the face strategy bit for bit.
+    // the flux on every outlet face, against F(Q) n, which every other strategy reproduces): its
+    // compiled code (ROCm 7.2, gfx950) leaves the inlet density in the right state's density register
+    // on outlet and periodic lanes.  The preloaded form, the same arithmetic, reproduces the face
+    // strategy bit for bit.
     if constexpr ((DIM == 2 || F32) && MAXF <= 4) {
         int32_t e[MAXF];
         int kind[MAXF], mask[MAXF];
