@@ -408,6 +408,8 @@ int mst_ctx_warm_start(mst_ctx *ctx, mst_mlp *m, const double *cond, int n_cond)
 #define MST_K_LUSGS 7 /* LU-SGS colour sweeps (mst_lusgs_step); algorithmic bytes: per launch, averaged */
 #define MST_K_VISC 8 /* viscous gradient + face kernels (mu or kappa > 0); algorithmic bytes: per launch, averaged */
 #define MST_K_COUNT 9
+/* on: 0 off, 1 events on every launch, n > 1 on one launch in n of each kernel class (counted from
+   this call; a timed dispatch costs a few microseconds of throughput) */
 int mst_timing_enable(mst_ctx *ctx, int on);
 /* per kernel class: launches timed and total milliseconds since the last reset */
 int mst_timing_get(mst_ctx *ctx, int kernel, int64_t *count, double *total_ms);

@@ -348,7 +348,7 @@ __global__ __launch_bounds__(NT) void kb_cfl(const BArgs<T> a, const double *__r
         for (int64_t c = I.begin + (int)threadIdx.x; c < I.end; c += NT) {
             double q[NV];
             ldq<NV>(Q + c * NV, q);
-            v[0] = fmin(v[0], local_dt<DIM>(q, a.u.sh, c, a.u.vol[c], gam, cfl));
+            v[0] = fmin(v[0], local_dt<DIM>(q, a.u.sh + c * DIM, a.u.vol[c], gam, cfl));
         }
         block_partial<1, true, NT>(v, a.part_dt + it, 1);
     }

@@ -505,17 +505,17 @@ __device__ __forceinline__ void block_partial(double (&v)[W], double *__restrict
 }
 
 template <int DIM>
-__device__ __forceinline__ double local_dt(const double *q, const double *__restrict__ sh, int64_t c, double V,
-                                           double gamma, double cfl) {
+__device__ __forceinline__ double local_dt(const double *q, const double *shc, double V, double gamma, double cfl) {
     constexpr int NV = DIM + 2;
+    const double ir = 1.0 / q[0];  // one division: the velocities and the pressure multiply by 1/rho
     double ke = 0;
 #pragma unroll
     for (int k = 1; k <= DIM; k++) ke += q[k] * q[k];
-    const double p = (q[NV - 1] - 0.5 * ke / q[0]) * (gamma - 1);
-    const double snd = sqrt(gamma * p / q[0]);
+    const double p = (q[NV - 1] - 0.5 * ke * ir) * (gamma - 1);
+    const double snd = sqrt(gamma * p * ir);
     double lam = 0;
 #pragma unroll
-    for (int d = 0; d < DIM; d++) lam += (fabs(q[1 + d] / q[0]) + snd) * sh[c * DIM + d];
+    for (int d = 0; d < DIM; d++) lam += (fabs(q[1 + d]) * ir + snd) * shc[d];
     const double v = cfl * V / lam;
     return v == v ? v : -1.0;  // a NaN state reports a negative time step
 }
@@ -544,12 +544,16 @@ __device__ __forceinline__ void accumulate(double (&r)[NV], double (&cmp)[NV], c
 template <int NV>
 struct StageIn {
     double V, qs[NV], qn[NV];
+    double sh[NV - 2];  // the cell's CFL half-sums, on a launch that reduces the time step
 };
 
 template <int NV, bool F32>
 __device__ __forceinline__ void stage_load(const UpdArgs &a, int64_t c, StageIn<NV> &in) {
     in.V = a.vol[c];
     ldq<NV>(a.Qs + c * NV, in.qs);
+    if (a.dtred)
+#pragma unroll
+        for (int d = 0; d < NV - 2; d++) in.sh[d] = a.sh[c * (NV - 2) + d];
     // Q^n: the stage combination (FP64 form), or the successor's combination after the first stage
     const bool pre = F32 && a.pre;
     if (!a.stage0 && (!pre || a.wb != 0)) ldq<NV>(a.Qn + c * NV, in.qn);
@@ -599,7 +603,7 @@ __device__ __forceinline__ void stage_cell(const UpdArgs &a, int64_t c, const St
             red[k] += x * x;
         }
     }
-    if (DT) red[NV] = fmin(red[NV], local_dt<DIM>(out, a.sh, c, V, a.gamma, a.cfl));
+    if (DT) red[NV] = fmin(red[NV], local_dt<DIM>(out, in.sh, V, a.gamma, a.cfl));
 }
 
 template <int NV, bool F32>
@@ -1328,7 +1332,7 @@ __global__ __launch_bounds__(BLOCK) void k_cfl(int64_t nc, const double *__restr
     if (c < nc) {
         double q[NV];
         ldq<NV>(Q + c * NV, q);
-        v[0] = local_dt<DIM>(q, sh, c, vol[c], gamma, cfl);
+        v[0] = local_dt<DIM>(q, sh + c * DIM, vol[c], gamma, cfl);
     }
     block_partial<1, true>(v, part_dt + blk, 1);
 }

@@ -536,7 +536,8 @@ class Solver:
         return g, phi
 
     def timing(self, on=True):
-        _check(lib().mst_timing_enable(self.h, 1 if on else 0), "mst_timing_enable")
+        """True / 1: HIP events on every launch; an int n > 1: on one launch in n of each kernel class."""
+        _check(lib().mst_timing_enable(self.h, int(on)), "mst_timing_enable")
 
     def timing_get(self, kernel):
         n, ms = C.c_int64(), C.c_double()

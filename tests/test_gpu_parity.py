@@ -372,6 +372,17 @@ def test_kernel_timing_counts_and_leaves_results_unchanged(strategy):
     assert total < wall_ms
     s.timing_reset()
     assert s.timing_get(M.K_GRAD) == (0, 0.0)
+    # sampled: one launch in 5 of each class (counted from the enable), the same results
+    s.set_state(Q)
+    s.timing(5)
+    s.step(4)
+    s.sync()
+    s.timing(False)
+    assert np.array_equal(s.get_state(), plain.get_state())
+    for k, n_want in want.items():
+        n, ms = s.timing_get(k)
+        assert n == (n_want + 4) // 5, (M.KERNEL_NAMES[k], n)
+        assert (ms > 0) == (n > 0)
 
 
 @pytest.mark.parametrize("mname", ["allbc", "tet"])
