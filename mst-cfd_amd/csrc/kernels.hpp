@@ -1425,6 +1425,28 @@ static __global__ __launch_bounds__(FIN) void k_finish(const FinArgs a) {
     }
 }
 
+// One minimum (the CFL time step) over n partials in one block: each lane's eight loads are in flight
+// together.  A minimum does not depend on the combining order, so this equals k_finish's result.
+template <int U>
+__global__ __launch_bounds__(FIN) void k_finish_min(const double *__restrict__ src, int64_t n, double *__restrict__ dst) {
+    __shared__ double s[FIN / 64];
+    double v = INFINITY;
+    for (int64_t i = threadIdx.x; i < n; i += 8 * FIN) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) x[u] = i + u * FIN < n ? src[i + u * FIN] : INFINITY;
+#pragma unroll
+        for (int u = 0; u < 8; u++) v = fmin(v, x[u]);
+    }
+    for (int m = 32; m > 0; m >>= 1) v = fmin(v, __shfl_xor(v, m, 64));
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < FIN / 64; w++) v = fmin(v, s[w]);
+        dst[0] = v;
+    }
+}
+
 // ------------------------------------------------------------------------------------ warm start
 // features (cell centre, conditions) for the network and the decoding of its outputs
 //   y = (log(rho/rho_inf), u_1/|u_inf|, .., u_dim/|u_inf|, log(p/p_inf)),  rho_inf = 1, p_inf = 1/gamma,

@@ -549,11 +549,15 @@ int run_stage(mst_ctx *x, int mode, int is, int in, double ca, double cb, int st
 int launch_finish(mst_ctx *x, FinArgs f) {
     f.chunk = f.nblk;
     f.mid = nullptr;
-    // sums keep one fixed two-level order above 2 FIN_CHUNK partials (bit-reproducible norms); a
-    // minimum does not depend on the order, so one block takes up to 8 FIN_CHUNK partials in one launch
-    bool min_only = true;
-    for (int k = 0; k < f.n; k++) min_only = min_only && f.kind[k] == 1;
-    if (f.nblk > (min_only ? 8 : 2) * FIN_CHUNK) {
+    // sums keep one fixed two-level order above 2 FIN_CHUNK partials (bit-reproducible norms); a lone
+    // minimum (the CFL time step) does not depend on the order: one block, one launch, up to
+    // 16 FIN_CHUNK partials
+    if (f.n == 1 && f.kind[0] == 1 && f.post[0] == 0 && f.nblk <= 16 * FIN_CHUNK) {
+        hipLaunchKernelGGL(k_finish_min<0>, dim3(1), dim3(FIN), 0, x->stream, f.src[0], f.nblk, f.dst[0]);
+        HIPCHK(hipGetLastError());
+        return MST_OK;
+    }
+    if (f.nblk > 2 * FIN_CHUNK) {
         const int64_t g = (f.nblk + FIN_CHUNK - 1) / FIN_CHUNK;
         FinArgs a = f;
         a.chunk = FIN_CHUNK;
