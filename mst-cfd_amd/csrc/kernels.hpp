@@ -515,7 +515,7 @@ __device__ __forceinline__ double local_dt(const double *q, const double *shc, d
     const double snd = sqrt(gamma * p * ir);
     double lam = 0;
 #pragma unroll
-    for (int d = 0; d < DIM; d++) lam += (fabs(q[1 + d]) * ir + snd) * shc[d];
+    for (int d = 0; d < DIM; d++) lam += (fabs(q[1 + d] * ir) + snd) * shc[d];
     const double v = cfl * V / lam;
     return v == v ? v : -1.0;  // a NaN state reports a negative time step
 }
