@@ -383,6 +383,8 @@ def test_kernel_timing_counts_and_leaves_results_unchanged(strategy):
         n, ms = s.timing_get(k)
         assert n == (n_want + 4) // 5, (M.KERNEL_NAMES[k], n)
         assert (ms > 0) == (n > 0)
+    with pytest.raises(M.MstError, match="timing"):
+        s.timing(-1)
 
 
 @pytest.mark.parametrize("mname", ["allbc", "tet"])
