@@ -479,9 +479,6 @@ int bucket_of(int64_t nc) {
 struct mst_batch {
     int n = 0, dim = 2, nv = 4, maxf = 3, slots = 3, order = 1, limiter = 0, rk = 1;
     bool mixed = false;
-    // the stage being enqueued: pre-combined stage states (mixed form) and the successor's coefficients
-    int upd_pre = 0;
-    double upd_wa = 0, upd_wb = 0;
     std::vector<mst_config> cfgs;
     std::vector<mst::Prepared> preps;
     std::vector<int64_t> cstart, fstart;
@@ -779,9 +776,6 @@ BArgs<T> base_args(const mst_batch *b, int isrc) {
     u.sh = b->cshalf;
     u.maxf = b->maxf;
     u.bad = b->bad;
-    u.pre = b->upd_pre;
-    u.wa = b->upd_wa;
-    u.wb = b->upd_wb;
     a.scheme = b->scheme;
     a.dts = b->dts;
     a.gam = b->gam;
@@ -843,7 +837,7 @@ int run_flux(mst_batch *b, int isrc) {
 }
 
 template <int DIM, typename T, int NT>
-int run_update(mst_batch *b, int is, int in, double ca, double cb, int stage0, int iout, bool l2, bool dt,
+int run_update(mst_batch *b, int is, int in, const Stage &sg, int iout, bool l2, bool dt,
                bool check) {
     constexpr int NV = DIM + 2;
     BArgs<T> a = base_args<T>(b, is);
@@ -854,9 +848,7 @@ int run_update(mst_batch *b, int is, int in, double ca, double cb, int stage0, i
     a.u.Qn = b->Q[in];
     a.u.Qout = b->Q[iout];
     a.u.Qout32 = b->mixed ? b->Q32[iout] : nullptr;
-    a.u.ca = ca;
-    a.u.cb = cb;
-    a.u.stage0 = stage0;
+    set_stage(a.u, sg);
     a.u.l2 = l2;
     a.u.dtred = dt;
     a.u.check = check;
@@ -868,7 +860,7 @@ int run_update(mst_batch *b, int is, int in, double ca, double cb, int stage0, i
 // stage s of a step pulls from counter UPD (even s) or FLUX (odd s): without a gradient kernel between
 // them, consecutive cell kernels must not share a counter (each zeroes only the others at its start)
 template <int DIM, typename T, int NT>
-int run_cell(mst_batch *b, int stage, int is, int in, double ca, double cb, int stage0, int iout, bool l2, bool dt,
+int run_cell(mst_batch *b, int stage, int is, int in, const Stage &sg, int iout, bool l2, bool dt,
              bool check) {
     BArgs<T> a = base_args<T>(b, is);
     a.items = b->citems;
@@ -878,9 +870,7 @@ int run_cell(mst_batch *b, int stage, int is, int in, double ca, double cb, int 
     a.u.Qn = b->Q[in];
     a.u.Qout = b->Q[iout];
     a.u.Qout32 = b->mixed ? b->Q32[iout] : nullptr;
-    a.u.ca = ca;
-    a.u.cb = cb;
-    a.u.stage0 = stage0;
+    set_stage(a.u, sg);
     a.u.l2 = l2;
     a.u.dtred = dt;
     a.u.check = check;
@@ -894,7 +884,7 @@ int run_cell(mst_batch *b, int stage, int is, int in, double ca, double cb, int 
 }
 
 template <int DIM, typename T>
-int run_patch(mst_batch *b, int stage, int is, int in, double ca, double cb, int stage0, int iout, bool l2, bool dt,
+int run_patch(mst_batch *b, int stage, int is, int in, const Stage &sg, int iout, bool l2, bool dt,
               bool check) {
     BArgs<T> a = base_args<T>(b, is);
     a.items = b->citems;
@@ -904,9 +894,7 @@ int run_patch(mst_batch *b, int stage, int is, int in, double ca, double cb, int
     a.u.Qn = b->Q[in];
     a.u.Qout = b->Q[iout];
     a.u.Qout32 = b->mixed ? b->Q32[iout] : nullptr;
-    a.u.ca = ca;
-    a.u.cb = cb;
-    a.u.stage0 = stage0;
+    set_stage(a.u, sg);
     a.u.l2 = l2;
     a.u.dtred = dt;
     a.u.check = check;
@@ -925,7 +913,7 @@ int run_patch(mst_batch *b, int stage, int is, int in, double ca, double cb, int
 }
 
 template <int DIM, typename T>
-int run_fused(mst_batch *b, int stage, int is, int in, double ca, double cb, int stage0, int iout, bool l2, bool dt,
+int run_fused(mst_batch *b, int stage, int is, int in, const Stage &sg, int iout, bool l2, bool dt,
               bool check) {
     BArgs<T> a = base_args<T>(b, is);
     a.items = b->citems;
@@ -935,9 +923,7 @@ int run_fused(mst_batch *b, int stage, int is, int in, double ca, double cb, int
     a.u.Qn = b->Q[in];
     a.u.Qout = b->Q[iout];
     a.u.Qout32 = b->mixed ? b->Q32[iout] : nullptr;
-    a.u.ca = ca;
-    a.u.cb = cb;
-    a.u.stage0 = stage0;
+    set_stage(a.u, sg);
     a.u.l2 = l2;
     a.u.dtred = dt;
     a.u.check = check;
@@ -1007,19 +993,19 @@ int enqueue_step_t(mst_batch *b, int iold, int inew) {
     // of its FP64 state (the FP32 kernels read only the shadow), so no later stage reads Q^n again
     // (SSP-RK2: none; RK3: the middle stage)
     const bool pre = b->mixed;
-    struct Reset {
-        mst_batch *b;
-        ~Reset() { b->upd_pre = 0, b->upd_wa = b->upd_wb = 0; }
-    } reset{b};
     int src = iold;
     for (int s = 0; s < ns; s++) {
         const int dst = (s == ns - 1) ? inew : spare[s & 1];
-        b->upd_pre = pre;
-        b->upd_wa = pre && s < ns - 1 ? A[s + 1] : 0;
-        b->upd_wb = pre && s < ns - 1 ? B[s + 1] : 0;
+        Stage sg;
+        sg.ca = A[s];
+        sg.cb = B[s];
+        sg.stage0 = s == 0;
+        sg.pre = pre;
+        sg.wa = pre && s < ns - 1 ? A[s + 1] : 0;
+        sg.wb = pre && s < ns - 1 ? B[s + 1] : 0;
         int rc;
         if (b->strategy == MST_STRATEGY_FUSED && b->order >= 2) {
-            rc = run_fused<DIM, T>(b, s, src, iold, A[s], B[s], s == 0 ? 1 : 0, dst, s == 0, s == ns - 1, s == ns - 1);
+            rc = run_fused<DIM, T>(b, s, src, iold, sg, dst, s == 0, s == ns - 1, s == ns - 1);
             if (rc) return rc;
             src = dst;
             continue;
@@ -1029,15 +1015,15 @@ int enqueue_step_t(mst_batch *b, int iold, int inew) {
             if (rc) return rc;
         }
         if (b->strategy == MST_STRATEGY_PATCH || b->strategy == MST_STRATEGY_FUSED) {  // FUSED at order 1: PATCH
-            rc = run_patch<DIM, T>(b, s, src, iold, A[s], B[s], s == 0 ? 1 : 0, dst, s == 0, s == ns - 1, s == ns - 1);
+            rc = run_patch<DIM, T>(b, s, src, iold, sg, dst, s == 0, s == ns - 1, s == ns - 1);
             if (rc) return rc;
         } else if (b->strategy == MST_STRATEGY_CELL) {
-            rc = run_cell<DIM, T, NT>(b, s, src, iold, A[s], B[s], s == 0 ? 1 : 0, dst, s == 0, s == ns - 1, s == ns - 1);
+            rc = run_cell<DIM, T, NT>(b, s, src, iold, sg, dst, s == 0, s == ns - 1, s == ns - 1);
             if (rc) return rc;
         } else {
             rc = run_flux<DIM, T, NT>(b, src);
             if (rc) return rc;
-            rc = run_update<DIM, T, NT>(b, src, iold, A[s], B[s], s == 0 ? 1 : 0, dst, s == 0, s == ns - 1, s == ns - 1);
+            rc = run_update<DIM, T, NT>(b, src, iold, sg, dst, s == 0, s == ns - 1, s == ns - 1);
             if (rc) return rc;
         }
         src = dst;

@@ -460,6 +460,25 @@ struct UpdArgs {
     int *__restrict__ bad;          // set to 1 when an updated state is not physical
 };
 
+// The coefficients of one stage update (host side; copied into UpdArgs): Q_out = ca Q^n + cb (Q_s -
+// dt/V R), or Q_s - dt/V R on the first stage (stage0).  Mixed form: pre = Q_s already holds ca Q^n +
+// cb Q_s, and (wa, wb), wb != 0 on a stage with a successor, are the successor's coefficients.
+struct Stage {
+    double ca = 0, cb = 1;
+    int stage0 = 1;
+    int pre = 0;
+    double wa = 0, wb = 0;
+};
+
+__host__ inline void set_stage(UpdArgs &a, const Stage &sg) {
+    a.ca = sg.ca;
+    a.cb = sg.cb;
+    a.stage0 = sg.stage0;
+    a.pre = sg.pre;
+    a.wa = sg.wa;
+    a.wb = sg.wb;
+}
+
 // mode bits: UPD_STAGE writes the stage update (else only R), UPD_L2 reduces |R/V|^2,
 // UPD_DT reduces the CFL time step of the updated state (for the next step); UPD_CHECK (a runtime
 // flag, not a kernel variant) checks the output for a non-physical state.  Intermediate RK stages

@@ -59,9 +59,6 @@ struct mst_ctx {
     Params<double> p64;
     Params<float> p32;
     bool mixed = false;
-    // the stage being launched: pre-combined stage states (mixed form) and the successor's coefficients
-    int upd_pre = 0;
-    double upd_wa = 0, upd_wb = 0;
     int strategy = MST_STRATEGY_AUTO;
     int resolved = -1;  // strategy in use (AUTO: measured at the first stage after a configuration)
     hipStream_t stream = nullptr;
@@ -307,7 +304,7 @@ int run_face_flux(mst_ctx *x, int isrc) {
     return run_viscous(x, isrc);
 }
 
-UpdArgs upd_args(const mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
+UpdArgs upd_args(const mst_ctx *x, int mode, int is, int in, const Stage &sg, int iout) {
     UpdArgs a;
     a.nc = x->nc;
     a.ell = x->ell;
@@ -317,12 +314,7 @@ UpdArgs upd_args(const mst_ctx *x, int mode, int is, int in, double ca, double c
     a.vol = x->cvol;
     a.sh = x->cshalf;
     a.dt = x->scal;
-    a.ca = ca;
-    a.cb = cb;
-    a.stage0 = stage0;
-    a.pre = x->upd_pre;
-    a.wa = x->upd_wa;
-    a.wb = x->upd_wb;
+    set_stage(a, sg);
     a.maxf = x->maxf;
     a.l2 = (mode & UPD_L2) != 0;
     a.dtred = (mode & UPD_DT) != 0;
@@ -339,8 +331,8 @@ UpdArgs upd_args(const mst_ctx *x, int mode, int is, int in, double ca, double c
 }
 
 template <int NV, bool GT>
-int run_update_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
-    const UpdArgs a = upd_args(x, mode, is, in, ca, cb, stage0, iout);
+int run_update_t(mst_ctx *x, int mode, int is, int in, const Stage &sg, int iout) {
+    const UpdArgs a = upd_args(x, mode, is, in, sg, iout);
 #define MST_UPD(MF)                                                                                             \
     switch (mode & ~UPD_CHECK) {                                                                                             \
         case UPD_RESIDUAL: return launch(x, MST_K_UPDATE, k_update<NV, MF, UPD_RESIDUAL, GT>, x->nc, a);        \
@@ -357,23 +349,23 @@ int run_update_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int
 #undef MST_UPD
 }
 
-int run_update(mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
+int run_update(mst_ctx *x, int mode, int is, int in, const Stage &sg, int iout) {
     int rc;
     if (x->mixed)
-        rc = x->nv == 4 ? run_update_t<4, true>(x, mode, is, in, ca, cb, stage0, iout)
-                        : run_update_t<5, true>(x, mode, is, in, ca, cb, stage0, iout);
+        rc = x->nv == 4 ? run_update_t<4, true>(x, mode, is, in, sg, iout)
+                        : run_update_t<5, true>(x, mode, is, in, sg, iout);
     else
-        rc = x->nv == 4 ? run_update_t<4, false>(x, mode, is, in, ca, cb, stage0, iout)
-                        : run_update_t<5, false>(x, mode, is, in, ca, cb, stage0, iout);
+        rc = x->nv == 4 ? run_update_t<4, false>(x, mode, is, in, sg, iout)
+                        : run_update_t<5, false>(x, mode, is, in, sg, iout);
     if (rc == MST_OK && iout >= 0) x->shadow_ok[iout] = x->mixed;
     return rc;
 }
 
 // cell strategy: gradient + limiter (order 2), then the fused flux + gather + stage update
 template <int DIM, typename T>
-int run_cell_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
+int run_cell_t(mst_ctx *x, int mode, int is, int in, const Stage &sg, int iout) {
     const FluxArgs<T> fa = flux_args<T>(x, is);
-    const UpdArgs a = upd_args(x, mode, is, in, ca, cb, stage0, iout);
+    const UpdArgs a = upd_args(x, mode, is, in, sg, iout);
     const bool second = x->cfg.order >= 2;
     const bool roe = x->cfg.scheme == MST_SCHEME_ROE;
     // slot count of the padded slot-major table: 3 (triangles), 4 (quadrilaterals, tetrahedra), 8
@@ -391,10 +383,10 @@ int run_cell_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int s
 
 // patch strategy: gradient + limiter (order 2), then the patch kernel (faces into LDS, sum, update)
 template <int DIM, typename T>
-int run_patch_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
+int run_patch_t(mst_ctx *x, int mode, int is, int in, const Stage &sg, int iout) {
     if (!x->patch_ok) return mst::fail(MST_E_UNSUPPORTED, "no patch tables for this mesh");
     const FluxArgs<T> fa = flux_args<T>(x, is);
-    const UpdArgs a = upd_args(x, mode, is, in, ca, cb, stage0, iout);
+    const UpdArgs a = upd_args(x, mode, is, in, sg, iout);
     PatchArgs pa;
     pa.start = x->pstart;
     pa.face = x->pface;
@@ -432,7 +424,7 @@ int run_patch_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int 
 template <int DIM>
 int run_patch_residual_t(mst_ctx *x, int mode, int is) {
     const FluxArgs<double> fa = flux_args<double>(x, is);
-    const UpdArgs a = upd_args(x, mode, is, -1, 0, 0, 1, -1);
+    const UpdArgs a = upd_args(x, mode, is, -1, Stage{0, 0}, -1);
     PatchArgs pa;
     pa.start = x->pstart;
     pa.face = x->pface;
@@ -456,14 +448,14 @@ int ensure_halo_geo(mst_ctx *x);
 
 // fused strategy (order 2): gradients, face states, fluxes, sums and update of a patch in one kernel
 template <int DIM, typename T>
-int run_fused_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
-    if (x->cfg.order < 2) return run_patch_t<DIM, T>(x, mode, is, in, ca, cb, stage0, iout);
+int run_fused_t(mst_ctx *x, int mode, int is, int in, const Stage &sg, int iout) {
+    if (x->cfg.order < 2) return run_patch_t<DIM, T>(x, mode, is, in, sg, iout);
     if (!x->fused_ok) return mst::fail(MST_E_UNSUPPORTED, "fused strategy: a patch's face states exceed 64 KiB of LDS");
     int rc = ensure_halo_geo(x);
     if (rc) return rc;
     const GradArgs<T> ga = grad_args<T>(x, is, false);
     const FluxArgs<T> fa = flux_args<T>(x, is);
-    const UpdArgs a = upd_args(x, mode, is, in, ca, cb, stage0, iout);
+    const UpdArgs a = upd_args(x, mode, is, in, sg, iout);
     PatchArgs pa;
     pa.start = x->pstart;
     pa.face = x->pface;
@@ -500,7 +492,7 @@ int run_fused_t(mst_ctx *x, int mode, int is, int in, double ca, double cb, int 
 int resolve_strategy(mst_ctx *x);
 
 // one residual evaluation + stage update (mode includes UPD_STAGE) with the context's strategy
-int run_stage(mst_ctx *x, int mode, int is, int in, double ca, double cb, int stage0, int iout) {
+int run_stage(mst_ctx *x, int mode, int is, int in, const Stage &sg, int iout) {
     if (x->resolved < 0) {
         int rc = resolve_strategy(x);
         if (rc) return rc;
@@ -508,17 +500,17 @@ int run_stage(mst_ctx *x, int mode, int is, int in, double ca, double cb, int st
     if (x->resolved == MST_STRATEGY_FACE || x->viscous) {  // the viscous kernels follow the flux kernel
         int rc = run_face_flux(x, is);
         if (rc) return rc;
-        return run_update(x, mode, is, in, ca, cb, stage0, iout);
+        return run_update(x, mode, is, in, sg, iout);
     }
     int rc = ensure_shadow(x, is);
     if (rc) return rc;
     if (x->resolved == MST_STRATEGY_FUSED) {
         if (x->mixed)
-            rc = x->dim == 2 ? run_fused_t<2, float>(x, mode, is, in, ca, cb, stage0, iout)
-                             : run_fused_t<3, float>(x, mode, is, in, ca, cb, stage0, iout);
+            rc = x->dim == 2 ? run_fused_t<2, float>(x, mode, is, in, sg, iout)
+                             : run_fused_t<3, float>(x, mode, is, in, sg, iout);
         else
-            rc = x->dim == 2 ? run_fused_t<2, double>(x, mode, is, in, ca, cb, stage0, iout)
-                             : run_fused_t<3, double>(x, mode, is, in, ca, cb, stage0, iout);
+            rc = x->dim == 2 ? run_fused_t<2, double>(x, mode, is, in, sg, iout)
+                             : run_fused_t<3, double>(x, mode, is, in, sg, iout);
         if (rc == MST_OK && iout >= 0) x->shadow_ok[iout] = x->mixed;
         return rc;
     }
@@ -528,17 +520,17 @@ int run_stage(mst_ctx *x, int mode, int is, int in, double ca, double cb, int st
     }
     if (x->resolved == MST_STRATEGY_PATCH) {
         if (x->mixed)
-            rc = x->dim == 2 ? run_patch_t<2, float>(x, mode, is, in, ca, cb, stage0, iout)
-                             : run_patch_t<3, float>(x, mode, is, in, ca, cb, stage0, iout);
+            rc = x->dim == 2 ? run_patch_t<2, float>(x, mode, is, in, sg, iout)
+                             : run_patch_t<3, float>(x, mode, is, in, sg, iout);
         else
-            rc = x->dim == 2 ? run_patch_t<2, double>(x, mode, is, in, ca, cb, stage0, iout)
-                             : run_patch_t<3, double>(x, mode, is, in, ca, cb, stage0, iout);
+            rc = x->dim == 2 ? run_patch_t<2, double>(x, mode, is, in, sg, iout)
+                             : run_patch_t<3, double>(x, mode, is, in, sg, iout);
     } else if (x->mixed) {
-        rc = x->dim == 2 ? run_cell_t<2, float>(x, mode, is, in, ca, cb, stage0, iout)
-                         : run_cell_t<3, float>(x, mode, is, in, ca, cb, stage0, iout);
+        rc = x->dim == 2 ? run_cell_t<2, float>(x, mode, is, in, sg, iout)
+                         : run_cell_t<3, float>(x, mode, is, in, sg, iout);
     } else {
-        rc = x->dim == 2 ? run_cell_t<2, double>(x, mode, is, in, ca, cb, stage0, iout)
-                         : run_cell_t<3, double>(x, mode, is, in, ca, cb, stage0, iout);
+        rc = x->dim == 2 ? run_cell_t<2, double>(x, mode, is, in, sg, iout)
+                         : run_cell_t<3, double>(x, mode, is, in, sg, iout);
     }
     if (rc == MST_OK && iout >= 0) x->shadow_ok[iout] = x->mixed;
     return rc;
@@ -988,7 +980,7 @@ int lu_iteration(mst_ctx *x, bool l2) {
     } else {
         rc = run_face_flux(x, x->iold);
         if (rc) return rc;
-        rc = run_update(x, mode, x->iold, -1, 0, 0, 1, -1);
+        rc = run_update(x, mode, x->iold, -1, Stage{0, 0}, -1);
     }
     if (rc) return rc;
     if (l2) {
@@ -1045,10 +1037,10 @@ int resolve_strategy(mst_ctx *x) {
         if ((s == MST_STRATEGY_PATCH || s == MST_STRATEGY_FUSED) && !x->patch_ok) continue;
         if (s == MST_STRATEGY_FUSED && !x->fused_ok) continue;
         x->resolved = s;
-        rc = run_stage(x, UPD_STAGE, x->iold, x->iold, 0.0, 1.0, 1, spare);  // warm-up
+        rc = run_stage(x, UPD_STAGE, x->iold, x->iold, Stage{}, spare);  // warm-up
         if (rc) break;
         if (hipEventRecord(e0, x->stream) != hipSuccess) break;
-        for (int r = 0; r < 3 && !rc; r++) rc = run_stage(x, UPD_STAGE, x->iold, x->iold, 0.0, 1.0, 1, spare);
+        for (int r = 0; r < 3 && !rc; r++) rc = run_stage(x, UPD_STAGE, x->iold, x->iold, Stage{}, spare);
         if (rc || hipEventRecord(e1, x->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) break;
         float ms = 0;
         if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best) {
@@ -1261,14 +1253,14 @@ int mst_face_flux(mst_ctx *ctx) {
 
 int mst_residual_update(mst_ctx *ctx) {
     CTX_GUARD(ctx);
-    int rc = run_update(ctx, UPD_STAGE | UPD_CHECK, ctx->iold, ctx->iold, 0.0, 1.0, 1, ctx->inew);
+    int rc = run_update(ctx, UPD_STAGE | UPD_CHECK, ctx->iold, ctx->iold, Stage{}, ctx->inew);
     if (rc) return rc;
     return check_bad(ctx);
 }
 
 int mst_solve(mst_ctx *ctx) {
     CTX_GUARD(ctx);
-    int rc = run_stage(ctx, UPD_STAGE | UPD_CHECK, ctx->iold, ctx->iold, 0.0, 1.0, 1, ctx->inew);
+    int rc = run_stage(ctx, UPD_STAGE | UPD_CHECK, ctx->iold, ctx->iold, Stage{}, ctx->inew);
     if (rc) return rc;
     return check_bad(ctx);
 }
@@ -1326,19 +1318,19 @@ int mst_step(mst_ctx *ctx, int nsteps) {
         // place of its FP64 state (the FP32 kernels read only the shadow), so no later stage reads Q^n
         // again (SSP-RK2: none; RK3: the middle stage).  The viscous kernels read the FP64 stage state.
         const bool pre = ctx->mixed && !ctx->viscous;
-        struct Reset {
-            mst_ctx *x;
-            ~Reset() { x->upd_pre = 0, x->upd_wa = x->upd_wb = 0; }
-        } reset{ctx};
         int src = ctx->iold;
         for (int s = 0; s < ns; s++) {
             const int dst = (s == ns - 1) ? ctx->inew : spare[s & 1];
             const int mode = UPD_STAGE | ((l2 && s == 0) ? UPD_L2 : 0) | ((cfl && s == ns - 1) ? UPD_DT : 0) |
                              (s == ns - 1 ? UPD_CHECK : 0);
-            ctx->upd_pre = pre;
-            ctx->upd_wa = pre && s < ns - 1 ? A[s + 1] : 0;
-            ctx->upd_wb = pre && s < ns - 1 ? B[s + 1] : 0;
-            rc = run_stage(ctx, mode, src, ctx->iold, A[s], B[s], s == 0 ? 1 : 0, dst);
+            Stage sg;
+            sg.ca = A[s];
+            sg.cb = B[s];
+            sg.stage0 = s == 0;
+            sg.pre = pre;
+            sg.wa = pre && s < ns - 1 ? A[s + 1] : 0;
+            sg.wb = pre && s < ns - 1 ? B[s + 1] : 0;
+            rc = run_stage(ctx, mode, src, ctx->iold, sg, dst);
             if (rc) return rc;
             src = dst;
         }
@@ -1553,7 +1545,7 @@ int mst_get_residual(mst_ctx *ctx, double *R, int64_t n_cells) {
     }
     int rc = run_face_flux(ctx, ctx->iold);
     if (rc) return rc;
-    rc = run_update(ctx, UPD_RESIDUAL, -1, -1, 0, 0, 0, -1);
+    rc = run_update(ctx, UPD_RESIDUAL, -1, -1, Stage{0, 0, 0}, -1);
     if (rc) return rc;
     return fetch_cells(ctx, ctx->R, ctx->nv, R);
 }
