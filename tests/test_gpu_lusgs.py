@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import mstcfd as M
-from helpers import cons, quad_grid, rel_state_error
+from helpers import cons, periodic_in_x, quad_grid, rel_state_error
 from mstcfd import ramp
 from oracle import pyoracle as O
 from test_oracle_lusgs import incidences
@@ -44,6 +44,13 @@ CASES = {
     "tet_roe2": (lambda: M.Mesh.gen_tet(5, 4, 4, (0, 0, 0), (1, 0.8, 0.7), 0.1, 5,
                                         (M.BC_INLET, M.BC_OUTLET, M.BC_SYMMETRY, M.BC_WALL, M.BC_SYMMETRY,
                                          M.BC_SYMMETRY)), M.ROE, 2, M.LIM_MINMOD),
+    # 3D Roe with the residual computed in the forward sweeps, and a 3D periodic box (outlet and
+    # periodic faces are where a per-slot loop over face kinds once compiled wrongly)
+    "tet_roe1": (lambda: M.Mesh.gen_tet(6, 5, 4, (0, 0, 0), (1, 0.8, 0.7), 0.15, 12,
+                                        (M.BC_INLET, M.BC_OUTLET, M.BC_SYMMETRY, M.BC_WALL, M.BC_SYMMETRY,
+                                         M.BC_SYMMETRY)), M.ROE, 1, M.LIM_NONE),
+    "tet_periodic_ausm1": (lambda: M.Mesh.from_raw(periodic_in_x(M.Mesh.gen_tet(
+        6, 5, 4, (0, 0, 0), (1, 0.8, 0.7), 0.0, 13, (M.BC_WALL,) * 6).raw())), M.AUSM, 1, M.LIM_NONE),
     # four-slot 2D cells (two colours)
     "quad_roe1": (lambda: M.Mesh.from_raw(quad_grid(24, bc=M.BC_WALL, h=0.05, jitter=0.2, seed=4)), M.ROE, 1,
                   M.LIM_NONE),
