@@ -178,3 +178,52 @@ def periodic_in_x(raw):
         out["partner"][i], out["partner"][j] = j, i
         out["ftype"][i] = out["ftype"][j] = 12
     return out
+
+
+def brick_grid(m, n, h=1.0, jitter=0.0, seed=0, bc=3):
+    """A running-bond brick wall: n rows of bricks 2h wide and h high, odd rows shifted by h (a half
+    brick at each end), so a full brick has six faces (two below, two above, one each side) and the
+    cell tables need eight slots.  Interior nodes move by up to jitter * h; boundary faces of Fluent
+    type bc.  The raw arrays of Mesh.from_raw, interior faces first."""
+    nid = lambda i, j: j * (2 * m + 1) + i
+    xy = np.array([(i, j) for j in range(n + 1) for i in range(2 * m + 1)], np.float64) * h
+    if jitter:
+        rng = np.random.default_rng(seed)
+        inner = np.array([0 < i < 2 * m and 0 < j < n for j in range(n + 1) for i in range(2 * m + 1)])
+        xy[inner] += rng.uniform(-jitter * h, jitter * h, (int(inner.sum()), 2))
+    first, count = [], 0  # the first cell of each row
+    for j in range(n):
+        first.append(count)
+        count += m if j % 2 == 0 else m + 1
+    cell = lambda x, j: first[j] + (x // 2 if j % 2 == 0 else (x + 1) // 2)  # the row-j cell over [x, x+1]
+    fn, c0, c1 = [], [], []
+    for j in range(n):  # vertical faces inside a row
+        for x in range(1, 2 * m):
+            if (x % 2 == 0) == (j % 2 == 0):
+                fn.append((nid(x, j), nid(x, j + 1)))
+                c0.append(cell(x - 1, j))
+                c1.append(cell(x, j))
+    for j in range(1, n):  # unit segments between rows j - 1 and j
+        for x in range(2 * m):
+            fn.append((nid(x, j), nid(x + 1, j)))
+            c0.append(cell(x, j - 1))
+            c1.append(cell(x, j))
+    nint = len(fn)
+    for x in range(2 * m):
+        fn.append((nid(x, 0), nid(x + 1, 0)))
+        c0.append(cell(x, 0))
+        c1.append(-1)
+        fn.append((nid(x, n), nid(x + 1, n)))
+        c0.append(cell(x, n - 1))
+        c1.append(-1)
+    for j in range(n):
+        fn.append((nid(0, j), nid(0, j + 1)))
+        c0.append(cell(0, j))
+        c1.append(-1)
+        fn.append((nid(2 * m, j), nid(2 * m, j + 1)))
+        c0.append(cell(2 * m - 1, j))
+        c1.append(-1)
+    nf = len(fn)
+    return dict(dim=2, xyz=xy, fnodes=np.array(fn, np.int32), c0=np.array(c0, np.int32), c1=np.array(c1, np.int32),
+                ftype=np.array([2] * nint + [bc] * (nf - nint), np.int32), partner=np.full(nf, -1, np.int32),
+                n_cells=count, n_int=nint)

@@ -7,8 +7,8 @@ import numpy as np
 import pytest
 
 import mstcfd as M
-from helpers import (quad_grid, cons, flip_faces, flux_scale, jumpy_state, normal_sums, periodic_in_x, rel_state_error,
-                     residual_parity_error, smooth_state, sod_state)
+from helpers import (brick_grid, quad_grid, cons, flip_faces, flux_scale, jumpy_state, normal_sums, periodic_in_x,
+                     rel_state_error, residual_parity_error, smooth_state, sod_state)
 from oracle import pyoracle as O
 
 pytestmark = pytest.mark.gpu
@@ -37,6 +37,8 @@ def mesh(name):
         elif name == "tet":
             m = M.Mesh.gen_tet(6, 5, 4, (0, 0, 0), (1, 0.8, 0.7), 0.15, 11,
                                (M.BC_INLET, M.BC_OUTLET, M.BC_SYMMETRY, M.BC_WALL, M.BC_SYMMETRY, M.BC_SYMMETRY))
+        elif name == "brick":  # six-faced cells: the 8-slot kernels
+            m = M.Mesh.from_raw(brick_grid(24, 20, h=0.05, jitter=0.1, seed=5, bc=M.BC_WALL))
         elif name == "tet_periodic":  # periodic in x, walls and symmetry planes around
             bc = (M.BC_WALL, M.BC_WALL, M.BC_SYMMETRY, M.BC_SYMMETRY, M.BC_WALL, M.BC_SYMMETRY)
             box = M.Mesh.gen_tet(6, 5, 4, (0, 0, 0), (1, 0.8, 0.7), 0.0, 11, bc)
@@ -100,6 +102,7 @@ CASES = [
     ("allbc", ROE, 2, M.LIM_VANLEER), ("periodic", ROE, 2, M.LIM_VANLEER), ("periodic", AUSM, 2, M.LIM_NONE),
     ("tet", AUSM, 1, 0), ("tet", ROE, 1, 0), ("tet", AUSM, 2, M.LIM_MINMOD), ("tet", ROE, 2, M.LIM_VANLEER),
     ("tet_periodic", ROE, 1, 0), ("tet_periodic", AUSM, 2, M.LIM_MINMOD),
+    ("brick", ROE, 1, 0), ("brick", AUSM, 2, M.LIM_VANLEER),
 ]
 
 
@@ -121,7 +124,8 @@ def test_residual_parity(mname, scheme, order, lim, kind):
 
 
 @pytest.mark.parametrize("mname,lim", [("allbc", M.LIM_MINMOD), ("periodic", M.LIM_VANLEER), ("tet", M.LIM_MINMOD),
-                                       ("strip_wall", M.LIM_NONE), ("tet_periodic", M.LIM_VANLEER)])
+                                       ("strip_wall", M.LIM_NONE), ("tet_periodic", M.LIM_VANLEER),
+                                       ("brick", M.LIM_MINMOD)])
 def test_gradient_and_limiter_parity(mname, lim):
     m, om, geo = mesh(mname)
     dim = m.dim
@@ -141,6 +145,19 @@ def test_gradient_and_limiter_parity(mname, lim):
 
 
 STRATEGIES = [M.STRATEGY_FACE, M.STRATEGY_CELL, M.STRATEGY_PATCH, M.STRATEGY_FUSED]
+
+
+def strategies_for(m, cg):
+    """The strategies a mesh takes: the patch kernels hold cells of at most four faces, and the library
+    refuses them for others (checked here)."""
+    raw = m.raw()
+    if np.bincount(np.concatenate([raw["c0"], raw["c1"][raw["c1"] >= 0]])).max() <= 4:
+        return STRATEGIES
+    s = M.Solver(m, cg)
+    for st in (M.STRATEGY_PATCH, M.STRATEGY_FUSED):
+        with pytest.raises(M.MstError, match="at most 4 faces per cell"):
+            s.strategy = st
+    return [M.STRATEGY_FACE, M.STRATEGY_CELL]
 
 
 @pytest.mark.parametrize("strategy", STRATEGIES)
@@ -232,7 +249,7 @@ def test_monitor_maxrel_matches_reference_rule():
 @pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_MIXED])
 @pytest.mark.parametrize("mname,scheme,lim", [("periodic", ROE, M.LIM_VANLEER), ("allbc", AUSM, M.LIM_MINMOD),
                                               ("tet", ROE, M.LIM_MINMOD), ("quad", AUSM, M.LIM_VANLEER),
-                                              ("tet_periodic", AUSM, M.LIM_MINMOD)])
+                                              ("tet_periodic", AUSM, M.LIM_MINMOD), ("brick", ROE, M.LIM_MINMOD)])
 def test_cell_strategy_matches_face_strategy(mname, scheme, lim, precision):
     """The fused cell kernel solves each interior face twice, the patch kernel each face of a patch
     once (a border face in both patches), always from the same memory inputs with the same
@@ -243,16 +260,17 @@ def test_cell_strategy_matches_face_strategy(mname, scheme, lim, precision):
                        precision=precision)
     Q = smooth_state(geo["ccen"], seed=6, mach=0.7)
     out = []
-    for strategy in STRATEGIES:
+    strats = strategies_for(m, cg)
+    for strategy in strats:
         s = M.Solver(m, cg)
         s.strategy = strategy
         assert s.strategy == strategy
         s.set_state(Q)
         s.step(20)
         out.append((s.get_state(), s.residual_norms()))
-    for k in range(1, len(STRATEGIES)):
+    for k in range(1, len(strats)):
         err = rel_state_error(out[k][0], out[0][0])
-        print(f"{mname} prec={precision}: strategy {STRATEGIES[k]} vs face after 20 RK3 steps {err:.2e}",
+        print(f"{mname} prec={precision}: strategy {strats[k]} vs face after 20 RK3 steps {err:.2e}",
               flush=True)
         assert np.array_equal(out[k][0], out[0][0])
         assert np.array_equal(out[k][1], out[0][1])
@@ -260,7 +278,7 @@ def test_cell_strategy_matches_face_strategy(mname, scheme, lim, precision):
 
 @pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_MIXED])
 @pytest.mark.parametrize("mname,scheme", [("periodic", ROE), ("allbc", AUSM), ("tet", ROE), ("quad", AUSM),
-                                          ("tet_periodic", ROE)])
+                                          ("tet_periodic", ROE), ("brick", AUSM)])
 def test_first_order_strategies_match(mname, scheme, precision):
     """Order 1: the patch kernel stages the cell states of its faces in LDS; every strategy still
     reproduces the face split bit for bit."""
@@ -269,14 +287,15 @@ def test_first_order_strategies_match(mname, scheme, precision):
     cg = M.make_config(dim=dim, scheme=scheme, order=1, rk=2, cfl=0.45, q_inlet=inlet(dim), precision=precision)
     Q = smooth_state(geo["ccen"], seed=8, mach=0.7)
     out = []
-    for strategy in STRATEGIES:
+    strats = strategies_for(m, cg)
+    for strategy in strats:
         s = M.Solver(m, cg)
         s.strategy = strategy
         s.set_state(Q)
         s.step(20)
         out.append((s.get_state(), s.residual_norms()))
-    for k in range(1, len(STRATEGIES)):
-        assert np.array_equal(out[k][0], out[0][0]), (mname, precision, STRATEGIES[k])
+    for k in range(1, len(strats)):
+        assert np.array_equal(out[k][0], out[0][0]), (mname, precision, strats[k])
         assert np.array_equal(out[k][1], out[0][1])
 
 
