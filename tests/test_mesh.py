@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import mstcfd as M
-from helpers import quad_grid
+from helpers import brick_grid, periodic_in_x, quad_grid
 from oracle import pyoracle as O
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -266,3 +266,35 @@ def test_quad_cells_geometry_matches_oracle():
     # (mesh/Cell.cpp:28-49); the device context refuses such a mesh instead of producing NaNs
     bad = ~(g["vol"] > 0)
     assert bad.sum() == 20 and np.isnan(g["vol"][bad]).all()
+
+
+def test_brick_and_periodic_box_meshes():
+    """The test meshes built from raw arrays: a running-bond brick wall (six-faced cells) and a tet box
+    made periodic in x.  Geometry bit-identical to the oracle's; the cells tile the domain; every cell's
+    outward normals close; the periodic partners pair faces of equal area."""
+    for raw, area in ((brick_grid(10, 7, h=0.1, jitter=0.1, seed=2), 2.0 * 0.7),
+                      (periodic_in_x(M.Mesh.gen_tet(5, 4, 3, (0, 0, 0), (1, 0.8, 0.7), 0.0, 3, (M.BC_WALL,) * 6).raw()),
+                       1 * 0.8 * 0.7)):
+        m = M.Mesh.from_raw(raw)
+        g = M.Prep(m).geometry()
+        go = O.OracleMesh(raw).geometry()
+        for k in g:
+            assert np.array_equal(g[k], go[k]), k
+        assert abs(g["vol"].sum() - area) <= 1e-13 * area
+        out = np.zeros((m.n_cells, int(raw["dim"])))
+        sgn = g["dac"][:, None] * g["S"]  # the c0 outward normal
+        np.add.at(out, raw["c0"], sgn)
+        inner = raw["c1"] >= 0
+        np.add.at(out, raw["c1"][inner], -sgn[inner])
+        per = raw["ftype"] == 12
+        # periodic faces close their c0 cell's surface with their own normal
+        assert np.abs(out).max() <= 1e-14 * np.abs(g["S"]).max()
+        if per.any():
+            pa = raw["partner"][per]
+            area_f = np.linalg.norm(g["S"], axis=1)
+            assert np.allclose(area_f[per], area_f[pa], rtol=1e-14)
+    nfaces = np.bincount(np.concatenate([raw["c0"], raw["c1"][raw["c1"] >= 0]]))
+    assert nfaces.max() == 4  # (the tet box)
+    b = brick_grid(10, 7)
+    nb = np.bincount(np.concatenate([b["c0"], b["c1"][b["c1"] >= 0]]))
+    assert sorted(set(nb.tolist())) == [4, 6]  # half bricks at the row ends, six-faced bricks
