@@ -102,20 +102,13 @@ __device__ __forceinline__ void lu_flux(const double *q, const double *n, double
 // RSCHEME >= 0 (forward, order 1, cells of <= 4 faces): R of the cell is not read but computed here,
 // from the states the sweep loads anyway: the same face solves (face_solve, the c0 outward normal,
 // the c0 side first) summed in slot order as the gather kernels sum them, so R is the same bit for bit.
-template <int DIM, bool FWD, bool LAST, int MAXF, int RSCHEME = LU_R_ARRAY>
-__global__ __launch_bounds__(BLOCK) void k_lu_sweep(LuArgs a, int64_t t0, int64_t n) {
+// One cell's sweep (position t, cell c).  qown (forward only): the cell's state, already in registers,
+// in place of a.Q[c]; qnew (backward or LAST): receives the new state the sweep also stores in Qout.
+template <int DIM, bool FWD, bool LAST, int MAXF, int RSCHEME>
+__device__ __forceinline__ void lu_cell(const LuArgs &a, const int64_t t, const int64_t c, const double *qown,
+                                        double *qnew) {
     static_assert(RSCHEME == LU_R_ARRAY || (FWD && MAXF <= 4), "computed residual: forward sweeps of small cells");
     constexpr int NV = DIM + 2;
-    const int64_t l = block_id() * BLOCK + threadIdx.x;
-    if (l >= n) return;
-    const int64_t t = t0 + l;
-    if (a.pcase) {  // batch: this position's case
-        const int32_t cs = a.pcase[t];
-        if (!a.active[cs]) return;
-        a.gamma = a.gam[cs];
-        a.bad += cs;
-    }
-    const int64_t c = a.cells[t];
     double s[NV], qc[NV];
     if constexpr (FWD) {
         if constexpr (RSCHEME == LU_R_ARRAY) {
@@ -123,7 +116,12 @@ __global__ __launch_bounds__(BLOCK) void k_lu_sweep(LuArgs a, int64_t t0, int64_
 #pragma unroll
             for (int k = 0; k < NV; k++) s[k] = -s[k];
         }
-        ldq<NV>(a.Q + c * NV, qc);
+        if (qown) {
+#pragma unroll
+            for (int k = 0; k < NV; k++) qc[k] = qown[k];
+        } else {
+            ldq<NV>(a.Q + c * NV, qc);
+        }
     } else {
 #pragma unroll
         for (int k = 0; k < NV; k++) s[k] = 0;
@@ -277,7 +275,42 @@ __global__ __launch_bounds__(BLOCK) void k_lu_sweep(LuArgs a, int64_t t0, int64_
         stq<NV>(a.Qout + c * NV, out);
         const double p = lu_pressure<DIM>(out, a.gamma);
         if (!(out[0] > 0) || !(p > 0)) a.bad[0] = 1;
+        if (qnew)
+#pragma unroll
+            for (int k = 0; k < NV; k++) qnew[k] = out[k];
     }
+}
+
+template <int DIM, bool FWD, bool LAST, int MAXF, int RSCHEME = LU_R_ARRAY>
+__global__ __launch_bounds__(BLOCK) void k_lu_sweep(LuArgs a, int64_t t0, int64_t n) {
+    const int64_t l = block_id() * BLOCK + threadIdx.x;
+    if (l >= n) return;
+    const int64_t t = t0 + l;
+    if (a.pcase) {  // batch: this position's case
+        const int32_t cs = a.pcase[t];
+        if (!a.active[cs]) return;
+        a.gamma = a.gam[cs];
+        a.bad += cs;
+    }
+    lu_cell<DIM, FWD, LAST, MAXF, RSCHEME>(a, t, a.cells[t], nullptr, nullptr);
+}
+
+// Colour 0's backward sweep of one iteration fused with its forward sweep of the next (single
+// context, first order, the residual computed in the forward sweeps, two colours or more).  Colour 0
+// has no lower neighbours, so its forward sweep reads only the cell's new state (in registers here)
+// and its neighbours' new states, which the iteration's other colours have finished; no other cell of
+// the launch reads what either part writes.  The same code as the two separate launches: the same bits.
+template <int DIM, int MAXF, int RSCHEME>
+__global__ __launch_bounds__(BLOCK) void k_lu_sweep_bf(LuArgs a, int64_t t0, int64_t n) {
+    constexpr int NV = DIM + 2;
+    const int64_t l = block_id() * BLOCK + threadIdx.x;
+    if (l >= n) return;
+    const int64_t t = t0 + l, c = a.cells[t];
+    double q1[NV];
+    lu_cell<DIM, false, false, MAXF, LU_R_ARRAY>(a, t, c, nullptr, q1);
+    LuArgs b = a;
+    b.Q = a.Qout;  // the next iteration starts from the new state
+    lu_cell<DIM, true, false, MAXF, RSCHEME>(b, t, c, q1, nullptr);
 }
 
 }  // namespace mstk

@@ -961,6 +961,30 @@ int lu_sweeps(mst_ctx *x, const LuArgs &a, bool inline_res) {
     return rc;
 }
 
+// the sweeps' arguments for the current Old (the iteration's start) and New buffers
+LuArgs lu_args(const mst_ctx *x) {
+    LuArgs a;
+    memset(&a, 0, sizeof(a));  // the batch-only fields (pcase, gam, active, G) stay null: single context
+    a.nc = x->nc;
+    a.cells = x->lu_cells;
+    a.nb = x->lu_nb;
+    a.n = x->lu_n;
+    a.r = x->lu_r;
+    a.D = x->lu_D;
+    a.Q = x->Q[x->iold];
+    a.R = x->R;
+    a.dQ = x->lu_dQ;
+    a.Qout = x->Q[x->inew];
+    a.bad = x->bad;
+    a.gamma = x->cfg.gamma;
+    a.omega = x->lu_omega;
+    a.dscale = 1.0 / x->lu_cfl + 0.5 * x->lu_omega;
+    a.face = x->lu_face;
+    a.finfo = x->finfo;
+    a.P = x->p64;
+    return a;
+}
+
 // one LU-SGS iteration: residual of Old (+ its L2 norms when l2 is set), the two sweeps into New
 int lu_iteration(mst_ctx *x, bool l2) {
     int rc = MST_OK;
@@ -987,25 +1011,7 @@ int lu_iteration(mst_ctx *x, bool l2) {
         rc = run_finish(x, x->nv, 0);
         if (rc) return rc;
     }
-    LuArgs a;
-    memset(&a, 0, sizeof(a));  // the batch-only fields (pcase, gam, active, G) stay null: single context
-    a.nc = x->nc;
-    a.cells = x->lu_cells;
-    a.nb = x->lu_nb;
-    a.n = x->lu_n;
-    a.r = x->lu_r;
-    a.D = x->lu_D;
-    a.Q = x->Q[x->iold];
-    a.R = x->R;
-    a.dQ = x->lu_dQ;
-    a.Qout = x->Q[x->inew];
-    a.bad = x->bad;
-    a.gamma = x->cfg.gamma;
-    a.omega = x->lu_omega;
-    a.dscale = 1.0 / x->lu_cfl + 0.5 * x->lu_omega;
-    a.face = x->lu_face;
-    a.finfo = x->finfo;
-    a.P = x->p64;
+    const LuArgs a = lu_args(x);
     // the records are padded to the slot count of the device table (3, 4 or 8)
     if (x->dim == 2) {
         if (x->slots == 3) return lu_sweeps<2, 3>(x, a, inline_res);
@@ -1016,6 +1022,47 @@ int lu_iteration(mst_ctx *x, bool l2) {
     if (x->slots == 4) return lu_sweeps<3, 4>(x, a, inline_res);
     return lu_sweeps<3, 8>(x, a, inline_res);
 }
+
+// n >= 1 consecutive iterations with the residual computed in the forward sweeps (lu_inline, two
+// colours or more): colour 0's backward sweep of each iteration but the last runs fused with the next
+// iteration's forward sweep of colour 0 (k_lu_sweep_bf), 2 K - 2 launches per iteration for K colours
+// instead of 2 K - 1, with the same results.  Each iteration ends with the Old/New swap.
+template <int DIM, int MAXF, int RS>
+int lu_inline_run(mst_ctx *x, int n) {
+    const int ncol = (int)x->lu_start.size() - 1;
+    const int64_t n0 = x->lu_start[1] - x->lu_start[0];
+    int rc = MST_OK;
+    for (int it = 0; it < n && rc == MST_OK; it++) {
+        const LuArgs a = lu_args(x);
+        for (int k = it == 0 ? 0 : 1; k < ncol && rc == MST_OK; k++) {  // colour 0: done by the fused launch
+            const int64_t t0 = x->lu_start[k], m = x->lu_start[k + 1] - t0;
+            rc = k == ncol - 1 ? launch(x, MST_K_LUSGS, k_lu_sweep<DIM, true, true, MAXF, RS>, m, a, t0, m)
+                               : launch(x, MST_K_LUSGS, k_lu_sweep<DIM, true, false, MAXF, RS>, m, a, t0, m);
+        }
+        for (int k = ncol - 2; k >= 1 && rc == MST_OK; k--) {
+            const int64_t t0 = x->lu_start[k], m = x->lu_start[k + 1] - t0;
+            rc = launch(x, MST_K_LUSGS, k_lu_sweep<DIM, false, false, MAXF>, m, a, t0, m);
+        }
+        if (rc) break;
+        rc = it < n - 1 ? launch(x, MST_K_LUSGS, k_lu_sweep_bf<DIM, MAXF, RS>, n0, a, x->lu_start[0], n0)
+                        : launch(x, MST_K_LUSGS, k_lu_sweep<DIM, false, false, MAXF>, n0, a, x->lu_start[0], n0);
+        x->shadow_ok[x->inew] = false;
+        std::swap(x->iold, x->inew);
+    }
+    return rc;
+}
+
+template <int DIM, int MAXF>
+int lu_inline_run_s(mst_ctx *x, int n) {
+    return x->cfg.scheme == MST_SCHEME_ROE ? lu_inline_run<DIM, MAXF, MST_SCHEME_ROE>(x, n)
+                                           : lu_inline_run<DIM, MAXF, MST_SCHEME_AUSM>(x, n);
+}
+
+int lu_inline_iterations(mst_ctx *x, int n) {
+    if (x->dim == 2) return x->slots == 3 ? lu_inline_run_s<2, 3>(x, n) : lu_inline_run_s<2, 4>(x, n);
+    return x->slots == 3 ? lu_inline_run_s<3, 3>(x, n) : lu_inline_run_s<3, 4>(x, n);
+}
+
 
 // AUTO: time one stage of each strategy on this mesh and configuration (outputs go to a spare state
 // buffer; the Old and New states are untouched) and keep the faster
@@ -1397,11 +1444,21 @@ int mst_lusgs_step(mst_ctx *ctx, int n_iter) {
     if (n_iter < 0) return mst::fail(MST_E_ARG, "n_iter must be >= 0");
     int rc = build_colours(ctx);
     if (rc) return rc;
-    for (int it = 0; it < n_iter; it++) {
+    ctx->lu_inline = ctx->cfg.order == 1 && !ctx->mixed && !ctx->viscous && ctx->slots <= 4;
+    const bool fuse = ctx->lu_inline && ctx->lu_start.size() >= 3;  // two colours or more
+    for (int it = 0; it < n_iter;) {
+        if (fuse && it < n_iter - 1) {  // the iterations before the call's last, colour 0's sweeps fused
+            const int m = n_iter - 1 - it;
+            rc = lu_inline_iterations(ctx, m);
+            if (rc) return rc;
+            it += m;
+            continue;
+        }
         rc = lu_iteration(ctx, it == n_iter - 1);  // norms of the call's last iteration (a monitor)
         if (rc) return rc;
         ctx->shadow_ok[ctx->inew] = false;
         std::swap(ctx->iold, ctx->inew);
+        it++;
     }
     if (n_iter > 0) ctx->dt_valid = false;
     return check_bad(ctx);
@@ -1767,7 +1824,10 @@ int mst_algorithmic_bytes(const mst_ctx *ctx, int kernel, double *bytes) {
         }
         case MST_K_LUSGS: {  // one iteration's sweeps, each array once per sweep, over their launches
             if (ctx->lu_start.size() < 2) return mst::fail(MST_E_ARG, "no LU-SGS iteration has run");
-            const double S = ctx->slots, launches = 2.0 * (double)(ctx->lu_start.size() - 1) - 1;
+            // 2 K - 1 launches for K colours; 2 K - 2 where colour 0's sweeps of consecutive
+            // iterations run fused (the iterations of a call before its last)
+            const double K = (double)(ctx->lu_start.size() - 1);
+            const double S = ctx->slots, launches = (ctx->lu_inline && K >= 2) ? 2.0 * K - 2 : 2.0 * K - 1;
             // forward: positions, residual, states, neighbour updates, update out, D out; per slot the
             // record (neighbour code, normal) and r out.  backward: positions, D, states, updates in
             // and out, new state; per slot the record and r.

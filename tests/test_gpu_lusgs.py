@@ -94,7 +94,7 @@ def test_lusgs_iterations_match_oracle(name):
         s.set_state(Q0)
         s.set_steady_driver(M.STEADY_LUSGS, cfl, omega)
         Qr = Q0.copy()
-        for it in range(3):
+        for it in range(4):
             s.lusgs_step(1)
             Qr, res = om.lusgs_step(ocfg, Qr, col, cfl, omega)
             Qg = s.get_state()
@@ -102,11 +102,12 @@ def test_lusgs_iterations_match_oracle(name):
             assert err <= TOL, (name, cfl, it, err)
             rg = s.residual_norms()
             assert np.allclose(rg, res, rtol=1e-12, atol=0), (name, it, rg, res)
-        # one call of three iterations: only its last one reduces norms, and at order 1 the others
-        # compute R inside the forward sweep; the state must equal the one-by-one run bit for bit
+        # one call of four iterations: only its last one reduces norms, and at order 1 the others
+        # compute R inside the forward sweep, colour 0's backward sweep of each fused with the next
+        # one's forward sweep; the state must equal the one-by-one run bit for bit
         Qg = s.get_state()
         s.set_state(Q0)
-        s.lusgs_step(3)
+        s.lusgs_step(4)
         assert np.array_equal(s.get_state(), Qg), name
         assert np.allclose(s.residual_norms(), res, rtol=1e-12, atol=0)
     s.close()
